@@ -1,0 +1,167 @@
+/*
+ * mlxlearn_amd -- C ABI of the MI355X (gfx950) exact neighbour-search library.
+ *
+ * Every entry point returns an int status: 0 = ok, > 0 = invalid input (the
+ * Python layer raises ValueError), < 0 = device / allocation failure
+ * (BackendExecutionError / MemoryError). No C++ exception crosses the ABI and
+ * no partial output is promised on error. mlk_last_error() returns the
+ * calling thread's last message.
+ *
+ * Pointers named *_dev are device (HBM) pointers; outputs are caller-owned.
+ * `stream` is a hipStream_t (NULL = the default stream). One index may be used
+ * from several host threads: calls on one index serialise on its lock.
+ *
+ * Exactness contract (all routes, all precisions): k-NN rows are ordered by the
+ * exact real squared distance S(q, x) of the float32 coordinates, lower
+ * training index first on exact ties; radius membership is exact S <= R2 with
+ * R2 = fl64(r*r); k-means labels are the exact nearest centre, lowest index on
+ * ties. Reported distances are sqrt of the canonical float64 sum
+ * sum_k (double(q_k) - double(x_k))**2 evaluated left to right.
+ */
+#ifndef MLXLEARN_AMD_H
+#define MLXLEARN_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mlk_index mlk_index;
+typedef struct mlk_radius_result mlk_radius_result;
+
+/* status codes */
+#define MLK_OK 0
+#define MLK_EINVAL 1
+#define MLK_ECAPACITY 2
+#define MLK_ENOMEM (-1)
+#define MLK_EDEVICE (-2)
+
+/* query flags */
+#define MLK_EXCLUDE_SELF 0x1u     /* query row i is training row q_offset + i; drop it   */
+#define MLK_PREC_FP32 0x00u       /* screen precision (bits 4-5)                          */
+#define MLK_PREC_BF16 0x10u
+#define MLK_PREC_FP16 0x20u
+#define MLK_PREC_MASK 0x30u
+#define MLK_SORT_RESULTS 0x100u   /* radius: order rows by (distance, index)              */
+#define MLK_NEED_DISTANCE 0x200u  /* radius: keep distances                               */
+#define MLK_SQUARED_DIST 0x400u   /* k-NN: dist_dev receives squared distances s           */
+
+/* weights for mlk_knn_votes */
+#define MLK_WEIGHTS_UNIFORM 0
+#define MLK_WEIGHTS_DISTANCE 1
+
+/* test hooks (mlk_set_test_hooks) */
+#define MLK_HOOK_FORCE_FALLBACK 0x1u   /* every query takes the exact fallback route        */
+#define MLK_HOOK_CAPACITY 0x2u         /* candidate capacity per (query, split) = param     */
+#define MLK_HOOK_FAIL 0x4u             /* the next library call fails with MLK_EDEVICE      */
+#define MLK_HOOK_FORCE_UNCERTIFIED 0x8u /* rows listed by mlk_set_forced_rows lose certificates */
+
+/* Per-call statistics of the last query on this thread. */
+typedef struct {
+    int64_t queries;
+    int64_t candidates;        /* screen candidates appended over all rounds       */
+    int64_t reranked;          /* candidates that reached the float64 exact rank    */
+    int64_t fallback_queries;  /* queries resolved by the exact fallback scan       */
+    int64_t rounds;            /* screen rounds launched                            */
+    int64_t splits;            /* database splits per round                         */
+    int64_t capacity;          /* candidate slots per (query, split)                */
+    int64_t screen_precision;  /* MLK_PREC_* actually used                          */
+    double screen_ms;          /* device time of the screen kernels (events)        */
+    double total_ms;           /* device time of the whole query (events)           */
+} mlk_stats;
+
+const char* mlk_last_error(void);
+int mlk_version(void);
+int mlk_get_stats(mlk_stats* out);
+int mlk_set_test_hooks(uint32_t hooks, int64_t param);
+/* Rows (query positions within the next call) whose screen certificate is withdrawn. */
+int mlk_set_forced_rows(const int64_t* rows_host, int64_t count);
+
+/* Tuning overrides (0 = automatic): rows of database splits, rounds schedule. */
+int mlk_set_tuning(int32_t splits, int32_t capacity, int32_t query_tiles, int32_t n_rounds,
+                   const double* round_fractions);
+
+/*
+ * Exact brute-force index over X (n x d float32, row-major, device memory).
+ * The index owns a copy of X plus its centre, float64 row norms and the
+ * screen mirrors. Replaces DeviceNeighborIndex / fit_neighbors
+ * (src/mlxlearn_device/_neighbors.py:243-270, 447-466) and
+ * build_index (src/mlxlearn/_kernels/distance.py:366-392).
+ */
+int mlk_index_create(const float* X_dev, int64_t n, int32_t d, uint32_t flags, void* stream,
+                     mlk_index** out);
+int mlk_index_destroy(mlk_index* index);
+int mlk_index_info(const mlk_index* index, int64_t* n, int32_t* d, int32_t* integral);
+
+/*
+ * Exact k nearest neighbours of Q (m x d float32, device). Writes idx_dev
+ * (m x k int64) and, when dist_dev is not NULL, dist_dev (m x k float64,
+ * Euclidean). Replaces DeviceNeighborIndex.query
+ * (src/mlxlearn_device/_neighbors.py:271-443) behind NeighborIndex.query /
+ * query_device (src/mlxlearn/_kernels/distance.py:211-270).
+ */
+int mlk_knn_query(mlk_index* index, const float* Q_dev, int64_t m, int32_t k, int64_t q_offset,
+                  uint32_t flags, double* dist_dev, int64_t* idx_dev, void* stream);
+
+/*
+ * Several query batches (concatenated rows; batch b is rows
+ * [offsets[b], offsets[b+1])) answered by one screen launch per round.
+ * Results are identical to one mlk_knn_query per batch.
+ */
+int mlk_knn_query_batched(mlk_index* index, const float* Q_dev, const int64_t* offsets_host,
+                          int32_t n_batches, int32_t k, uint32_t flags, double* dist_dev,
+                          int64_t* idx_dev, void* stream);
+
+/*
+ * Exact radius neighbourhoods: S(q, x) <= R2. Two phases: mlk_radius_query
+ * computes membership and fills indptr_dev (m+1 int64, CSR row pointers) and
+ * *nnz; mlk_radius_fill copies indices (int64) and optional distances
+ * (float64, sqrt(min(s, R2))) into caller buffers of nnz entries; then
+ * mlk_radius_free. Rows are in ascending index order, or (distance, index)
+ * with MLK_SORT_RESULTS. Replaces NeighborIndex.query_radius_csr
+ * (src/mlxlearn/_kernels/distance.py:285-336) and the Metal route's device
+ * screen + host finisher (radius_metal_v1.py:257-611; _radius_cpu_native.cpp:129-200).
+ */
+int mlk_radius_query(mlk_index* index, const float* Q_dev, int64_t m, double R2, int64_t q_offset,
+                     uint32_t flags, int64_t* indptr_dev, int64_t* nnz, mlk_radius_result** out,
+                     void* stream);
+int mlk_radius_fill(mlk_radius_result* result, int64_t* indices_dev, double* dist_dev, void* stream);
+int mlk_radius_free(mlk_radius_result* result);
+
+/*
+ * k-means assignment: exact nearest centre of every row of X (n x d) among
+ * C (kc x d), lowest index on ties; residual_dev gets the canonical float64
+ * squared distance. Replaces KMeans._assignment / e_means.assign
+ * (src/mlxlearn/cluster/_k_means.py:326-349; src/mlxlearn/_kernels/e_means.py:245-345).
+ */
+int mlk_kmeans_assign(const float* X_dev, int64_t n, int32_t d, const float* C_dev, int64_t kc,
+                      uint32_t flags, int64_t* labels_dev, double* residual_dev, void* stream);
+
+/*
+ * k-means update: float64 per-cluster coordinate sums (kc x d), counts (kc)
+ * and inertia (sum of residuals), deterministic (fixed summation order).
+ * Replaces KMeans._statistics / e_means.batch_stats
+ * (src/mlxlearn/cluster/_k_means.py:351-361; src/mlxlearn/_kernels/e_means.py:371-382).
+ */
+int mlk_kmeans_update(const float* X_dev, int64_t n, int32_t d, const int64_t* labels_dev,
+                      int64_t kc, const double* residual_dev, double* sums_dev, int64_t* counts_dev,
+                      double* inertia_dev, void* stream);
+
+/*
+ * Class votes of k neighbours: pred_dev (m) = class code with the largest
+ * vote, smallest code on ties; proba_dev (m x n_classes, optional) = votes
+ * normalised per row. Distance weights follow sklearn (1/d; rows with a zero
+ * distance use the indicator of zero distances). Replaces
+ * class_predictions / class_probabilities (src/mlxlearn_device/_neighbors_votes.py:188-244)
+ * and the CPU votes of KNeighborsClassifier (src/mlxlearn/neighbors/_classification.py:141-154).
+ */
+int mlk_knn_votes(const int64_t* idx_dev, const double* dist_dev, int64_t m, int32_t k,
+                  const int32_t* codes_dev, int32_t n_classes, int32_t weights, int64_t* pred_dev,
+                  double* proba_dev, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MLXLEARN_AMD_H */

@@ -1,0 +1,304 @@
+// Index construction: owned float32 copy, centre, per-precision screen
+// mirrors with float64 norms, safe-range flags and Gram biases.
+//
+// Reference counterpart: DeviceNeighborIndex preparation
+// (src/mlxlearn_device/_neighbors.py:232-239 _prepare_gram_arrays: anchored
+// mean centre, centred train, float32 norms). Here the centre is rounded to an
+// integer for integer data (so centred integer data stay exact in the screen
+// types), norms are float64, and each row carries a safe-range flag instead of
+// the reference's "unsafe -> lower bound 0" rule (_neighbors_general.py:188-223).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "mlk_device.h"
+#include "mlk_internal.h"
+
+namespace mlk {
+
+int DevBuf::reserve(size_t need) {
+    if (need <= bytes && ptr) return MLK_OK;
+    release();
+    size_t want = need < 256 ? 256 : need;
+    hipError_t e = hipMalloc(&ptr, want);
+    if (e != hipSuccess) {
+        ptr = nullptr;
+        bytes = 0;
+        (void)hipGetLastError();
+        set_error(std::string("device allocation of ") + std::to_string(want) + " bytes failed: " +
+                  hipGetErrorString(e));
+        return MLK_ENOMEM;
+    }
+    bytes = want;
+    return MLK_OK;
+}
+
+void DevBuf::release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+}
+
+// Per-block float64 column sums (rows [b*rows_per_block, ...)) and an
+// "is every value an integer" flag.
+__global__ void k_colsum(const float* __restrict__ X, int64_t n, int d, int64_t rows_per_block,
+                         double* __restrict__ partial, unsigned* __restrict__ nonint) {
+    const int64_t r0 = (int64_t)blockIdx.x * rows_per_block;
+    const int64_t r1 = r0 + rows_per_block < n ? r0 + rows_per_block : n;
+    bool frac = false;
+    for (int c = threadIdx.x; c < d; c += blockDim.x) {
+        double s = 0.0;
+        for (int64_t r = r0; r < r1; ++r) {
+            float v = X[r * d + c];
+            s += (double)v;
+            frac |= rintf(v) != v;
+        }
+        partial[(size_t)blockIdx.x * d + c] = s;
+    }
+    if (frac) atomicOr(nonint, 1u);
+}
+
+// max |x - c| over the matrix (non-negative float bits order like integers).
+__global__ void k_maxabs(const float* __restrict__ X, int64_t total, int d, const float* __restrict__ c,
+                         unsigned* __restrict__ out) {
+    float m = 0.0f;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        double v = fabs((double)X[i] - (double)c[i % d]);
+        m = fmaxf(m, (float)v);
+    }
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(m));
+}
+
+template <int PREC> struct ScreenType;
+template <> struct ScreenType<PREC_FP32> { typedef float T; };
+template <> struct ScreenType<PREC_BF16> { typedef __bf16 T; };
+template <> struct ScreenType<PREC_FP16> { typedef _Float16 T; };
+
+// Safe range of the scaled centred value for each screen type: zero, or a
+// magnitude whose rounding stays normal and whose products with any other safe
+// value are normal float32 numbers.
+template <int PREC>
+__device__ __forceinline__ bool safe_value(double v) {
+    double a = fabs(v);
+    if (a == 0.0) return true;
+    if (PREC == PREC_FP16) return a >= 0x1p-14 && a <= 0x1p15;
+    return a >= 0x1p-60 && a <= 0x1p56;
+}
+
+template <int PREC>
+__device__ __forceinline__ typename ScreenType<PREC>::T to_screen(double v) {
+    // float64 -> float32 (RN) -> screen type (RN); the double rounding is
+    // covered by the screen's error constant.
+    return (typename ScreenType<PREC>::T)(float)v;
+}
+
+// One wave per row: screen mirror, float64 norm, safe flag, exactness flag.
+// Used for database rows (flags_out per row) and for query rows.
+template <int PREC>
+__global__ void k_mirror(const float* __restrict__ X, int64_t n, int64_t n_pad, int d, int dp,
+                         const float* __restrict__ c, double scale, typename ScreenType<PREC>::T* __restrict__ xs,
+                         double* __restrict__ norms, uint8_t* __restrict__ flags, unsigned* __restrict__ inexact) {
+    typedef typename ScreenType<PREC>::T T;
+    const int lane = threadIdx.x & 63;
+    const int64_t row = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (row >= n_pad) return;
+    double s = 0.0;
+    bool safe = true, exact = true;
+    for (int j = lane; j < dp; j += 64) {
+        T out = (T)0.0f;
+        if (row < n && j < d) {
+            double v = ((double)X[row * d + j] - (double)c[j]) * scale;
+            s += v * v;
+            safe &= safe_value<PREC>(v);
+            out = to_screen<PREC>(v);
+            exact &= (double)(float)out == v;
+        }
+        xs[row * dp + j] = out;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        s += __shfl_xor(s, o);
+        safe = __shfl_xor((int)safe, o) & safe;
+        exact = __shfl_xor((int)exact, o) & exact;
+    }
+    if (lane == 0) {
+        norms[row] = row < n ? s : 0.0;
+        flags[row] = (row < n && safe) ? 1 : 0;
+        if (row < n && safe && !exact) atomicOr(inexact, 1u);
+    }
+}
+
+// Bias a_x = round_down((1 - G) * nx / 2); unsafe rows get a bias that no
+// threshold reaches.
+__global__ void k_bias(const double* __restrict__ nx, const uint8_t* __restrict__ flags, int64_t n, double G,
+                       float* __restrict__ bias) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    bias[i] = flags[i] ? __double2float_rd((1.0 - G) * nx[i] * 0.5) : 3.0e38f;
+}
+
+__global__ void k_unsafe_list(const uint8_t* __restrict__ flags, int64_t n, int* __restrict__ list,
+                              unsigned* __restrict__ count) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && !flags[i]) {
+        unsigned at = atomicAdd(count, 1u);
+        list[at] = (int)i;
+    }
+}
+
+template <int PREC>
+static void launch_mirror(const float* X, int64_t n, int64_t n_pad, int d, int dp, const float* c,
+                          double scale, void* xs, double* norms, uint8_t* flags, unsigned* inexact,
+                          hipStream_t stream) {
+    const int rows_per_block = 4;
+    dim3 grid((unsigned)((n_pad + rows_per_block - 1) / rows_per_block));
+    hipLaunchKernelGGL(k_mirror<PREC>, grid, dim3(256), 0, stream, X, n, n_pad, d, dp, c, scale,
+                       (typename ScreenType<PREC>::T*)xs, norms, flags, inexact);
+}
+
+int mirror_rows(int prec, const float* X, int64_t n, int64_t n_pad, int d, int dp, const float* c,
+                double scale, void* xs, double* norms, uint8_t* flags, unsigned* inexact,
+                hipStream_t stream) {
+    if (n_pad == 0) return MLK_OK;
+    switch (prec) {
+        case PREC_FP32: launch_mirror<PREC_FP32>(X, n, n_pad, d, dp, c, scale, xs, norms, flags, inexact, stream); break;
+        case PREC_BF16: launch_mirror<PREC_BF16>(X, n, n_pad, d, dp, c, scale, xs, norms, flags, inexact, stream); break;
+        default: launch_mirror<PREC_FP16>(X, n, n_pad, d, dp, c, scale, xs, norms, flags, inexact, stream); break;
+    }
+    MLK_CHECK_HIP(hipGetLastError());
+    return MLK_OK;
+}
+
+int pad32(int d) { return (d + 31) / 32 * 32; }
+
+int prepare_mirror(mlk_index* index, int prec, hipStream_t stream) {
+    Mirror& mr = index->mirror[prec];
+    if (mr.ready) return MLK_OK;
+    const int64_t n = index->n;
+    const int d = index->d;
+    mr.dp = pad32(d);
+    double scale = 1.0;
+    DevBuf tmp;
+    MLK_TRY(tmp.reserve(16));
+    unsigned* dflag = tmp.as<unsigned>();
+    if (prec == PREC_FP16) {
+        MLK_CHECK_HIP(hipMemsetAsync(dflag, 0, 4, stream));
+        hipLaunchKernelGGL(k_maxabs, dim3(1024), dim3(256), 0, stream, index->X.as<float>(), n * (int64_t)d, d,
+                           index->centre.as<float>(), dflag);
+        MLK_CHECK_HIP(hipGetLastError());
+        unsigned bits = 0;
+        MLK_CHECK_HIP(hipMemcpyAsync(&bits, dflag, 4, hipMemcpyDeviceToHost, stream));
+        MLK_CHECK_HIP(hipStreamSynchronize(stream));
+        float mx;
+        std::memcpy(&mx, &bits, 4);
+        if (mx > 0.0f && std::isfinite(mx)) {
+            int e;
+            std::frexp((double)mx, &e);  // mx < 2^e
+            scale = std::ldexp(1.0, 14 - e);  // scaled max < 2^14
+        }
+    }
+    mr.scale = scale;
+    MLK_TRY(mr.xs.reserve((size_t)n * mr.dp * elem_bytes(prec)));
+    MLK_TRY(mr.nx.reserve((size_t)n * sizeof(double)));
+    MLK_TRY(mr.flags.reserve((size_t)n));
+    MLK_CHECK_HIP(hipMemsetAsync(dflag, 0, 8, stream));
+    MLK_TRY(mirror_rows(prec, index->X.as<float>(), n, n, d, mr.dp, index->centre.as<float>(), scale,
+                        mr.xs.ptr, mr.nx.as<double>(), mr.flags.as<uint8_t>(), dflag, stream));
+    MLK_TRY(mr.unsafe.reserve((size_t)n * sizeof(int)));
+    hipLaunchKernelGGL(k_unsafe_list, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream,
+                       mr.flags.as<uint8_t>(), n, mr.unsafe.as<int>(), dflag + 1);
+    MLK_CHECK_HIP(hipGetLastError());
+    unsigned host[2] = {0, 0};
+    MLK_CHECK_HIP(hipMemcpyAsync(host, dflag, 8, hipMemcpyDeviceToHost, stream));
+    MLK_CHECK_HIP(hipStreamSynchronize(stream));
+    mr.exact = host[0] == 0;
+    mr.n_unsafe = host[1];
+    if (mr.n_unsafe) {
+        // Deterministic order of the unsafe list (atomics appended in any order).
+        std::vector<int> rows(mr.n_unsafe);
+        MLK_CHECK_HIP(hipMemcpy(rows.data(), mr.unsafe.ptr, rows.size() * sizeof(int), hipMemcpyDeviceToHost));
+        std::sort(rows.begin(), rows.end());
+        MLK_CHECK_HIP(hipMemcpy(mr.unsafe.ptr, rows.data(), rows.size() * sizeof(int), hipMemcpyHostToDevice));
+    }
+    mr.bias_h = -1.0;
+    mr.ready = true;
+    return MLK_OK;
+}
+
+int prepare_bias(mlk_index* index, int prec, double G, hipStream_t stream) {
+    Mirror& mr = index->mirror[prec];
+    if (mr.bias_h == G) return MLK_OK;
+    MLK_TRY(mr.bias.reserve((size_t)index->n * sizeof(float)));
+    hipLaunchKernelGGL(k_bias, dim3((unsigned)((index->n + 255) / 256)), dim3(256), 0, stream,
+                       mr.nx.as<double>(), mr.flags.as<uint8_t>(), index->n, G, mr.bias.as<float>());
+    MLK_CHECK_HIP(hipGetLastError());
+    mr.bias_h = G;
+    return MLK_OK;
+}
+
+int create_index(const float* X, int64_t n, int d, hipStream_t stream, mlk_index** out) {
+    mlk_index* index = new (std::nothrow) mlk_index();
+    if (!index) {
+        set_error("host allocation failed");
+        return MLK_ENOMEM;
+    }
+    index->n = n;
+    index->d = d;
+    int rc = MLK_OK;
+    do {
+        if ((rc = index->X.reserve((size_t)n * d * sizeof(float)))) break;
+        if (hipMemcpyAsync(index->X.ptr, X, (size_t)n * d * sizeof(float), hipMemcpyDeviceToDevice, stream) !=
+            hipSuccess) {
+            set_error("copy of the training matrix failed");
+            rc = MLK_EDEVICE;
+            break;
+        }
+        // Centre: float64 column means; integral data get an integer centre.
+        const int64_t rows_per_block = 4096;
+        const int64_t blocks = (n + rows_per_block - 1) / rows_per_block;
+        DevBuf partial;
+        if ((rc = partial.reserve((size_t)blocks * d * sizeof(double) + 16))) break;
+        unsigned* nonint = (unsigned*)((char*)partial.ptr + (size_t)blocks * d * sizeof(double));
+        if (hipMemsetAsync(nonint, 0, 4, stream) != hipSuccess) { rc = MLK_EDEVICE; break; }
+        hipLaunchKernelGGL(k_colsum, dim3((unsigned)blocks), dim3(256), 0, stream, index->X.as<float>(), n, d,
+                           rows_per_block, partial.as<double>(), nonint);
+        if (hipGetLastError() != hipSuccess) { set_error("column sum launch failed"); rc = MLK_EDEVICE; break; }
+        std::vector<double> host((size_t)blocks * d);
+        unsigned host_nonint = 0;
+        if (hipMemcpyAsync(host.data(), partial.ptr, host.size() * sizeof(double), hipMemcpyDeviceToHost, stream) !=
+                hipSuccess ||
+            hipMemcpyAsync(&host_nonint, nonint, 4, hipMemcpyDeviceToHost, stream) != hipSuccess ||
+            hipStreamSynchronize(stream) != hipSuccess) {
+            set_error("reading column sums failed");
+            rc = MLK_EDEVICE;
+            break;
+        }
+        index->integral = host_nonint == 0;
+        std::vector<float> centre(d);
+        for (int c = 0; c < d; ++c) {
+            double s = 0.0;
+            for (int64_t b = 0; b < blocks; ++b) s += host[(size_t)b * d + c];
+            double mean = s / (double)n;
+            centre[c] = index->integral ? (float)std::nearbyint(mean) : (float)mean;
+            if (!std::isfinite(centre[c])) centre[c] = 0.0f;
+        }
+        if ((rc = index->centre.reserve((size_t)d * sizeof(float)))) break;
+        if (hipMemcpy(index->centre.ptr, centre.data(), d * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
+            set_error("uploading the centre failed");
+            rc = MLK_EDEVICE;
+            break;
+        }
+    } while (0);
+    if (rc) {
+        delete index;
+        return rc;
+    }
+    *out = index;
+    return MLK_OK;
+}
+
+}  // namespace mlk

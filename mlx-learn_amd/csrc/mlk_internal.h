@@ -1,0 +1,118 @@
+// Host-side internals shared by the library's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/mlxlearn_amd.h"
+
+namespace mlk {
+
+// ------------------------------------------------------------------ errors
+void set_error(const std::string& msg);
+struct Status {
+    int code = MLK_OK;
+    static Status ok() { return {}; }
+};
+
+#define MLK_CHECK_HIP(expr)                                                                   \
+    do {                                                                                      \
+        hipError_t _e = (expr);                                                               \
+        if (_e != hipSuccess) {                                                               \
+            ::mlk::set_error(std::string(#expr " failed: ") + hipGetErrorString(_e));          \
+            return _e == hipErrorOutOfMemory ? MLK_ENOMEM : MLK_EDEVICE;                      \
+        }                                                                                     \
+    } while (0)
+
+#define MLK_TRY(expr)                                                                         \
+    do {                                                                                      \
+        int _rc = (expr);                                                                     \
+        if (_rc != MLK_OK) return _rc;                                                        \
+    } while (0)
+
+// Grow-only device buffer.
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    int reserve(size_t need);
+    void release();
+    ~DevBuf() { release(); }
+    template <class T> T* as() const { return static_cast<T*>(ptr); }
+};
+
+// Screen precisions.
+enum Prec { PREC_FP32 = 0, PREC_BF16 = 1, PREC_FP16 = 2 };
+
+inline int elem_bytes(int prec) { return prec == PREC_FP32 ? 4 : 2; }
+
+// Per-precision screen mirror of the database.
+struct Mirror {
+    bool ready = false;
+    int dp = 0;             // padded feature count (multiple of 32)
+    double scale = 1.0;     // power of two applied before rounding (float16 range)
+    bool exact = false;     // every scaled centred coordinate is exact in the screen type
+    int64_t n_unsafe = 0;   // rows whose coordinates leave the screen's safe range
+    DevBuf xs;              // n x dp screen values
+    DevBuf nx;              // n float64 squared norms of scale * (x - c)
+    DevBuf flags;           // n uint8: bit0 = safe
+    DevBuf unsafe;          // n_unsafe int32 row ids
+    DevBuf bias;            // n float, for `bias_h`
+    double bias_h = -1.0;   // error constant the bias was built for
+};
+
+// Tuning knobs (0 = automatic).
+struct Tuning {
+    int splits = 0;
+    int capacity = 0;
+    int query_tiles = 0;
+    std::vector<double> rounds;
+};
+Tuning& tuning();
+
+struct TestHooks {
+    uint32_t hooks = 0;
+    int64_t param = 0;
+    std::vector<int64_t> forced_rows;
+};
+TestHooks& hooks();
+mlk_stats& stats();
+
+}  // namespace mlk
+
+struct mlk_index {
+    int64_t n = 0;
+    int32_t d = 0;
+    bool integral = false;   // every coordinate is an integer
+    mlk::DevBuf X;           // n x d float32 copy (authoritative coordinates)
+    mlk::DevBuf centre;      // d float32
+    mlk::Mirror mirror[3];
+    std::mutex lock;
+    // per-call workspace
+    mlk::DevBuf ws[16];
+};
+
+struct mlk_radius_result {
+    int64_t m = 0;
+    int64_t nnz = 0;
+    bool has_dist = false;
+    mlk::DevBuf indices;  // nnz int64
+    mlk::DevBuf dist;     // nnz double
+};
+
+namespace mlk {
+// implemented in mlk_index.hip
+int prepare_mirror(mlk_index* index, int prec, hipStream_t stream);
+int prepare_bias(mlk_index* index, int prec, double H, hipStream_t stream);
+// implemented in mlk_knn.hip
+int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q_offset, uint32_t flags,
+                   double* dist, int64_t* idx, hipStream_t stream, const int64_t* batch_offsets,
+                   int n_batches);
+// implemented in mlk_radius.hip
+int radius_query_impl(mlk_index* index, const float* Q, int64_t m, double R2, int64_t q_offset,
+                      uint32_t flags, int64_t* indptr, int64_t* nnz, mlk_radius_result** out,
+                      hipStream_t stream);
+}  // namespace mlk

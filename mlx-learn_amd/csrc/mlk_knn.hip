@@ -1,0 +1,686 @@
+// Exact k-nearest-neighbour search on gfx950.
+//
+// Pipeline (one stream, rounds launched back to back):
+//   qprep   queries -> screen mirror (bf16 / fp16 / fp32), float64 norms, flags
+//   tau0    per query: k-th smallest certified upper bound over 64 sampled rows
+//   screen  fused MFMA Gram tile + threshold filter; hits append (row, acc) to a
+//           per-(query, split) candidate list; the N x M matrix never leaves
+//           registers (replaces gram_lower + select_topk, reference
+//           src/mlxlearn_device/_neighbors_general.py:188-329 and
+//           _neighbors_fused.py:421-506)
+//   merge   per query: k-th smallest certified upper bound over all
+//           candidates -> tighter threshold for the next round
+//   rank    per query: lower-bound filter, float64 distances, exact (S, index)
+//           order with interval tests and exact integer fall back (replaces the
+//           refine / candidate-order / certificate kernels,
+//           _neighbors_fused.py:545-834 and _neighbors_precision.py)
+//   fallback  queries whose candidate lists overflowed (or whose coordinates
+//           leave the screen's safe range): float64 scan of every row, radix
+//           select of the k-th value, collection of the interval band, exact
+//           rank (replaces _neighbors_fallback.py partitioned_neighbors).
+//
+// Certificate. For a screen value acc (MFMA accumulation of the rounded
+// centred coordinates, started from the bias -a_x), every row x satisfies
+//   L(x) <= S(q, x) <= U(x),  L/U = (nq + nx)(1 -/+ H) - 2(acc + a_x) -/+ (g nx + 2 f0)
+// with H covering operand rounding (2 eta + eta^2), accumulation in any order
+// with any rounding (gamma_{K+1}, unit 2^-23), float64 norm errors and flushed
+// subnormal partial sums (f0). A query keeps every row with L(x) <= tau, where
+// tau is an upper bound of its k-th smallest S (the k-th smallest U seen so
+// far), so the kept set contains every true neighbour; the rank stage orders
+// the kept set exactly. Overflow of a candidate list withdraws the
+// certificate and sends the query to the exact fallback.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "mlk_device.h"
+#include "mlk_internal.h"
+#include "mlk_screen.h"
+
+namespace mlk {
+
+int mirror_rows(int prec, const float* X, int64_t n, int64_t n_pad, int d, int dp, const float* c,
+                double scale, void* xs, double* norms, uint8_t* flags, unsigned* inexact, hipStream_t stream);
+
+// status bits per query
+constexpr int ST_UNSAFE = 1;     // query coordinates outside the screen's safe range
+constexpr int ST_OVERFLOW = 2;   // a candidate list overflowed
+constexpr int ST_FORCED = 4;     // certificate withdrawn by a test hook
+constexpr int ST_FALLBACK = 8;   // resolved by the exact fallback
+
+constexpr int LCAP = 512;        // candidates ranked in LDS per query
+
+// ---------------------------------------------------------------- tau0
+__global__ void k_tau0(const float* __restrict__ X, int64_t n, int d, const float* __restrict__ Q, int64_t m,
+                       int64_t m_pad, int k, int64_t self_off, int exclude, const double* __restrict__ nq,
+                       const uint8_t* __restrict__ qflags, Consts c, float* __restrict__ tau,
+                       float* __restrict__ thr, int* __restrict__ status) {
+    const int lane = threadIdx.x & 63;
+    const int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (q >= m_pad) return;
+    if (q >= m) {
+        if (lane == 0) { tau[q] = -1.0f; thr[q] = INFINITY; status[q] = 0; }
+        return;
+    }
+    const int samples = n < 64 ? (int)n : 64;
+    uint32_t bits = 0x7f800000u;
+    if (lane < samples) {
+        const int64_t row = (int64_t)lane * n / samples;
+        if (!(exclude && row == self_off + q)) {
+            const double s = canonical_sq(Q + q * d, X + row * d, d);
+            bits = __float_as_uint(__double2float_ru(s * (1.0 + f64_interval(d)) * c.scale2));
+        }
+    }
+    __shared__ uint32_t sb[4][64];
+    sb[threadIdx.x >> 6][lane] = bits;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    const uint32_t* mine = sb[threadIdx.x >> 6];
+    uint32_t kb = wave_kth_bits(64, k, [&](int j) { return mine[j]; });
+    float t0 = __uint_as_float(kb);
+    if (lane == 0) {
+        const int st = qflags[q] ? 0 : ST_UNSAFE;
+        status[q] = st;
+        tau[q] = t0;
+        thr[q] = st ? INFINITY : thr_from_tau((double)t0, nq[q], c);
+    }
+}
+
+// ---------------------------------------------------------------- merge
+__global__ void k_merge(int64_t m, int k, const int* __restrict__ cnt, const int* __restrict__ cidx,
+                        const float* __restrict__ cacc, int S, int cap, const double* __restrict__ nq,
+                        const double* __restrict__ nx, const float* __restrict__ bias, Consts c,
+                        float* __restrict__ tau, float* __restrict__ thr, int* __restrict__ status) {
+    __shared__ uint32_t sU[4][LCAP];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t q = (int64_t)blockIdx.x * 4 + w;
+    if (q >= m) return;
+    if (status[q]) return;
+    // totals and per-lane prefix over splits
+    int total = 0;
+    bool over = false;
+    for (int s = lane; s < S; s += 64) {
+        int cs = cnt[q * S + s];
+        over |= cs > cap;
+        total += cs < cap ? cs : cap;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        total += __shfl_xor(total, o);
+        over = __shfl_xor((int)over, o) | over;
+    }
+    if (over) {
+        if (lane == 0) { status[q] = ST_OVERFLOW; thr[q] = INFINITY; }
+        return;
+    }
+    if (total < k || total > LCAP) return;  // threshold stays valid (just not tighter)
+    // compact upper bounds into LDS: lanes take splits, exclusive scan of counts
+    int base = 0;
+    for (int s0 = 0; s0 < S; s0 += 64) {
+        const int s = s0 + lane;
+        int cs = 0;
+        if (s < S) { cs = cnt[q * S + s]; cs = cs < cap ? cs : cap; }
+        int incl = cs;
+        for (int o = 1; o < 64; o <<= 1) {
+            int v = __shfl_up(incl, o);
+            if (lane >= o) incl += v;
+        }
+        const int at = base + incl - cs;
+        const double nqq = nq[q];
+        for (int j = 0; j < cs; ++j) {
+            const size_t off = ((size_t)q * S + s) * cap + j;
+            const int row = cidx[off];
+            const double U = upper_bound(cacc[off], bias[row], nqq, nx[row], c);
+            sU[w][at + j] = __float_as_uint(fmaxf(__double2float_ru(U), 0.0f));
+        }
+        base += __shfl(incl, 63);
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    const uint32_t* mine = sU[w];
+    const uint32_t kb = wave_kth_bits(total, k, [&](int j) { return mine[j]; });
+    if (lane == 0) {
+        const float t_new = fminf(tau[q], __uint_as_float(kb));
+        tau[q] = t_new;
+        thr[q] = thr_from_tau((double)t_new, nq[q], c);
+    }
+}
+
+// ---------------------------------------------------------------- rank
+struct RankArgs {
+    const float* X;
+    int64_t n;
+    int d;
+    const float* Q;
+    int64_t m;
+    int k;
+    // screen candidate lists
+    const int* cnt;
+    const int* cidx;
+    const float* cacc;
+    int S, cap;
+    const double* nq;
+    const double* nx;
+    const float* bias;
+    const float* tau;
+    Consts c;
+    const int* unsafe;
+    int n_unsafe;
+    // fallback plain lists: candidates of fallback slot f are plist/ps[f*pstride + j], j < pcount[f]
+    const int* fq;
+    const int* plist;
+    const double* ps;
+    const int* pcount;
+    int64_t pstride;
+    int mode;  // 0 = screen lists, 1 = fallback lists
+    int64_t self_off;
+    int exclude;
+    int* status;
+    int64_t* out_idx;
+    double* out_dist;
+    int squared;
+};
+
+struct Cand {
+    double s;
+    long long i;
+};
+
+__device__ __forceinline__ Cand shfl_cand(Cand v, int o) {
+    Cand r;
+    r.s = __shfl_xor(v.s, o);
+    r.i = __shfl_xor(v.i, o);
+    return r;
+}
+
+// One wave per query: exact (S, index) order of the candidate set.
+__global__ __launch_bounds__(256) void k_rank(const RankArgs a, int64_t n_items) {
+    __shared__ int sI[4][LCAP];
+    __shared__ double sS[4][LCAP];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t item = (int64_t)blockIdx.x * 4 + w;
+    if (item >= n_items) return;
+    const int d = a.d, k = a.k;
+    const double g = f64_interval(d);
+    int64_t q;
+    if (a.mode == 0) {
+        q = item;
+        if (a.status[q]) return;  // handled by the fallback
+    } else {
+        q = a.fq[item];
+    }
+    const float* qv = a.Q + q * d;
+    int* li = sI[w];
+    double* ls = sS[w];
+    int count = 0;
+    if (a.mode == 0) {
+        const double tau = (double)a.tau[q];
+        const double nqq = a.nq[q];
+        const int64_t self = a.exclude ? a.self_off + q : -1;
+        // Rows whose certified lower bound does not exceed tau, compacted in
+        // two passes (count per lane, exclusive scan, place).
+        int mine = 0;
+        for (int s0 = 0; s0 < a.S; s0 += 64) {
+            const int s = s0 + lane;
+            int cs = 0;
+            if (s < a.S) { cs = a.cnt[q * a.S + s]; cs = cs < a.cap ? cs : a.cap; }
+            for (int j = 0; j < cs; ++j) {
+                const size_t off = ((size_t)q * a.S + s) * a.cap + j;
+                const int row = a.cidx[off];
+                mine += lower_bound(a.cacc[off], a.bias[row], nqq, a.nx[row], a.c) <= tau;
+            }
+        }
+        int incl = mine;
+        for (int o = 1; o < 64; o <<= 1) {
+            int v = __shfl_up(incl, o);
+            if (lane >= o) incl += v;
+        }
+        int total = __shfl(incl, 63);
+        int extra = 0;
+        for (int u = 0; u < a.n_unsafe; ++u) extra += a.unsafe[u] != self;
+        if (total + extra > LCAP) {
+            if (lane == 0) a.status[q] = ST_OVERFLOW;
+            return;
+        }
+        int at = incl - mine;
+        for (int s0 = 0; s0 < a.S; s0 += 64) {
+            const int s = s0 + lane;
+            int cs = 0;
+            if (s < a.S) { cs = a.cnt[q * a.S + s]; cs = cs < a.cap ? cs : a.cap; }
+            for (int j = 0; j < cs; ++j) {
+                const size_t off = ((size_t)q * a.S + s) * a.cap + j;
+                const int row = a.cidx[off];
+                if (lower_bound(a.cacc[off], a.bias[row], nqq, a.nx[row], a.c) <= tau) li[at++] = row;
+            }
+        }
+        if (lane == 0) {
+            int u2 = total;
+            for (int u = 0; u < a.n_unsafe; ++u)
+                if (a.unsafe[u] != self) li[u2++] = a.unsafe[u];
+        }
+        count = total + extra;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        // float64 distances, then drop rows provably beyond tau
+        const double cut = tau / a.c.scale2;
+        int kept = 0;
+        for (int j = lane; j < count; j += 64) {
+            const double s = canonical_sq(qv, a.X + (size_t)li[j] * d, d);
+            const bool ok = s * (1.0 - g) <= cut;
+            ls[j] = ok ? s : INFINITY;
+            kept += ok;
+        }
+        for (int o = 32; o > 0; o >>= 1) kept += __shfl_xor(kept, o);
+        if (kept < k) {
+            if (lane == 0) a.status[q] = ST_OVERFLOW;
+            return;
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    } else {
+        count = a.pcount[item];
+        if (count <= LCAP) {
+            for (int j = lane; j < count; j += 64) {
+                li[j] = a.plist[item * a.pstride + j];
+                ls[j] = a.ps[item * a.pstride + j];
+            }
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        } else {
+            // Large tie sets: k rounds of exact successor extraction from global memory.
+            Cand prev{-1.0, -1};
+            for (int r = 0; r < k; ++r) {
+                Cand best{INFINITY, -1};
+                for (int j = lane; j < count; j += 64) {
+                    Cand cj{a.ps[item * a.pstride + j], (long long)a.plist[item * a.pstride + j]};
+                    if (prev.i >= 0 && !exact_precedes(prev.s, prev.i, cj.s, cj.i, g, qv, a.X, d)) continue;
+                    if (best.i < 0 || exact_precedes(cj.s, cj.i, best.s, best.i, g, qv, a.X, d)) best = cj;
+                }
+                for (int o = 32; o > 0; o >>= 1) {
+                    Cand other = shfl_cand(best, o);
+                    if (other.i >= 0 &&
+                        (best.i < 0 || exact_precedes(other.s, other.i, best.s, best.i, g, qv, a.X, d)))
+                        best = other;
+                }
+                if (lane == 0) {
+                    a.out_idx[q * k + r] = best.i;
+                    if (a.out_dist) a.out_dist[q * k + r] = a.squared ? best.s : sqrt(best.s);
+                }
+                prev = best;
+            }
+            return;
+        }
+    }
+    // rank counting: position of each kept candidate in the exact (S, index) order
+    for (int j = lane; j < count; j += 64) {
+        const double sj = ls[j];
+        if (!(sj < INFINITY)) continue;
+        const long long ij = li[j];
+        int rank = 0;
+        for (int i = 0; i < count && rank < k; ++i) {
+            const double si = ls[i];
+            if (!(si < INFINITY) || i == j) continue;
+            rank += exact_precedes(si, li[i], sj, ij, g, qv, a.X, d);
+        }
+        if (rank < k) {
+            a.out_idx[q * k + rank] = ij;
+            if (a.out_dist) a.out_dist[q * k + rank] = a.squared ? sj : sqrt(sj);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- fallback
+__global__ void k_fb_dist(const float* __restrict__ X, int64_t n, int d, const float* __restrict__ Q,
+                          const int* __restrict__ fq, int64_t self_off, int exclude, double* __restrict__ s_out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = blockIdx.y;
+    if (i >= n) return;
+    const int64_t q = fq[f];
+    double s = (exclude && i == self_off + q) ? INFINITY : canonical_sq(Q + q * d, X + i * d, d);
+    s_out[(size_t)f * n + i] = s;
+}
+
+// k-th smallest float64 of each fallback row by 8-bit radix select over the bit pattern.
+__global__ void k_fb_select(const double* __restrict__ s_in, int64_t n, int k, int d, double* __restrict__ cut) {
+    __shared__ unsigned hist[256];
+    __shared__ unsigned long long s_prefix;
+    __shared__ unsigned s_rank;
+    const int f = blockIdx.x;
+    const double* s = s_in + (size_t)f * n;
+    unsigned long long prefix = 0, mask = 0;
+    unsigned rank = (unsigned)k;  // 1-based rank within the current prefix class
+    for (int shift = 56; shift >= 0; shift -= 8) {
+        for (int b = threadIdx.x; b < 256; b += blockDim.x) hist[b] = 0;
+        __syncthreads();
+        for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+            const unsigned long long v = (unsigned long long)__double_as_longlong(s[i]);
+            if ((v & mask) == prefix) atomicAdd(&hist[(v >> shift) & 255], 1u);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned acc = 0;
+            int b = 0;
+            for (; b < 256; ++b) {
+                if (acc + hist[b] >= rank) break;
+                acc += hist[b];
+            }
+            if (b == 256) b = 255;
+            s_prefix = prefix | ((unsigned long long)b << shift);
+            s_rank = rank - acc;
+        }
+        __syncthreads();
+        prefix = s_prefix;
+        rank = s_rank;
+        mask |= 255ull << shift;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double tk = __longlong_as_double((long long)prefix);
+        const double g = f64_interval(d);
+        cut[f] = tk < INFINITY ? tk * (1.0 + g) / (1.0 - g) * (1.0 + 0x1p-50) : INFINITY;
+    }
+}
+
+__global__ void k_fb_collect(const double* __restrict__ s_in, int64_t n, const double* __restrict__ cut,
+                             int* __restrict__ pcount, int* __restrict__ plist, double* __restrict__ ps,
+                             int64_t pstride) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = blockIdx.y;
+    if (i >= n) return;
+    const double s = s_in[(size_t)f * n + i];
+    if (s <= cut[f]) {
+        const int at = atomicAdd(&pcount[f], 1);
+        plist[(size_t)f * pstride + at] = (int)i;
+        ps[(size_t)f * pstride + at] = s;
+    }
+}
+
+__global__ void k_qprep_status(int64_t m, const int* __restrict__ status, int* __restrict__ fq,
+                               unsigned* __restrict__ nfq) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < m && status[q]) {
+        const unsigned at = atomicAdd(nfq, 1u);
+        fq[at] = (int)q;
+    }
+}
+
+__global__ void k_force(int64_t m, const int64_t* __restrict__ rows, int64_t count, int* __restrict__ status,
+                        float* __restrict__ thr) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count && rows[i] >= 0 && rows[i] < m) {
+        status[rows[i]] |= ST_FORCED;
+        thr[rows[i]] = INFINITY;
+    }
+}
+
+// ---------------------------------------------------------------- host
+typedef void (*ScreenFn)(const ScreenArgs);
+
+template <int PREC, int QT>
+static ScreenFn pick_dp(int dp) {
+    switch (dp) {
+        case 32: return k_screen<PREC, 32, QT, 2, SCREEN_LISTS>;
+        case 64: return k_screen<PREC, 64, QT, 2, SCREEN_LISTS>;
+        case 96: return k_screen<PREC, 96, QT, 2, SCREEN_LISTS>;
+        case 128: return k_screen<PREC, 128, QT, 2, SCREEN_LISTS>;
+        default: return nullptr;
+    }
+}
+
+static ScreenFn pick_screen(int prec, int dp, int qt) {
+    if (prec == PREC_FP32) return qt == 1 ? pick_dp<PREC_FP32, 1>(dp) : pick_dp<PREC_FP32, 2>(dp);
+    if (prec == PREC_BF16) return qt == 1 ? pick_dp<PREC_BF16, 1>(dp) : pick_dp<PREC_BF16, 2>(dp);
+    return qt == 1 ? pick_dp<PREC_FP16, 1>(dp) : pick_dp<PREC_FP16, 2>(dp);
+}
+
+static double unit_roundoff(int prec) {
+    switch (prec) {
+        case PREC_FP32: return 0x1p-24;
+        case PREC_BF16: return 0x1p-8 + 0x1p-23;
+        default: return 0x1p-11 + 0x1p-23;
+    }
+}
+
+Consts make_consts(int prec, int dp, int d, bool exact_inputs, double scale) {
+    Consts c;
+    const double eta = exact_inputs ? 0.0 : unit_roundoff(prec);
+    const double Kp = dp + 1;
+    c.gamma = Kp * 0x1p-23 / (1.0 - Kp * 0x1p-23);
+    c.H = 2 * eta + eta * eta + c.gamma * (1 + eta) * (1 + eta) + (d + 8) * 0x1p-50;
+    c.G = c.H + c.gamma + 0x1p-40;
+    c.f0 = (dp + 2) * 0x1p-126;
+    c.scale2 = scale * scale;
+    return c;
+}
+
+struct Plan {
+    int qt, splits, cap;
+    std::vector<double> rounds;  // cumulative fractions of each split, ending at 1
+};
+
+static Plan make_plan(int prec, int dp, int64_t n, int64_t m, int k) {
+    Plan p;
+    const Tuning& t = tuning();
+    p.qt = t.query_tiles ? t.query_tiles : 2;
+    const int64_t BQ = 4 * p.qt * 32;
+    const int64_t qb = (m + BQ - 1) / BQ;
+    if (t.splits) {
+        p.splits = t.splits;
+    } else {
+        // about 4 blocks per CU, at least 128 rows per split, multiple of 8
+        int64_t want = (1024 + qb - 1) / qb;
+        int64_t most = std::max<int64_t>(1, n / 128);
+        want = std::min(want, most);
+        want = std::max<int64_t>(want, 1);
+        if (want >= 8) want = (want + 7) / 8 * 8;
+        p.splits = (int)std::min<int64_t>(want, 4096);
+    }
+    p.cap = t.capacity ? t.capacity : (prec == PREC_FP32 ? 32 : 64);
+    if (!t.rounds.empty()) {
+        p.rounds = t.rounds;
+    } else {
+        p.rounds = {1.0 / 64, 1.0 / 8, 1.0};
+    }
+    (void)k;
+    return p;
+}
+
+int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q_offset, uint32_t flags,
+                   double* dist, int64_t* idx, hipStream_t stream, const int64_t* /*batch_offsets*/,
+                   int /*n_batches*/) {
+    mlk_stats& st = stats();
+    st = mlk_stats{};
+    st.queries = m;
+    if (m == 0) return MLK_OK;
+    const int64_t n = index->n;
+    const int d = index->d;
+    const int prec = (int)((flags & MLK_PREC_MASK) >> 4);
+    const bool exclude = (flags & MLK_EXCLUDE_SELF) != 0;
+    st.screen_precision = flags & MLK_PREC_MASK;
+
+    hipEvent_t ev0, ev1, ev2;
+    MLK_CHECK_HIP(hipEventCreate(&ev0));
+    MLK_CHECK_HIP(hipEventCreate(&ev1));
+    MLK_CHECK_HIP(hipEventCreate(&ev2));
+    struct EvGuard {
+        hipEvent_t a, b, c;
+        ~EvGuard() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); (void)hipEventDestroy(c); }
+    } evg{ev0, ev1, ev2};
+    MLK_CHECK_HIP(hipEventRecord(ev0, stream));
+
+    MLK_TRY(prepare_mirror(index, prec, stream));
+    Mirror& mr = index->mirror[prec];
+    const int dp = mr.dp;
+    ScreenFn screen = nullptr;
+    Plan plan = make_plan(prec, dp, n, m, k);
+    screen = pick_screen(prec, dp, plan.qt);
+    const bool all_fallback = !screen || k > 64 || mr.n_unsafe > 64 || (hooks().hooks & MLK_HOOK_FORCE_FALLBACK);
+    if (hooks().hooks & MLK_HOOK_CAPACITY) plan.cap = (int)std::max<int64_t>(1, hooks().param);
+
+    const int BQ = 4 * plan.qt * 32;
+    const int64_t QB = (m + BQ - 1) / BQ;
+    const int64_t m_pad = QB * BQ;
+    const int S = plan.splits;
+    const int cap = plan.cap;
+    st.splits = S;
+    st.capacity = cap;
+
+    // workspace
+    DevBuf* ws = index->ws;
+    const size_t eb = elem_bytes(prec);
+    MLK_TRY(ws[0].reserve((size_t)m_pad * dp * eb));          // query mirror
+    MLK_TRY(ws[1].reserve((size_t)m_pad * sizeof(double)));   // nq
+    MLK_TRY(ws[2].reserve((size_t)m_pad + 64));               // qflags (+ flag words)
+    MLK_TRY(ws[3].reserve((size_t)m_pad * sizeof(float)));    // tau
+    MLK_TRY(ws[4].reserve((size_t)m_pad * sizeof(float)));    // thr
+    MLK_TRY(ws[5].reserve((size_t)m_pad * sizeof(int)));      // status
+    unsigned* dflag;
+    MLK_TRY(ws[6].reserve(64));
+    dflag = ws[6].as<unsigned>();
+    MLK_CHECK_HIP(hipMemsetAsync(dflag, 0, 16, stream));
+    MLK_TRY(mirror_rows(prec, Q, m, m_pad, d, dp, index->centre.as<float>(), mr.scale, ws[0].ptr,
+                        ws[1].as<double>(), ws[2].as<uint8_t>(), dflag, stream));
+    unsigned q_inexact = 1;
+    MLK_CHECK_HIP(hipMemcpyAsync(&q_inexact, dflag, 4, hipMemcpyDeviceToHost, stream));
+    MLK_CHECK_HIP(hipStreamSynchronize(stream));
+    const Consts c = make_consts(prec, dp, d, mr.exact && q_inexact == 0, mr.scale);
+    MLK_TRY(prepare_bias(index, prec, c.G, stream));
+
+    const float* X = index->X.as<float>();
+    float* tau = ws[3].as<float>();
+    float* thr = ws[4].as<float>();
+    int* status = ws[5].as<int>();
+    hipLaunchKernelGGL(k_tau0, dim3((unsigned)((m_pad + 3) / 4)), dim3(256), 0, stream, X, n, d, Q, m, m_pad, k,
+                       q_offset, (int)exclude, ws[1].as<double>(), ws[2].as<uint8_t>(), c, tau, thr, status);
+    MLK_CHECK_HIP(hipGetLastError());
+
+    if ((hooks().hooks & MLK_HOOK_FORCE_UNCERTIFIED) && !hooks().forced_rows.empty()) {
+        const auto& rows = hooks().forced_rows;
+        MLK_TRY(ws[7].reserve(rows.size() * sizeof(int64_t)));
+        MLK_CHECK_HIP(hipMemcpyAsync(ws[7].ptr, rows.data(), rows.size() * sizeof(int64_t), hipMemcpyHostToDevice,
+                                     stream));
+        hipLaunchKernelGGL(k_force, dim3((unsigned)((rows.size() + 255) / 256)), dim3(256), 0, stream, m,
+                           ws[7].as<int64_t>(), (int64_t)rows.size(), status, thr);
+        MLK_CHECK_HIP(hipGetLastError());
+    }
+
+    MLK_CHECK_HIP(hipEventRecord(ev1, stream));
+    if (!all_fallback) {
+        MLK_TRY(ws[8].reserve((size_t)m_pad * S * sizeof(int)));
+        MLK_TRY(ws[9].reserve((size_t)m_pad * S * cap * sizeof(int)));
+        MLK_TRY(ws[10].reserve((size_t)m_pad * S * cap * sizeof(float)));
+        MLK_CHECK_HIP(hipMemsetAsync(ws[8].ptr, 0, (size_t)m_pad * S * sizeof(int), stream));
+        ScreenArgs sa;
+        sa.xs = mr.xs.ptr;
+        sa.bias = mr.bias.as<float>();
+        sa.qs = ws[0].ptr;
+        sa.thr = thr;
+        sa.cnt = ws[8].as<int>();
+        sa.cidx = ws[9].as<int>();
+        sa.cacc = ws[10].as<float>();
+        sa.n = n;
+        sa.split_rows = (n + S - 1) / S;
+        sa.S = S;
+        sa.cap = cap;
+        sa.QB = (int)QB;
+        sa.self_off = q_offset;
+        sa.exclude = exclude;
+        sa.xcd_map = (S % 8) == 0;
+        const unsigned blocks = (unsigned)(QB * S);
+        double prev = 0.0;
+        for (size_t r = 0; r < plan.rounds.size(); ++r) {
+            const double hi = r + 1 == plan.rounds.size() ? 1.0 : plan.rounds[r];
+            sa.off_lo = (int64_t)std::llround(prev * sa.split_rows);
+            sa.off_hi = r + 1 == plan.rounds.size() ? sa.split_rows : (int64_t)std::llround(hi * sa.split_rows);
+            prev = hi;
+            if (sa.off_hi <= sa.off_lo) continue;
+            hipLaunchKernelGGL(screen, dim3(blocks), dim3(256), 0, stream, sa);
+            MLK_CHECK_HIP(hipGetLastError());
+            ++st.rounds;
+            if (r + 1 < plan.rounds.size()) {
+                hipLaunchKernelGGL(k_merge, dim3((unsigned)((m + 3) / 4)), dim3(256), 0, stream, m, k, sa.cnt,
+                                   sa.cidx, sa.cacc, S, cap, ws[1].as<double>(), mr.nx.as<double>(), sa.bias, c, tau,
+                                   thr, status);
+                MLK_CHECK_HIP(hipGetLastError());
+            }
+        }
+        // final overflow check (the last round has no merge)
+        hipLaunchKernelGGL(k_merge, dim3((unsigned)((m + 3) / 4)), dim3(256), 0, stream, m, k, sa.cnt, sa.cidx,
+                           sa.cacc, S, cap, ws[1].as<double>(), mr.nx.as<double>(), sa.bias, c, tau, thr, status);
+        MLK_CHECK_HIP(hipGetLastError());
+        MLK_CHECK_HIP(hipEventRecord(ev2, stream));
+
+        RankArgs ra{};
+        ra.X = X; ra.n = n; ra.d = d; ra.Q = Q; ra.m = m; ra.k = k;
+        ra.cnt = sa.cnt; ra.cidx = sa.cidx; ra.cacc = sa.cacc; ra.S = S; ra.cap = cap;
+        ra.nq = ws[1].as<double>(); ra.nx = mr.nx.as<double>(); ra.bias = sa.bias; ra.tau = tau; ra.c = c;
+        ra.unsafe = mr.unsafe.as<int>(); ra.n_unsafe = (int)mr.n_unsafe;
+        ra.mode = 0; ra.self_off = q_offset; ra.exclude = exclude; ra.status = status;
+        ra.out_idx = idx; ra.out_dist = dist; ra.squared = (flags & MLK_SQUARED_DIST) != 0;
+        hipLaunchKernelGGL(k_rank, dim3((unsigned)((m + 3) / 4)), dim3(256), 0, stream, ra, m);
+        MLK_CHECK_HIP(hipGetLastError());
+    } else {
+        MLK_CHECK_HIP(hipEventRecord(ev2, stream));
+        std::vector<int> all(m, ST_FALLBACK);
+        MLK_CHECK_HIP(hipMemcpyAsync(status, all.data(), m * sizeof(int), hipMemcpyHostToDevice, stream));
+        MLK_CHECK_HIP(hipStreamSynchronize(stream));
+    }
+
+    // queries without a certificate -> exact fallback
+    MLK_TRY(ws[11].reserve((size_t)m * sizeof(int)));
+    int* fq = ws[11].as<int>();
+    MLK_CHECK_HIP(hipMemsetAsync(dflag + 2, 0, 4, stream));
+    hipLaunchKernelGGL(k_qprep_status, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, stream, m, status, fq,
+                       dflag + 2);
+    MLK_CHECK_HIP(hipGetLastError());
+    unsigned nf = 0;
+    MLK_CHECK_HIP(hipMemcpyAsync(&nf, dflag + 2, 4, hipMemcpyDeviceToHost, stream));
+    MLK_CHECK_HIP(hipStreamSynchronize(stream));
+    st.fallback_queries = nf;
+    if (nf) {
+        std::vector<int> fq_host(nf);
+        MLK_CHECK_HIP(hipMemcpy(fq_host.data(), fq, nf * sizeof(int), hipMemcpyDeviceToHost));
+        std::sort(fq_host.begin(), fq_host.end());
+        MLK_CHECK_HIP(hipMemcpy(fq, fq_host.data(), nf * sizeof(int), hipMemcpyHostToDevice));
+        // batches bounded by ~1 GiB of float64 distances + candidate lists
+        const int64_t per_q = n * (int64_t)(sizeof(double) * 2 + sizeof(int));
+        int64_t fb = std::max<int64_t>(1, std::min<int64_t>(nf, (int64_t(1) << 30) / std::max<int64_t>(per_q, 1)));
+        MLK_TRY(ws[12].reserve((size_t)fb * n * sizeof(double)));  // distances
+        MLK_TRY(ws[13].reserve((size_t)fb * n * sizeof(int)));     // plist
+        MLK_TRY(ws[14].reserve((size_t)fb * n * sizeof(double)));  // ps
+        MLK_TRY(ws[15].reserve((size_t)fb * (sizeof(int) + sizeof(double)) + 64));
+        int* pcount = ws[15].as<int>();
+        double* cut = (double*)((char*)ws[15].ptr + ((fb * sizeof(int) + 15) / 16) * 16);
+        for (int64_t f0 = 0; f0 < (int64_t)nf; f0 += fb) {
+            const int64_t nb = std::min<int64_t>(fb, nf - f0);
+            dim3 g2((unsigned)((n + 255) / 256), (unsigned)nb);
+            hipLaunchKernelGGL(k_fb_dist, g2, dim3(256), 0, stream, X, n, d, Q, fq + f0, q_offset, (int)exclude,
+                               ws[12].as<double>());
+            hipLaunchKernelGGL(k_fb_select, dim3((unsigned)nb), dim3(256), 0, stream, ws[12].as<double>(), n, k, d,
+                               cut);
+            MLK_CHECK_HIP(hipMemsetAsync(pcount, 0, nb * sizeof(int), stream));
+            hipLaunchKernelGGL(k_fb_collect, g2, dim3(256), 0, stream, ws[12].as<double>(), n, cut, pcount,
+                               ws[13].as<int>(), ws[14].as<double>(), n);
+            RankArgs ra{};
+            ra.X = X; ra.n = n; ra.d = d; ra.Q = Q; ra.m = m; ra.k = k;
+            ra.fq = fq + f0; ra.plist = ws[13].as<int>(); ra.ps = ws[14].as<double>(); ra.pcount = pcount;
+            ra.pstride = n; ra.mode = 1; ra.self_off = q_offset; ra.exclude = exclude; ra.status = status;
+            ra.out_idx = idx; ra.out_dist = dist; ra.squared = (flags & MLK_SQUARED_DIST) != 0;
+            hipLaunchKernelGGL(k_rank, dim3((unsigned)((nb + 3) / 4)), dim3(256), 0, stream, ra, nb);
+            MLK_CHECK_HIP(hipGetLastError());
+        }
+    }
+    hipEvent_t ev3;
+    MLK_CHECK_HIP(hipEventCreate(&ev3));
+    MLK_CHECK_HIP(hipEventRecord(ev3, stream));
+    MLK_CHECK_HIP(hipEventSynchronize(ev3));
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, ev1, ev2) == hipSuccess) st.screen_ms = ms;
+    if (hipEventElapsedTime(&ms, ev0, ev3) == hipSuccess) st.total_ms = ms;
+    (void)hipEventDestroy(ev3);
+    return MLK_OK;
+}
+
+}  // namespace mlk

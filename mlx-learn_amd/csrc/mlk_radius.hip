@@ -1,0 +1,432 @@
+// Exact radius neighbourhoods on gfx950.
+//
+// Membership contract (reference radius_contract.py:1-25, 129-217;
+// _radius_common.h:1-18, 337-357): row x belongs to query q iff the exact
+// S(q, x) <= R2 = fl64(r*r). The screen keeps every row whose certified lower
+// bound L(x) <= R2 (MFMA tile + threshold, hits staged per wave in LDS and
+// appended to one pair buffer, so heavy-tailed neighbourhoods need no
+// per-query capacity). Candidate pairs are sorted by (query, row); each pair
+// gets the canonical float64 s, is classified against the outward-rounded
+// band [f64_in, f64_out] and the band is resolved exactly against
+// floor(R2 * 2^298). Members are compacted into CSR rows in ascending index
+// order, or stably re-sorted by distance for sort_results. Distances are
+// sqrt(min(s, R2)). This replaces the reference's per-tile device screen with
+// host finishing (radius_metal_v1.py:257-611, _radius_cpu_native.cpp:129-200)
+// by a device-only pipeline.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "mlk_device.h"
+#include "mlk_internal.h"
+#include "mlk_screen.h"
+
+namespace mlk {
+
+int mirror_rows(int prec, const float* X, int64_t n, int64_t n_pad, int d, int dp, const float* c,
+                double scale, void* xs, double* norms, uint8_t* flags, unsigned* inexact, hipStream_t stream);
+Consts make_consts(int prec, int dp, int d, bool exact_inputs, double scale);
+
+static void band(int d, double R2, double* f_in, double* f_out) {
+    // restates thresholds() of radius_contract.py:129-153
+    if (std::isinf(R2)) { *f_in = *f_out = INFINITY; return; }
+    if (R2 == 0.0) { *f_in = *f_out = 0.0; return; }
+    const double u = 0x1p-53;
+    const double g = (d + 2) * u / (1.0 - (d + 2) * u);
+    auto dn = [](double v) { return std::nextafter(v, -INFINITY); };
+    auto up = [](double v) { return std::nextafter(v, INFINITY); };
+    *f_in = std::max(dn(dn(R2 * dn(1.0 - 2.0 * g))), 0.0);
+    *f_out = up(up(R2 * up(1.0 + 2.0 * g)));
+}
+
+// floor(R2 * 2^298) in 20 little-endian 32-bit chunks; false when R2 >= 2^342.
+static bool r2_key(double R2, uint32_t key[SACC_LIMBS]) {
+    std::memset(key, 0, sizeof(uint32_t) * SACC_LIMBS);
+    if (!(R2 < 0x1p342)) return false;
+    if (!(R2 > 0)) return true;
+    int e;
+    const double fr = std::frexp(R2, &e);
+    const uint64_t mant = (uint64_t)std::ldexp(fr, 53);  // R2 = mant * 2^(e-53)
+    const int shift = e - 53 + 298;
+    if (shift >= 0) {
+        const unsigned limb = (unsigned)shift / 32, bit = (unsigned)shift % 32;
+        unsigned __int128 wide = (unsigned __int128)mant << bit;
+        for (unsigned i = limb; i < (unsigned)SACC_LIMBS && wide; ++i) {
+            key[i] = (uint32_t)wide;
+            wide >>= 32;
+        }
+    } else if (-shift < 64) {
+        const uint64_t v = mant >> (unsigned)(-shift);
+        key[0] = (uint32_t)v;
+        key[1] = (uint32_t)(v >> 32);
+    }
+    return true;
+}
+
+struct RadiusConsts {
+    double R2, f_in, f_out;
+    uint32_t key[SACC_LIMBS];
+};
+
+__global__ void k_radius_thr(int64_t m, int64_t m_pad, const double* __restrict__ nq, const uint8_t* __restrict__ qflags,
+                             double tau, Consts c, float* __restrict__ thr) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= m_pad) return;
+    if (q >= m) { thr[q] = INFINITY; return; }
+    // unsafe query: every row is a candidate
+    thr[q] = qflags[q] ? thr_from_tau(tau, nq[q], c) : -INFINITY;
+}
+
+// Unsafe database rows never hit the screen; they are candidates of every query.
+__global__ void k_unsafe_pairs(int64_t m, const int* __restrict__ unsafe, int n_unsafe, int64_t self_off, int exclude,
+                               int2* __restrict__ pairs, unsigned long long* __restrict__ count,
+                               unsigned long long cap) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m * n_unsafe) return;
+    const int64_t q = t / n_unsafe;
+    const int row = unsafe[t % n_unsafe];
+    if (exclude && row == self_off + q) return;
+    const unsigned long long at = atomicAdd(count, 1ull);
+    if (at < cap) pairs[at] = int2{(int)q, row};
+}
+
+__global__ void k_pair_keys(const int2* __restrict__ pairs, int64_t np, unsigned long long* __restrict__ keys) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < np) keys[i] = ((unsigned long long)(unsigned)pairs[i].x << 32) | (unsigned)pairs[i].y;
+}
+
+__global__ void k_classify(const unsigned long long* __restrict__ keys, int64_t np, const float* __restrict__ X,
+                           const float* __restrict__ Q, int d, RadiusConsts rc, uint8_t* __restrict__ member,
+                           double* __restrict__ dist) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    const unsigned long long key = keys[i];
+    const int64_t q = (int64_t)(key >> 32);
+    const int64_t row = (int64_t)(key & 0xffffffffull);
+    const float* qv = Q + q * d;
+    const float* xv = X + row * d;
+    const double s = canonical_sq(qv, xv, d);
+    bool in;
+    if (s <= rc.f_in) in = true;
+    else if (!(s <= rc.f_out)) in = false;
+    else in = exact_cmp_key(qv, xv, d, rc.key) <= 0;
+    member[i] = in;
+    dist[i] = sqrt(s < rc.R2 ? s : rc.R2);
+}
+
+__global__ void k_row_counts(const unsigned long long* __restrict__ keys, int64_t nnz, unsigned long long* __restrict__ counts) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nnz) atomicAdd(&counts[keys[i] >> 32], 1ull);
+}
+
+__global__ void k_key_to_index(const unsigned long long* __restrict__ keys, int64_t nnz, int64_t* __restrict__ idx) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nnz) idx[i] = (int64_t)(keys[i] & 0xffffffffull);
+}
+
+__global__ void k_dist_bits(const double* __restrict__ dist, int64_t nnz, unsigned long long* __restrict__ bits) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nnz) bits[i] = (unsigned long long)__double_as_longlong(dist[i]);
+}
+
+__global__ void k_bits_dist(const unsigned long long* __restrict__ bits, int64_t nnz, double* __restrict__ dist) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nnz) dist[i] = __longlong_as_double((long long)bits[i]);
+}
+
+// All rows (except self) are members: R2 >= 2^342 or +inf.
+__global__ void k_everything(const float* __restrict__ X, int64_t n, int d, const float* __restrict__ Q, int64_t m,
+                             int64_t self_off, int exclude, double R2, int64_t* __restrict__ idx,
+                             double* __restrict__ dist) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t per = n - (exclude ? 1 : 0);
+    if (t >= m * per) return;
+    const int64_t q = t / per;
+    int64_t row = t % per;
+    if (exclude && row >= self_off + q) ++row;
+    idx[t] = row;
+    if (dist) {
+        const double s = canonical_sq(Q + q * d, X + row * d, d);
+        dist[t] = sqrt(s < R2 ? s : R2);
+    }
+}
+
+__global__ void k_everything_indptr(int64_t m, int64_t per, int64_t* __restrict__ indptr) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q <= m) indptr[q] = q * per;
+}
+
+typedef void (*ScreenFn)(const ScreenArgs);
+
+template <int PREC, int QT>
+static ScreenFn pick_dp_pairs(int dp) {
+    switch (dp) {
+        case 32: return k_screen<PREC, 32, QT, 2, SCREEN_PAIRS>;
+        case 64: return k_screen<PREC, 64, QT, 2, SCREEN_PAIRS>;
+        case 96: return k_screen<PREC, 96, QT, 2, SCREEN_PAIRS>;
+        case 128: return k_screen<PREC, 128, QT, 2, SCREEN_PAIRS>;
+        default: return nullptr;
+    }
+}
+
+static ScreenFn pick_pairs(int prec, int dp) {
+    if (prec == PREC_FP32) return pick_dp_pairs<PREC_FP32, 1>(dp);
+    if (prec == PREC_BF16) return pick_dp_pairs<PREC_BF16, 2>(dp);
+    return pick_dp_pairs<PREC_FP16, 2>(dp);
+}
+
+static inline unsigned blocks_for(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
+
+int radius_query_impl(mlk_index* index, const float* Q, int64_t m, double R2, int64_t q_offset, uint32_t flags,
+                      int64_t* indptr, int64_t* nnz_out, mlk_radius_result** out, hipStream_t stream) {
+    mlk_stats& st = stats();
+    st = mlk_stats{};
+    st.queries = m;
+    mlk_radius_result* res = new (std::nothrow) mlk_radius_result();
+    if (!res) { set_error("host allocation failed"); return MLK_ENOMEM; }
+    struct Guard {
+        mlk_radius_result*& r;
+        bool keep = false;
+        ~Guard() { if (!keep) delete r; }
+    } guard{res};
+    const bool sort_rows = (flags & MLK_SORT_RESULTS) != 0;
+    res->m = m;
+    res->has_dist = (flags & (MLK_NEED_DISTANCE | MLK_SORT_RESULTS)) != 0;
+    const int64_t n = index->n;
+    const int d = index->d;
+    const bool exclude = (flags & MLK_EXCLUDE_SELF) != 0;
+    const float* X = index->X.as<float>();
+
+    if (m == 0) {
+        MLK_CHECK_HIP(hipMemsetAsync(indptr, 0, sizeof(int64_t), stream));
+        MLK_CHECK_HIP(hipStreamSynchronize(stream));
+        *nnz_out = 0;
+        guard.keep = true;
+        *out = res;
+        return MLK_OK;
+    }
+    RadiusConsts rc;
+    rc.R2 = R2;
+    band(d, R2, &rc.f_in, &rc.f_out);
+    if (!r2_key(R2, rc.key)) {
+        const int64_t per = n - (exclude ? 1 : 0);
+        const int64_t total = m * per;
+        MLK_TRY(res->indices.reserve((size_t)std::max<int64_t>(total, 1) * sizeof(int64_t)));
+        if (res->has_dist) MLK_TRY(res->dist.reserve((size_t)std::max<int64_t>(total, 1) * sizeof(double)));
+        hipLaunchKernelGGL(k_everything, dim3(blocks_for(total)), dim3(256), 0, stream, X, n, d, Q, m, q_offset,
+                           (int)exclude, R2, res->indices.as<int64_t>(), res->has_dist ? res->dist.as<double>() : nullptr);
+        hipLaunchKernelGGL(k_everything_indptr, dim3(blocks_for(m + 1)), dim3(256), 0, stream, m, per, indptr);
+        MLK_CHECK_HIP(hipGetLastError());
+        MLK_CHECK_HIP(hipStreamSynchronize(stream));
+        res->nnz = total;
+        *nnz_out = total;
+        guard.keep = true;
+        *out = res;
+        return MLK_OK;
+    }
+
+    const int prec = (int)((flags & MLK_PREC_MASK) >> 4);
+    st.screen_precision = flags & MLK_PREC_MASK;
+    MLK_TRY(prepare_mirror(index, prec, stream));
+    Mirror& mr = index->mirror[prec];
+    const int dp = mr.dp;
+    ScreenFn screen = pick_pairs(prec, dp);
+    const int qt = prec == PREC_FP32 ? 1 : 2;
+    const int BQ = 4 * qt * 32;
+    const int64_t QB = (m + BQ - 1) / BQ;
+    const int64_t m_pad = QB * BQ;
+    const Tuning& tn = tuning();
+    int S = tn.splits;
+    if (!S) {
+        int64_t want = std::max<int64_t>(1, (1024 + QB - 1) / QB);
+        want = std::min<int64_t>(want, std::max<int64_t>(1, n / 128));
+        if (want >= 8) want = (want + 7) / 8 * 8;
+        S = (int)std::min<int64_t>(want, 4096);
+    }
+    st.splits = S;
+
+    DevBuf* ws = index->ws;
+    const size_t eb = elem_bytes(prec);
+    MLK_TRY(ws[0].reserve((size_t)m_pad * std::max(dp, 32) * eb));
+    MLK_TRY(ws[1].reserve((size_t)m_pad * sizeof(double)));
+    MLK_TRY(ws[2].reserve((size_t)m_pad + 64));
+    MLK_TRY(ws[4].reserve((size_t)m_pad * sizeof(float)));
+    MLK_TRY(ws[6].reserve(64));
+    unsigned* dflag = ws[6].as<unsigned>();
+    MLK_CHECK_HIP(hipMemsetAsync(dflag, 0, 16, stream));
+    MLK_TRY(mirror_rows(prec, Q, m, m_pad, d, dp, index->centre.as<float>(), mr.scale, ws[0].ptr, ws[1].as<double>(),
+                        ws[2].as<uint8_t>(), dflag, stream));
+    unsigned q_inexact = 1;
+    MLK_CHECK_HIP(hipMemcpyAsync(&q_inexact, dflag, 4, hipMemcpyDeviceToHost, stream));
+    MLK_CHECK_HIP(hipStreamSynchronize(stream));
+    const Consts c = make_consts(prec, dp, d, mr.exact && q_inexact == 0, mr.scale);
+    MLK_TRY(prepare_bias(index, prec, c.G, stream));
+    const double tau = R2 * c.scale2 * (1.0 + 0x1p-50);
+    float* thr = ws[4].as<float>();
+    hipLaunchKernelGGL(k_radius_thr, dim3(blocks_for(m_pad)), dim3(256), 0, stream, m, m_pad, ws[1].as<double>(),
+                       ws[2].as<uint8_t>(), tau, c, thr);
+    MLK_CHECK_HIP(hipGetLastError());
+
+    // candidate pairs; the buffer grows and the screen reruns on overflow
+    unsigned long long cap = std::max<unsigned long long>((unsigned long long)m * 64, 1ull << 20);
+    if (ws[9].bytes / sizeof(int2) > cap) cap = ws[9].bytes / sizeof(int2);
+    unsigned long long count = 0;
+    unsigned long long* dcount = (unsigned long long*)(dflag + 2);
+    hipEvent_t e0, e1;
+    MLK_CHECK_HIP(hipEventCreate(&e0));
+    MLK_CHECK_HIP(hipEventCreate(&e1));
+    for (int attempt = 0; attempt < 3; ++attempt) {
+        MLK_TRY(ws[9].reserve((size_t)cap * sizeof(int2)));
+        cap = ws[9].bytes / sizeof(int2);
+        MLK_CHECK_HIP(hipMemsetAsync(dcount, 0, sizeof(unsigned long long), stream));
+        MLK_CHECK_HIP(hipEventRecord(e0, stream));
+        if (screen) {
+            ScreenArgs sa{};
+            sa.xs = mr.xs.ptr;
+            sa.bias = mr.bias.as<float>();
+            sa.qs = ws[0].ptr;
+            sa.thr = thr;
+            sa.n = n;
+            sa.split_rows = (n + S - 1) / S;
+            sa.off_lo = 0;
+            sa.off_hi = sa.split_rows;
+            sa.S = S;
+            sa.QB = (int)QB;
+            sa.self_off = q_offset;
+            sa.exclude = exclude;
+            sa.xcd_map = (S % 8) == 0;
+            sa.pairs = ws[9].as<int2>();
+            sa.pair_count = dcount;
+            sa.pair_cap = cap;
+            hipLaunchKernelGGL(screen, dim3((unsigned)(QB * S)), dim3(256), 0, stream, sa);
+            MLK_CHECK_HIP(hipGetLastError());
+        } else {
+            set_error("radius search supports at most 128 features");
+            return MLK_EINVAL;
+        }
+        MLK_CHECK_HIP(hipEventRecord(e1, stream));
+        if (mr.n_unsafe) {
+            hipLaunchKernelGGL(k_unsafe_pairs, dim3(blocks_for(m * mr.n_unsafe)), dim3(256), 0, stream, m,
+                               mr.unsafe.as<int>(), (int)mr.n_unsafe, q_offset, (int)exclude, ws[9].as<int2>(), dcount,
+                               cap);
+            MLK_CHECK_HIP(hipGetLastError());
+        }
+        MLK_CHECK_HIP(hipMemcpyAsync(&count, dcount, sizeof(count), hipMemcpyDeviceToHost, stream));
+        MLK_CHECK_HIP(hipStreamSynchronize(stream));
+        if (count <= cap) break;
+        cap = count + count / 8 + 1024;
+        if (attempt == 2) {
+            set_error("radius candidate buffer kept overflowing");
+            return MLK_ECAPACITY;
+        }
+    }
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, e0, e1) == hipSuccess) st.screen_ms = ms;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    const int64_t np = (int64_t)count;
+    st.candidates = np;
+
+    // sort pairs by (query, row)
+    MLK_TRY(ws[10].reserve((size_t)std::max<int64_t>(np, 1) * 8));   // keys in
+    MLK_TRY(ws[11].reserve((size_t)std::max<int64_t>(np, 1) * 8));   // keys sorted
+    MLK_TRY(ws[12].reserve((size_t)std::max<int64_t>(np, 1) * 8));   // dist
+    MLK_TRY(ws[13].reserve((size_t)std::max<int64_t>(np, 1) + 16));  // member flags
+    unsigned long long* keys = ws[10].as<unsigned long long>();
+    unsigned long long* skeys = ws[11].as<unsigned long long>();
+    if (np) {
+        hipLaunchKernelGGL(k_pair_keys, dim3(blocks_for(np)), dim3(256), 0, stream, ws[9].as<int2>(), np, keys);
+        int qbits = 1;
+        while ((1ll << qbits) < m) ++qbits;
+        size_t tmp_bytes = 0;
+        MLK_CHECK_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp_bytes, keys, skeys, (int)np, 0, 32 + qbits, stream));
+        MLK_TRY(ws[14].reserve(tmp_bytes + 16));
+        MLK_CHECK_HIP(hipcub::DeviceRadixSort::SortKeys(ws[14].ptr, tmp_bytes, keys, skeys, (int)np, 0, 32 + qbits, stream));
+        hipLaunchKernelGGL(k_classify, dim3(blocks_for(np)), dim3(256), 0, stream, skeys, np, X, Q, d, rc,
+                           ws[13].as<uint8_t>(), ws[12].as<double>());
+        MLK_CHECK_HIP(hipGetLastError());
+    }
+    // compact members (order preserved)
+    MLK_TRY(ws[15].reserve(64));
+    int64_t* dnnz = ws[15].as<int64_t>();
+    MLK_CHECK_HIP(hipMemsetAsync(dnnz, 0, sizeof(int64_t), stream));
+    DevBuf mkeys, mdist;
+    MLK_TRY(mkeys.reserve((size_t)std::max<int64_t>(np, 1) * 8));
+    MLK_TRY(mdist.reserve((size_t)std::max<int64_t>(np, 1) * 8));
+    if (np) {
+        size_t tmp_bytes = 0;
+        MLK_CHECK_HIP(hipcub::DeviceSelect::Flagged(nullptr, tmp_bytes, skeys, ws[13].as<uint8_t>(),
+                                                    mkeys.as<unsigned long long>(), dnnz, (int)np, stream));
+        MLK_TRY(ws[14].reserve(tmp_bytes + 16));
+        MLK_CHECK_HIP(hipcub::DeviceSelect::Flagged(ws[14].ptr, tmp_bytes, skeys, ws[13].as<uint8_t>(),
+                                                    mkeys.as<unsigned long long>(), dnnz, (int)np, stream));
+        MLK_CHECK_HIP(hipcub::DeviceSelect::Flagged(nullptr, tmp_bytes, ws[12].as<double>(), ws[13].as<uint8_t>(),
+                                                    mdist.as<double>(), dnnz, (int)np, stream));
+        MLK_TRY(ws[14].reserve(tmp_bytes + 16));
+        MLK_CHECK_HIP(hipcub::DeviceSelect::Flagged(ws[14].ptr, tmp_bytes, ws[12].as<double>(), ws[13].as<uint8_t>(),
+                                                    mdist.as<double>(), dnnz, (int)np, stream));
+    }
+    int64_t nnz = 0;
+    MLK_CHECK_HIP(hipMemcpyAsync(&nnz, dnnz, sizeof(int64_t), hipMemcpyDeviceToHost, stream));
+    MLK_CHECK_HIP(hipStreamSynchronize(stream));
+
+    // CSR row pointers
+    DevBuf counts;
+    MLK_TRY(counts.reserve((size_t)(m + 1) * sizeof(unsigned long long)));
+    MLK_CHECK_HIP(hipMemsetAsync(counts.ptr, 0, (size_t)(m + 1) * sizeof(unsigned long long), stream));
+    if (nnz)
+        hipLaunchKernelGGL(k_row_counts, dim3(blocks_for(nnz)), dim3(256), 0, stream, mkeys.as<unsigned long long>(), nnz,
+                           counts.as<unsigned long long>());
+    {
+        size_t tmp_bytes = 0;
+        unsigned long long* cptr = counts.as<unsigned long long>();
+        unsigned long long* optr = (unsigned long long*)indptr;
+        MLK_CHECK_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, cptr, optr, (int)(m + 1), stream));
+        MLK_TRY(ws[14].reserve(tmp_bytes + 16));
+        MLK_CHECK_HIP(hipcub::DeviceScan::ExclusiveSum(ws[14].ptr, tmp_bytes, cptr, optr, (int)(m + 1), stream));
+    }
+    MLK_TRY(res->indices.reserve((size_t)std::max<int64_t>(nnz, 1) * sizeof(int64_t)));
+    if (nnz)
+        hipLaunchKernelGGL(k_key_to_index, dim3(blocks_for(nnz)), dim3(256), 0, stream, mkeys.as<unsigned long long>(), nnz,
+                           res->indices.as<int64_t>());
+    if (res->has_dist) {
+        MLK_TRY(res->dist.reserve((size_t)std::max<int64_t>(nnz, 1) * sizeof(double)));
+        if (nnz && !sort_rows)
+            MLK_CHECK_HIP(hipMemcpyAsync(res->dist.ptr, mdist.ptr, nnz * sizeof(double), hipMemcpyDeviceToDevice, stream));
+    }
+    if (sort_rows && nnz) {
+        // stable segmented sort of each row by the distance bits (non-negative doubles)
+        DevBuf bits_in, bits_out, idx_out;
+        MLK_TRY(bits_in.reserve((size_t)nnz * 8));
+        MLK_TRY(bits_out.reserve((size_t)nnz * 8));
+        MLK_TRY(idx_out.reserve((size_t)nnz * 8));
+        hipLaunchKernelGGL(k_dist_bits, dim3(blocks_for(nnz)), dim3(256), 0, stream, mdist.as<double>(), nnz,
+                           bits_in.as<unsigned long long>());
+        size_t tmp_bytes = 0;
+        MLK_CHECK_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(
+            nullptr, tmp_bytes, bits_in.as<unsigned long long>(), bits_out.as<unsigned long long>(),
+            res->indices.as<int64_t>(), idx_out.as<int64_t>(), (int)nnz, (int)m, indptr, indptr + 1, 0, 64, stream));
+        MLK_TRY(ws[14].reserve(tmp_bytes + 16));
+        MLK_CHECK_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(
+            ws[14].ptr, tmp_bytes, bits_in.as<unsigned long long>(), bits_out.as<unsigned long long>(),
+            res->indices.as<int64_t>(), idx_out.as<int64_t>(), (int)nnz, (int)m, indptr, indptr + 1, 0, 64, stream));
+        MLK_CHECK_HIP(hipMemcpyAsync(res->indices.ptr, idx_out.ptr, nnz * 8, hipMemcpyDeviceToDevice, stream));
+        hipLaunchKernelGGL(k_bits_dist, dim3(blocks_for(nnz)), dim3(256), 0, stream, bits_out.as<unsigned long long>(),
+                           nnz, res->dist.as<double>());
+        MLK_CHECK_HIP(hipStreamSynchronize(stream));
+    }
+    MLK_CHECK_HIP(hipGetLastError());
+    MLK_CHECK_HIP(hipStreamSynchronize(stream));
+    res->nnz = nnz;
+    *nnz_out = nnz;
+    st.reranked = np;
+    guard.keep = true;
+    *out = res;
+    return MLK_OK;
+}
+
+}  // namespace mlk

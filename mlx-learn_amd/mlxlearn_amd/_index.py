@@ -1,0 +1,283 @@
+"""Device index and the NumPy boundary of the exact neighbour search.
+
+``DeviceNeighborIndex`` is the resident index (torch tensors in, torch tensors
+out), the counterpart of the reference's NumPy-free device API
+``mlxlearn_device.DeviceNeighborIndex`` / ``fit_neighbors``
+(``src/mlxlearn_device/_neighbors.py:243-466``). ``NeighborIndex`` is the
+compatibility boundary used by the estimators, the counterpart of
+``src/mlxlearn/_kernels/distance.py:66-392``: it owns a float32 host copy,
+builds the device index lazily, and converts results to float64 distances and
+int64 indices.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from .exceptions import DeviceError
+
+__all__ = ["DeviceNeighborIndex", "NeighborIndex", "RadiusCSR", "build_index", "gpu_available"]
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def gpu_available() -> bool:
+    try:
+        return bool(_torch().cuda.is_available())
+    except Exception:  # pragma: no cover
+        return False
+
+
+def require_gpu():
+    torch = _torch()
+    if not torch.cuda.is_available():
+        raise DeviceError("no HIP device is available; the exact search runs on an MI355X GPU only")
+    return _lib.load()
+
+
+def _stream():
+    return ctypes.c_void_p(_torch().cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None and t.numel() else ctypes.c_void_p(0)
+
+
+def to_device_f32(A):
+    """A C-contiguous float32 CUDA tensor holding A (copying host arrays)."""
+    torch = _torch()
+    if isinstance(A, torch.Tensor):
+        t = A.detach()
+        if t.dtype != torch.float32:
+            t = t.to(torch.float32)
+        if not t.is_cuda:
+            t = t.cuda()
+        return t.contiguous()
+    arr = np.ascontiguousarray(A, dtype=np.float32)
+    return torch.from_numpy(arr).cuda()
+
+
+def precision_flag(precision: str) -> int:
+    try:
+        return _lib.PRECISIONS[precision]
+    except KeyError:
+        raise ValueError(f"precision must be one of {sorted(_lib.PRECISIONS)}, got {precision!r}") from None
+
+
+class DeviceNeighborIndex:
+    """Exact brute-force index resident in HBM (owned float32 copy of X)."""
+
+    def __init__(self, X):
+        lib = require_gpu()
+        torch = _torch()
+        Xd = to_device_f32(X)
+        if Xd.ndim != 2 or 0 in Xd.shape:
+            raise ValueError("X must be a nonempty two-dimensional matrix")
+        if Xd.shape[0] >= 2**31:
+            raise ValueError("training row count must fit int32 indices")
+        handle = ctypes.c_void_p()
+        _lib.check(lib.mlk_index_create(_ptr(Xd), Xd.shape[0], Xd.shape[1], 0, _stream(), ctypes.byref(handle)),
+                   "mlk_index_create")
+        self._handle = handle
+        self._lib = lib
+        self.n_samples, self.n_features = int(Xd.shape[0]), int(Xd.shape[1])
+        self._device = Xd.device
+        torch.cuda.current_stream().synchronize()
+
+    def close(self):
+        if getattr(self, "_handle", None):
+            self._lib.mlk_index_destroy(self._handle)
+            self._handle = None
+
+    def __del__(self):  # pragma: no cover - interpreter shutdown order
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def query(self, Q, k, *, return_distance=True, exclude_self=False, q_offset=0, precision="float32",
+              squared=False):
+        """(float64 distances | None, int64 indices) as CUDA tensors, ordered by (S, index)."""
+        torch = _torch()
+        Qd = to_device_f32(Q)
+        if Qd.ndim != 2 or Qd.shape[1] != self.n_features:
+            raise ValueError(f"queries must have {self.n_features} features")
+        m, k = int(Qd.shape[0]), int(k)
+        idx = torch.empty((m, k), dtype=torch.int64, device=self._device)
+        dist = torch.empty((m, k), dtype=torch.float64, device=self._device) if return_distance else None
+        flags = precision_flag(precision) | (_lib.EXCLUDE_SELF if exclude_self else 0)
+        flags |= _lib.SQUARED_DIST if squared else 0
+        _lib.check(self._lib.mlk_knn_query(self._handle, _ptr(Qd), m, k, int(q_offset), flags, _ptr(dist), _ptr(idx),
+                                           _stream()), "mlk_knn_query")
+        return dist, idx
+
+    def query_batched(self, batches, k, *, return_distance=True, precision="float32"):
+        """Several query sets answered by one screen launch per round; list of results."""
+        torch = _torch()
+        parts = [to_device_f32(b) for b in batches]
+        sizes = [int(p.shape[0]) for p in parts]
+        offsets = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+        Qd = torch.cat(parts) if parts else torch.empty((0, self.n_features), device=self._device)
+        m = int(offsets[-1])
+        idx = torch.empty((m, k), dtype=torch.int64, device=self._device)
+        dist = torch.empty((m, k), dtype=torch.float64, device=self._device) if return_distance else None
+        _lib.check(self._lib.mlk_knn_query_batched(
+            self._handle, _ptr(Qd), offsets.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), len(parts), int(k),
+            precision_flag(precision), _ptr(dist), _ptr(idx), _stream()), "mlk_knn_query_batched")
+        out = []
+        for b in range(len(parts)):
+            lo, hi = offsets[b], offsets[b + 1]
+            out.append((None if dist is None else dist[lo:hi], idx[lo:hi]))
+        return out
+
+    def radius(self, Q, R2, *, exclude_self=False, q_offset=0, sort_results=False, need_distance=True,
+               precision="float32"):
+        """Exact radius CSR as CUDA tensors: (indptr int64, indices int64, distances float64 | None)."""
+        torch = _torch()
+        Qd = to_device_f32(Q)
+        m = int(Qd.shape[0])
+        indptr = torch.empty(m + 1, dtype=torch.int64, device=self._device)
+        flags = precision_flag(precision) | (_lib.EXCLUDE_SELF if exclude_self else 0)
+        flags |= (_lib.SORT_RESULTS if sort_results else 0) | (_lib.NEED_DISTANCE if need_distance else 0)
+        nnz = ctypes.c_int64()
+        res = ctypes.c_void_p()
+        _lib.check(self._lib.mlk_radius_query(self._handle, _ptr(Qd), m, float(R2), int(q_offset), flags, _ptr(indptr),
+                                              ctypes.byref(nnz), ctypes.byref(res), _stream()), "mlk_radius_query")
+        try:
+            indices = torch.empty(nnz.value, dtype=torch.int64, device=self._device)
+            dist = (torch.empty(nnz.value, dtype=torch.float64, device=self._device)
+                    if (need_distance or sort_results) else None)
+            _lib.check(self._lib.mlk_radius_fill(res, _ptr(indices), _ptr(dist), _stream()), "mlk_radius_fill")
+        finally:
+            self._lib.mlk_radius_free(res)
+        return indptr, indices, (dist if need_distance else None)
+
+
+@dataclass
+class RadiusCSR:
+    """CSR neighbourhoods (reference ``radius_result.RadiusCSR``, radius_result.py:34-55)."""
+
+    indptr: np.ndarray
+    indices: np.ndarray
+    distances: np.ndarray | None
+
+    @property
+    def n_query(self) -> int:
+        return len(self.indptr) - 1
+
+    @property
+    def nnz(self) -> int:
+        return int(self.indptr[-1])
+
+    def to_ragged(self, return_distance=True):
+        """sklearn's object arrays of 1-d int64 / float64 rows (radius_result.py:109-127)."""
+        rows = np.empty(self.n_query, dtype=object)
+        drows = np.empty(self.n_query, dtype=object)
+        for i in range(self.n_query):
+            a, b = self.indptr[i], self.indptr[i + 1]
+            rows[i] = self.indices[a:b]
+            if self.distances is not None:
+                drows[i] = self.distances[a:b]
+        return (drows, rows) if return_distance else rows
+
+
+def validate_radius(radius) -> float:
+    """Restates radius_contract.validate_radius (radius_contract.py:78-98)."""
+    if isinstance(radius, np.ndarray):
+        if radius.ndim != 0:
+            raise TypeError(f"radius must be a scalar, got an array of shape {radius.shape}")
+        radius = radius[()]
+    from numbers import Real
+    if isinstance(radius, (bool, np.bool_)) or not isinstance(radius, (Real, np.floating, np.integer)):
+        raise TypeError(f"radius must be a real number, got {type(radius).__name__}")
+    value = float(radius)
+    if math.isnan(value):
+        raise ValueError("radius must not be NaN")
+    if value < 0:
+        raise ValueError(f"radius must be non-negative, got {value}")
+    return value + 0.0
+
+
+class NeighborIndex:
+    """Owned float32 training copy + lazily built device index (distance.py:66-364)."""
+
+    def __init__(self, X, *, precision="float32"):
+        X = np.array(X, dtype=np.float32, order="C", copy=True)
+        if X.ndim != 2 or 0 in X.shape:
+            raise ValueError("X must be a nonempty two-dimensional matrix")
+        if X.shape[0] >= 2**31:
+            raise ValueError("training row count must fit int32 indices")
+        self.X = X
+        self.precision = precision
+        self._device_index = None
+        self.last_query_backend = None
+
+    @property
+    def n_samples(self):
+        return int(self.X.shape[0])
+
+    @property
+    def n_features(self):
+        return int(self.X.shape[1])
+
+    @property
+    def device_index(self) -> DeviceNeighborIndex:
+        if self._device_index is None:
+            self._device_index = DeviceNeighborIndex(self.X)
+        return self._device_index
+
+    def __getstate__(self):
+        return {"X": self.X, "precision": self.precision}
+
+    def __setstate__(self, state):
+        self.X = state["X"]
+        self.precision = state.get("precision", "float32")
+        self._device_index = None
+        self.last_query_backend = None
+
+    def query(self, Q, k, *, return_distance=True, exclude_self=False, precision=None):
+        """float64 distances and int64 indices at the NumPy boundary (distance.py:242-270)."""
+        Q = np.ascontiguousarray(Q, dtype=np.float32)
+        k = int(k)
+        limit = self.n_samples - int(bool(exclude_self))
+        if k > limit:
+            raise ValueError(f"n_neighbors must be <= {limit}, got {k}")
+        if exclude_self and Q.shape[0] != self.n_samples:
+            raise ValueError("exclude_self requires the query matrix to be the training matrix: "
+                             f"got {Q.shape[0]} query rows and {self.n_samples} training rows.")
+        if Q.shape[0] == 0:
+            return (np.empty((0, k), np.float64) if return_distance else None), np.empty((0, k), np.int64)
+        dist, idx = self.device_index.query(Q, k, return_distance=return_distance, exclude_self=exclude_self,
+                                            precision=precision or self.precision)
+        self.last_query_backend = "gpu"
+        return (None if dist is None else dist.cpu().numpy()), idx.cpu().numpy()
+
+    def query_radius_csr(self, Q, radius, *, need_distance=True, exclude_self=False, sort_results=False,
+                         precision=None) -> RadiusCSR:
+        """Exact membership S <= fl64(radius**2) (distance.py:285-336)."""
+        radius = validate_radius(radius)
+        Q = np.ascontiguousarray(Q, dtype=np.float32)
+        if exclude_self and Q.shape[0] != self.n_samples:
+            raise ValueError("exclude_self requires the query matrix to be the training matrix: "
+                             f"got {Q.shape[0]} query rows and {self.n_samples} training rows.")
+        R2 = radius * radius
+        if Q.shape[0] == 0:
+            return RadiusCSR(np.zeros(1, np.int64), np.empty(0, np.int64),
+                             np.empty(0, np.float64) if need_distance else None)
+        indptr, indices, dist = self.device_index.radius(
+            Q, R2, exclude_self=exclude_self, sort_results=sort_results, need_distance=need_distance,
+            precision=precision or self.precision)
+        self.last_query_backend = "gpu"
+        return RadiusCSR(indptr.cpu().numpy(), indices.cpu().numpy(), None if dist is None else dist.cpu().numpy())
+
+
+def build_index(X, *, precision="float32") -> NeighborIndex:
+    """Own the training storage; the device index is built on the first query (distance.py:366-392)."""
+    return NeighborIndex(X, precision=precision)

@@ -1,0 +1,5 @@
+"""Exact Euclidean neighbour estimators (reference ``src/mlxlearn/neighbors/__init__.py:9-22``)."""
+from ._base import compute_weights
+from ._estimators import KNeighborsClassifier, KNeighborsRegressor, NearestNeighbors
+
+__all__ = ["KNeighborsClassifier", "KNeighborsRegressor", "NearestNeighbors", "compute_weights"]

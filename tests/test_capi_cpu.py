@@ -1,0 +1,150 @@
+"""CPU-side checks of the native library and the host logic (no device compute)."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "mlxlearn_amd.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(mlk_[a-z_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    from mlxlearn_amd import _lib
+    lib = _lib.load()
+    names = declared_symbols()
+    assert len(names) >= 15
+    for name in names:
+        assert hasattr(lib, name), f"{name} declared in include/mlxlearn_amd.h but not exported"
+    assert set(names) == set(_lib.SIGNATURES), "ctypes signatures must cover exactly the header"
+    assert lib.mlk_version() == 1
+
+
+def test_library_validation_without_device():
+    """Argument validation runs before any device call and returns MLK_EINVAL with a message."""
+    import ctypes
+    from mlxlearn_amd import _lib
+    lib = _lib.load()
+    out = ctypes.c_void_p()
+    rc = lib.mlk_index_create(None, 10, 3, 0, None, ctypes.byref(out))
+    assert rc == 1
+    assert b"device pointer" in lib.mlk_last_error()
+    rc = lib.mlk_knn_query(None, None, 0, 1, 0, 0, None, None, None)
+    assert rc == 1 and b"index is NULL" in lib.mlk_last_error()
+    assert lib.mlk_set_tuning(-1, 0, 0, 0, None) == 1
+    with pytest.raises(ValueError):
+        _lib.check(1)
+    with pytest.raises(MemoryError):
+        _lib.check(-1)
+    from mlxlearn_amd.exceptions import BackendExecutionError
+    with pytest.raises(BackendExecutionError):
+        _lib.check(-2)
+
+
+def test_estimators_fail_loudly_without_device():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a device is present")
+    from mlxlearn_amd import NearestNeighbors
+    from mlxlearn_amd.exceptions import DeviceError
+    nn = NearestNeighbors(n_neighbors=2).fit(np.eye(3))
+    with pytest.raises(DeviceError):
+        nn.kneighbors(np.eye(3))
+
+
+@pytest.mark.parametrize(
+    "kwargs, parameter",
+    [({"algorithm": "kd_tree"}, "algorithm"), ({"algorithm": "ball_tree"}, "algorithm"),
+     ({"metric": "manhattan"}, "metric"), ({"metric": "cosine"}, "metric"),
+     ({"metric": "minkowski", "p": 1}, "p"), ({"metric": "minkowski", "p": 3}, "p"),
+     ({"metric_params": {"w": [1.0]}}, "metric_params"), ({"weights": lambda d: d}, "weights")])
+def test_unsupported_parameters_raise_precisely(kwargs, parameter):
+    """tests/parity/test_neighbors.py:308-336."""
+    from mlxlearn_amd import KNeighborsClassifier
+    from mlxlearn_amd.exceptions import UnsupportedParameterError
+    X = np.random.default_rng(0).normal(size=(20, 5))
+    y = np.arange(20) % 2
+    with pytest.raises(UnsupportedParameterError) as excinfo:
+        KNeighborsClassifier(n_neighbors=3, **kwargs).fit(X, y)
+    assert excinfo.value.parameter == parameter
+
+
+def test_sparse_and_bad_inputs():
+    import scipy.sparse as sp
+    from mlxlearn_amd import KNeighborsClassifier, NearestNeighbors
+    from mlxlearn_amd.exceptions import SparseInputError, UnsupportedParameterError
+    X = np.random.default_rng(0).normal(size=(20, 5))
+    y = np.arange(20) % 2
+    with pytest.raises(SparseInputError) as e:
+        KNeighborsClassifier(n_neighbors=3).fit(sp.csr_matrix(X), y)
+    assert isinstance(e.value, TypeError) and "sparse" in str(e.value).lower()
+    with pytest.raises(UnsupportedParameterError, match="single-output"):
+        KNeighborsClassifier(n_neighbors=3).fit(X, np.stack([y, y], 1))
+    Xn = X.copy()
+    Xn[0, 0] = np.nan
+    with pytest.raises(ValueError):
+        NearestNeighbors(n_neighbors=2).fit(Xn)
+    with pytest.raises(ValueError, match="float32"):
+        NearestNeighbors(n_neighbors=2).fit(np.array([[1e300], [1.0], [2.0]]))
+
+
+def test_n_neighbors_messages():
+    """sklearn's messages (reference _base.py:466-488)."""
+    from mlxlearn_amd import NearestNeighbors
+    nn = NearestNeighbors(n_neighbors=2).fit(np.eye(6))
+    with pytest.raises(ValueError, match="Expected n_neighbors > 0. Got 0"):
+        nn.kneighbors(np.eye(6), n_neighbors=0)
+    with pytest.raises(ValueError, match="n_neighbors <= n_samples_fit"):
+        nn.kneighbors(np.eye(6), n_neighbors=7)
+    with pytest.raises(ValueError, match="n_neighbors < n_samples_fit"):
+        nn.kneighbors(None, n_neighbors=6)
+    with pytest.raises(TypeError, match="does not take"):
+        nn.kneighbors(np.eye(6), n_neighbors=2.5)
+
+
+def test_radius_validation_and_csr_shapes():
+    from mlxlearn_amd._index import RadiusCSR, validate_radius
+    assert validate_radius(-0.0) == 0.0
+    with pytest.raises(ValueError):
+        validate_radius(float("nan"))
+    with pytest.raises(ValueError):
+        validate_radius(-1)
+    with pytest.raises(TypeError):
+        validate_radius(True)
+    with pytest.raises(TypeError):
+        validate_radius(np.ones(2))
+    csr = RadiusCSR(np.array([0, 2, 2, 3]), np.array([4, 7, 1]), np.array([0.5, 0.75, 0.0]))
+    d, i = csr.to_ragged()
+    assert [list(r) for r in i] == [[4, 7], [], [1]]
+    assert [list(r) for r in d] == [[0.5, 0.75], [], [0.0]]
+
+
+def test_compute_weights_rules():
+    """reference _base.py:752-785: 1/d, coincident rows become indicators, all-zero rows raise."""
+    from mlxlearn_amd.neighbors import compute_weights
+    assert compute_weights(None, "uniform") is None
+    w = compute_weights(np.array([[1.0, 2.0], [0.0, 3.0]]), "distance")
+    np.testing.assert_array_equal(w, [[1.0, 0.5], [1.0, 0.0]])
+    with pytest.raises(ValueError):
+        compute_weights(np.array([[1.0]]), "bogus")
+
+
+def test_kmeans_parameter_validation():
+    from mlxlearn_amd import KMeans
+    for kw in ({"n_clusters": 0}, {"max_iter": 0}, {"tol": -1}, {"algorithm": "elkan"}, {"init": "bogus"}):
+        with pytest.raises(ValueError):
+            KMeans(**kw)._validate_parameters()
+
+
+def test_pickle_drops_device_state():
+    import pickle
+    from mlxlearn_amd import NearestNeighbors
+    nn = NearestNeighbors(n_neighbors=2).fit(np.eye(4))
+    nn2 = pickle.loads(pickle.dumps(nn))
+    np.testing.assert_array_equal(nn2._fit_X, np.eye(4, dtype=np.float32))
+    assert nn2._index_._device_index is None

@@ -1,0 +1,177 @@
+"""The oracle is pinned before it is trusted.
+
+* against golden vectors written by the reference's own native engines
+  (``tests/golden/reference_native.npz``, made by ``tests/golden/make_golden.py``);
+* against the reference engines themselves when ``oracle/_ref`` is built;
+* against Python big-integer arithmetic (neighbors_cpu.py:39-47, 174-181) on
+  adversarial bit patterns;
+* against closed forms (lattices with known distances, exact ties);
+* with planted mistakes that each check must catch.
+"""
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "reference_native.npz")
+REF_DIR = os.path.join(os.path.dirname(os.path.dirname(__file__)), "oracle", "_ref")
+
+
+@pytest.fixture(scope="module")
+def golden():
+    return np.load(GOLDEN)
+
+
+@pytest.mark.parametrize("name", ["gauss24", "ties8", "offset4_self", "scales3"])
+def test_knn_matches_reference_native_golden(golden, name):
+    X, Q = golden[f"knn_{name}_X"], golden[f"knn_{name}_Q"]
+    k, ex = int(golden[f"knn_{name}_k"]), bool(golden[f"knn_{name}_exclude"])
+    d, i = oracle.knn(X, Q, k, exclude_self=ex)
+    np.testing.assert_array_equal(i, golden[f"knn_{name}_idx"])
+    np.testing.assert_allclose(d, golden[f"knn_{name}_dist"], rtol=1e-15, atol=0)
+
+
+@pytest.mark.parametrize("key", ["uniform6", "int4_self"])
+@pytest.mark.parametrize("sort", [False, True])
+def test_radius_matches_reference_native_golden(golden, key, sort):
+    X, Q = golden[f"rad_{key}_X"], golden[f"rad_{key}_Q"]
+    R2, ex = float(golden[f"rad_{key}_R2"]), bool(golden[f"rad_{key}_exclude"])
+    tag = f"rad_{key}_{'sorted' if sort else 'plain'}"
+    ip, ind, dist = oracle.radius(X, Q, R2, exclude_self=ex, sort_results=sort)
+    np.testing.assert_array_equal(ip, golden[f"{tag}_indptr"])
+    np.testing.assert_array_equal(ind, golden[f"{tag}_indices"])
+    np.testing.assert_array_equal(dist, golden[f"{tag}_dist"])  # canonical sum: bit-identical
+
+
+def _ref_modules():
+    if not os.path.isdir(REF_DIR):
+        pytest.skip("reference native engines not built (oracle/_ref)")
+    sys.path.insert(0, REF_DIR)
+    try:
+        import _neighbors_cpu_native as rn
+        import _radius_cpu_native as rr
+    except ImportError:
+        pytest.skip("reference native engines not importable")
+    return rn, rr
+
+
+def test_knn_matches_reference_engine_fresh(rng):
+    rn, _ = _ref_modules()
+    for d in (1, 5, 17, 32):
+        X = rng.normal(size=(2500, d)).astype(np.float32)
+        X[::97] = X[3]
+        Q = rng.normal(size=(33, d)).astype(np.float32)
+        dist = np.empty((33, 7))
+        idx = np.empty((33, 7), np.int64)
+        rn.query(X, Q, 7, False, 0, dist, idx)
+        np.testing.assert_array_equal(oracle.knn(X, Q, 7)[1], idx)
+
+
+def test_big_integer_keys_across_exponent_range():
+    r = np.random.default_rng(9021)
+    bits = r.integers(0, 2**32, size=(200, 6), dtype=np.uint32)
+    bits = (bits & np.uint32(0x807FFFFF)) | (r.integers(0, 255, size=bits.shape, dtype=np.uint32) << np.uint32(23))
+    arr = bits.view(np.float32)
+    for a, b in zip(arr[:100], arr[100:]):
+        assert oracle.exact_key_c(a, b) == oracle.integer_key(a, b)
+
+
+def test_exact_compare_on_adjacent_floats():
+    q = np.zeros(3, np.float32)
+    a = np.array([1.0, 2.0, 3.0], np.float32)
+    b = a.copy()
+    b[2] = np.nextafter(b[2], np.float32(np.inf))
+    assert oracle.exact_compare(q, a, b) == -1
+    assert oracle.exact_compare(q, b, a) == 1
+    assert oracle.exact_compare(q, a, a.copy()) == 0
+    # a genuine tie with different coordinates: (-1, 0) vs (1, 0) from the origin
+    assert oracle.exact_compare(np.zeros(2, np.float32), np.array([-1, 0], np.float32),
+                                np.array([1, 0], np.float32)) == 0
+
+
+def test_integer_knn_agrees_on_adversarial_rows():
+    r = np.random.default_rng(5)
+    X = (r.normal(size=(80, 4)) * 2.0 ** r.integers(-40, 40, (80, 1))).astype(np.float32)
+    X[10:14] = X[2]
+    Q = X[:12] * np.float32(0.999)
+    np.testing.assert_array_equal(oracle.knn(X, Q, 9)[1], oracle.integer_knn(X, Q, 9))
+
+
+def test_lattice_closed_form():
+    """Points on an integer lattice: S is a known integer; ties break by index."""
+    g = np.stack(np.meshgrid(np.arange(5), np.arange(5), indexing="ij"), -1).reshape(-1, 2).astype(np.float32)
+    d, i = oracle.knn(g, np.array([[2.0, 2.0]], np.float32), 5)
+    np.testing.assert_array_equal(i[0], [12, 7, 11, 13, 17])  # centre, then the four at distance 1 by index
+    np.testing.assert_array_equal(d[0], [0, 1, 1, 1, 1])
+
+
+def test_radius_band_and_exact_membership():
+    # a point exactly on the radius is a member; the next float outside is not
+    q = np.zeros(1, np.float32)
+    x = np.array([3.0], np.float32)
+    assert oracle.exact_le(q, x, 9.0)
+    assert not oracle.exact_le(q, x, math.nextafter(9.0, 0.0))
+    lo, hi = oracle.radius_band(4, 2.0)
+    assert (lo, hi) == oracle.radius_thresholds(4, 2.0)
+    assert lo < 2.0 < hi
+
+
+def test_kmeans_assign_lowest_index_on_ties():
+    """tests/unit/test_e_means.py:115-126: identical centres -> label 0."""
+    r = np.random.default_rng(747)
+    x = r.normal(size=(41, 17)).astype(np.float32)
+    centres = np.tile(x[0], (67, 1))
+    labels, resid = oracle.kmeans_assign(x, centres)
+    np.testing.assert_array_equal(labels, 0)
+    np.testing.assert_allclose(resid, ((x.astype(np.float64) - centres[0]) ** 2).sum(1), rtol=1e-15)
+    assert resid[0] == 0
+
+
+def test_kmeans_update_sums():
+    r = np.random.default_rng(1)
+    X = r.normal(size=(500, 7)).astype(np.float32)
+    labels = r.integers(0, 9, 500)
+    sums, counts = oracle.kmeans_update(X, labels, 9)
+    for c in range(9):
+        np.testing.assert_allclose(sums[c], X[labels == c].astype(np.float64).sum(0), rtol=1e-12)
+    assert counts.sum() == 500
+
+
+def test_planted_mistakes_are_caught(rng):
+    """Each pin above must fail for a plausible oracle bug (dropped term, wrong tie order, sign)."""
+    X = rng.normal(size=(300, 5)).astype(np.float32)
+    X[40:45] = X[4]
+    Q = X[:6].copy()
+    d, i = oracle.knn(X, Q, 6)
+    want = oracle.integer_knn(X, Q, 6)
+    # dropped last feature
+    _, i_drop = oracle.knn(X[:, :4], Q[:, :4], 6)
+    assert not np.array_equal(i_drop, want)
+    # reversed tie order among the duplicate block
+    rev = i.copy()
+    row = 4
+    tied = np.flatnonzero(np.isin(rev[row], np.arange(40, 45)))
+    rev[row, tied] = rev[row, tied][::-1]
+    assert not np.array_equal(rev, want)
+    # sign error in one coordinate
+    _, i_sign = oracle.knn(X * np.array([1, 1, 1, 1, -1], np.float32), Q, 6)
+    assert not np.array_equal(i_sign, want)
+    np.testing.assert_array_equal(i, want)
+
+
+def test_generators_are_seeded_and_shaped():
+    X, Q = oracle.gen_c1(seed=1, n=100, m=10)
+    X2, _ = oracle.gen_c1(seed=1, n=100, m=10)
+    assert X.shape == (100, 64) and Q.shape == (10, 64) and np.array_equal(X, X2)
+    X, Q = oracle.gen_c2(seed=0, n=20_000, m=50)
+    assert X.dtype == np.float32 and X.shape == (20_000, 128)
+    assert len(np.unique(X.view(np.uint8).reshape(20_000, -1), axis=0)) <= 20_000 - 150
+    X, y, Q = oracle.gen_c3(seed=0, n=500, m=20)
+    assert X.shape == (500, 784) and set(np.unique(X)) <= set(range(256))
+    zeros = (X == 0).mean()
+    assert 0.7 < zeros < 0.9
+    assert set(np.unique(y)) <= set(range(10))
