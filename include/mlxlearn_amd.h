@@ -78,9 +78,11 @@ int mlk_set_test_hooks(uint32_t hooks, int64_t param);
 /* Rows (query positions within the next call) whose screen certificate is withdrawn. */
 int mlk_set_forced_rows(const int64_t* rows_host, int64_t count);
 
-/* Tuning overrides (0 = automatic): rows of database splits, rounds schedule. */
-int mlk_set_tuning(int32_t splits, int32_t capacity, int32_t query_tiles, int32_t n_rounds,
-                   const double* round_fractions);
+/* Tuning overrides (0 = automatic): database splits, candidate slots per
+ * sub-list, query tiles per wave (1-2), row groups per block (1-2) and the
+ * cumulative round fractions of each split. Results never depend on them. */
+int mlk_set_tuning(int32_t splits, int32_t capacity, int32_t query_tiles, int32_t row_groups,
+                   int32_t n_rounds, const double* round_fractions);
 
 /*
  * Exact brute-force index over X (n x d float32, row-major, device memory).

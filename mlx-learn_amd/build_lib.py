@@ -23,6 +23,8 @@ FLAGS = [
     # packed float32 VALU forms are disabled for every kernel
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",
+    # MFMA accumulators in VGPRs: the epilogue reads them every tile
+    "-mllvm", "-amdgpu-mfma-vgpr-form=1",
 ]
 
 

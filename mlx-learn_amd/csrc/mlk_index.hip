@@ -107,23 +107,25 @@ __global__ void k_mirror(const float* __restrict__ X, int64_t n, int64_t n_pad, 
     const int lane = threadIdx.x & 63;
     const int64_t row = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (row >= n_pad) return;
+    // pass 1: norm, safe range, exactness; pass 2: the mirror values (zero for
+    // rows outside the safe range, whose screen products must stay finite)
     double s = 0.0;
     bool safe = true, exact = true;
-    for (int j = lane; j < dp; j += 64) {
-        T out = (T)0.0f;
-        if (row < n && j < d) {
-            double v = ((double)X[row * d + j] - (double)c[j]) * scale;
-            s += v * v;
-            safe &= safe_value<PREC>(v);
-            out = to_screen<PREC>(v);
-            exact &= (double)(float)out == v;
-        }
-        xs[row * dp + j] = out;
+    for (int j = lane; j < d && row < n; j += 64) {
+        const double v = ((double)X[row * d + j] - (double)c[j]) * scale;
+        s += v * v;
+        safe &= safe_value<PREC>(v);
+        exact &= (double)(float)to_screen<PREC>(v) == v;
     }
     for (int o = 32; o > 0; o >>= 1) {
         s += __shfl_xor(s, o);
         safe = __shfl_xor((int)safe, o) & safe;
         exact = __shfl_xor((int)exact, o) & exact;
+    }
+    for (int j = lane; j < dp; j += 64) {
+        T out = (T)0.0f;
+        if (row < n && j < d && safe) out = to_screen<PREC>(((double)X[row * d + j] - (double)c[j]) * scale);
+        xs[row * dp + j] = out;
     }
     if (lane == 0) {
         norms[row] = row < n ? s : 0.0;
@@ -132,13 +134,18 @@ __global__ void k_mirror(const float* __restrict__ X, int64_t n, int64_t n_pad, 
     }
 }
 
-// Bias a_x = round_down((1 - G) * nx / 2); unsafe rows get a bias that no
-// threshold reaches.
+// Negated bias -a_x with a_x = round_down((1 - G) * nx / 2) (the screen's
+// accumulator start value); unsafe rows get a start value no threshold reaches.
 __global__ void k_bias(const double* __restrict__ nx, const uint8_t* __restrict__ flags, int64_t n, double G,
                        float* __restrict__ bias) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    bias[i] = flags[i] ? __double2float_rd((1.0 - G) * nx[i] * 0.5) : 3.0e38f;
+    bias[i] = flags[i] ? -__double2float_rd((1.0 - G) * nx[i] * 0.5) : -3.0e38f;
+}
+
+__global__ void k_to_f32(const double* __restrict__ v, int64_t n, float* __restrict__ out) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (float)v[i];
 }
 
 __global__ void k_unsafe_list(const uint8_t* __restrict__ flags, int64_t n, int* __restrict__ list,
@@ -208,6 +215,9 @@ int prepare_mirror(mlk_index* index, int prec, hipStream_t stream) {
     MLK_CHECK_HIP(hipMemsetAsync(dflag, 0, 8, stream));
     MLK_TRY(mirror_rows(prec, index->X.as<float>(), n, n, d, mr.dp, index->centre.as<float>(), scale,
                         mr.xs.ptr, mr.nx.as<double>(), mr.flags.as<uint8_t>(), dflag, stream));
+    MLK_TRY(mr.nx32.reserve((size_t)n * sizeof(float)));
+    hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, mr.nx.as<double>(), n,
+                       mr.nx32.as<float>());
     MLK_TRY(mr.unsafe.reserve((size_t)n * sizeof(int)));
     hipLaunchKernelGGL(k_unsafe_list, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream,
                        mr.flags.as<uint8_t>(), n, mr.unsafe.as<int>(), dflag + 1);

@@ -58,6 +58,7 @@ struct Mirror {
     int64_t n_unsafe = 0;   // rows whose coordinates leave the screen's safe range
     DevBuf xs;              // n x dp screen values
     DevBuf nx;              // n float64 squared norms of scale * (x - c)
+    DevBuf nx32;            // the same norms rounded to float32 (screen hit path)
     DevBuf flags;           // n uint8: bit0 = safe
     DevBuf unsafe;          // n_unsafe int32 row ids
     DevBuf bias;            // n float, for `bias_h`
@@ -69,6 +70,7 @@ struct Tuning {
     int splits = 0;
     int capacity = 0;
     int query_tiles = 0;
+    int row_groups = 0;
     std::vector<double> rounds;
 };
 Tuning& tuning();
@@ -92,7 +94,7 @@ struct mlk_index {
     mlk::Mirror mirror[3];
     std::mutex lock;
     // per-call workspace
-    mlk::DevBuf ws[16];
+    mlk::DevBuf ws[24];
 };
 
 struct mlk_radius_result {

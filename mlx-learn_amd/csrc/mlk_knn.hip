@@ -115,57 +115,80 @@ __global__ void k_tau0(const float* __restrict__ X, int64_t n, int d, const floa
 }
 
 // ---------------------------------------------------------------- merge
-__global__ void k_merge(int64_t m, int k, const int* __restrict__ cnt, const int* __restrict__ cidx,
-                        const float* __restrict__ cacc, int S, int cap, const double* __restrict__ nq,
-                        const double* __restrict__ nx, const float* __restrict__ bias, Consts c,
-                        float* __restrict__ tau, float* __restrict__ thr, int* __restrict__ status) {
+// Incremental merge after a round. Per query, kbest holds the k smallest
+// certified upper bounds over every candidate merged so far; the entries
+// appended since the previous merge (seen .. min(count, cap) of each sub-list)
+// are folded in, the k-th smallest value of the union becomes the new tau
+// (an upper bound of the query's k-th smallest S), and kbest is rewritten with
+// the k smallest values of the union.
+__global__ void k_merge(int64_t m, int k, const int* __restrict__ cnt, int* __restrict__ seen,
+                        const float* __restrict__ cP, const float* __restrict__ cN, int S2, int cap,
+                        float* __restrict__ kbest, const double* __restrict__ nq, Consts c, float* __restrict__ tau,
+                        float* __restrict__ thr, int* __restrict__ status) {
     __shared__ uint32_t sU[4][MCAP];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int64_t q = (int64_t)blockIdx.x * 4 + w;
     if (q >= m) return;
     if (status[q]) return;
-    // totals and per-lane prefix over splits
-    int total = 0;
+    int fresh = 0;
     bool over = false;
-    for (int s = lane; s < S; s += 64) {
-        int cs = cnt[q * S + s];
+    for (int s = lane; s < S2; s += 64) {
+        const int cs = cnt[q * S2 + s];
         over |= cs > cap;
-        total += cs < cap ? cs : cap;
+        fresh += (cs < cap ? cs : cap) - seen[q * S2 + s];
     }
     for (int o = 32; o > 0; o >>= 1) {
-        total += __shfl_xor(total, o);
+        fresh += __shfl_xor(fresh, o);
         over = __shfl_xor((int)over, o) | over;
     }
     if (over) {
         if (lane == 0) { status[q] = ST_OVERFLOW; thr[q] = INFINITY; }
         return;
     }
-    if (total < k || total > MCAP) return;  // threshold stays valid (just not tighter)
-    // compact upper bounds into LDS: lanes take splits, exclusive scan of counts
-    int base = 0;
-    for (int s0 = 0; s0 < S; s0 += 64) {
+    if (fresh == 0 || k + fresh > MCAP) return;  // tau stays valid (just not tighter)
+    uint32_t* buf = sU[w];
+    for (int j = lane; j < k; j += 64) buf[j] = __float_as_uint(kbest[q * k + j]);
+    int base = k;
+    for (int s0 = 0; s0 < S2; s0 += 64) {
         const int s = s0 + lane;
-        int cs = 0;
-        if (s < S) { cs = cnt[q * S + s]; cs = cs < cap ? cs : cap; }
-        int incl = cs;
+        int lo = 0, hi = 0;
+        if (s < S2) {
+            const int cs = cnt[q * S2 + s];
+            hi = cs < cap ? cs : cap;
+            lo = seen[q * S2 + s];
+            seen[q * S2 + s] = hi;
+        }
+        const int mine = hi - lo;
+        int incl = mine;
         for (int o = 1; o < 64; o <<= 1) {
-            int v = __shfl_up(incl, o);
+            const int v = __shfl_up(incl, o);
             if (lane >= o) incl += v;
         }
-        const int at = base + incl - cs;
+        int at = base + incl - mine;
         const double nqq = nq[q];
-        for (int j = 0; j < cs; ++j) {
-            const size_t off = ((size_t)q * S + s) * cap + j;
-            const int row = cidx[off];
-            const double U = upper_bound(cacc[off], bias[row], nqq, nx[row], c);
-            sU[w][at + j] = __float_as_uint(fmaxf(__double2float_ru(U), 0.0f));
+        for (int j = lo; j < hi; ++j) {
+            const size_t off = ((size_t)q * S2 + s) * cap + j;
+            double L, U;
+            hit_bounds(cP[off], cN[off], nqq, c, L, U);
+            buf[at++] = __float_as_uint(fmaxf(__double2float_ru(U), 0.0f));
         }
         base += __shfl(incl, 63);
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    const uint32_t* mine = sU[w];
-    const uint32_t kb = wave_kth_bits(total, k, [&](int j) { return mine[j]; });
+    const int total = base;
+    const uint32_t kb = wave_kth_bits(total, k, [&](int j) { return buf[j]; });
+    // kbest <- every value below the k-th, then copies of the k-th
+    int put = 0;
+    for (int j0 = 0; j0 < total; j0 += 64) {
+        const int j = j0 + lane;
+        const bool below = j < total && buf[j] < kb;
+        const unsigned long long mask = ballot(below);
+        const int pos = put + __popcll(mask & ((1ull << lane) - 1));
+        if (below) kbest[q * k + pos] = __uint_as_float(buf[j]);
+        put += __popcll(mask);
+    }
+    for (int j = put + lane; j < k; j += 64) kbest[q * k + j] = __uint_as_float(kb);
     if (lane == 0) {
         const float t_new = fminf(tau[q], __uint_as_float(kb));
         tau[q] = t_new;
@@ -184,11 +207,10 @@ struct RankArgs {
     // screen candidate lists
     const int* cnt;
     const int* cidx;
-    const float* cacc;
-    int S, cap;
+    const float* cP;
+    const float* cN;
     const double* nq;
-    const double* nx;
-    const float* bias;
+    int S, cap;
     const float* tau;
     Consts c;
     const int* unsafe;
@@ -242,32 +264,39 @@ __global__ __launch_bounds__(256) void k_rank(const RankArgs a, int64_t n_items)
     double* ls = sS[w];
     int count = 0;
     if (a.mode == 0) {
-        const double tau = (double)a.tau[q];
+        const float tau = a.tau[q];
         const double nqq = a.nq[q];
         const int64_t self = a.exclude ? a.self_off + q : -1;
         // Rows whose certified lower bound does not exceed tau, compacted in
         // two passes (count per lane, exclusive scan, place).
         int mine = 0, listed = 0;
+        bool over = false;
         for (int s0 = 0; s0 < a.S; s0 += 64) {
             const int s = s0 + lane;
             int cs = 0;
-            if (s < a.S) { cs = a.cnt[q * a.S + s]; cs = cs < a.cap ? cs : a.cap; }
+            if (s < a.S) {
+                cs = a.cnt[q * a.S + s];
+                over |= cs > a.cap;
+                cs = cs < a.cap ? cs : a.cap;
+            }
             listed += cs;
+            const size_t off = ((size_t)q * a.S + s) * a.cap;
             for (int j = 0; j < cs; ++j) {
-                const size_t off = ((size_t)q * a.S + s) * a.cap + j;
-                const int row = a.cidx[off];
-                mine += lower_bound(a.cacc[off], a.bias[row], nqq, a.nx[row], a.c) <= tau;
+                double L, U;
+                hit_bounds(a.cP[off + j], a.cN[off + j], nqq, a.c, L, U);
+                mine += L <= (double)tau;
             }
         }
         int incl = mine;
         for (int o = 1; o < 64; o <<= 1) {
-            int v = __shfl_up(incl, o);
+            const int v = __shfl_up(incl, o);
             if (lane >= o) incl += v;
         }
-        int total = __shfl(incl, 63);
+        const int total = __shfl(incl, 63);
+        for (int o = 32; o > 0; o >>= 1) over = __shfl_xor((int)over, o) | over;
         int extra = 0;
         for (int u = 0; u < a.n_unsafe; ++u) extra += a.unsafe[u] != self;
-        if (total + extra > LCAP) {
+        if (over || total + extra > LCAP) {
             if (lane == 0) a.status[q] = ST_OVERFLOW;
             return;
         }
@@ -276,10 +305,11 @@ __global__ __launch_bounds__(256) void k_rank(const RankArgs a, int64_t n_items)
             const int s = s0 + lane;
             int cs = 0;
             if (s < a.S) { cs = a.cnt[q * a.S + s]; cs = cs < a.cap ? cs : a.cap; }
+            const size_t off = ((size_t)q * a.S + s) * a.cap;
             for (int j = 0; j < cs; ++j) {
-                const size_t off = ((size_t)q * a.S + s) * a.cap + j;
-                const int row = a.cidx[off];
-                if (lower_bound(a.cacc[off], a.bias[row], nqq, a.nx[row], a.c) <= tau) li[at++] = row;
+                double L, U;
+                hit_bounds(a.cP[off + j], a.cN[off + j], nqq, a.c, L, U);
+                if (L <= (double)tau) li[at++] = a.cidx[off + j];
             }
         }
         if (lane == 0) {
@@ -290,14 +320,27 @@ __global__ __launch_bounds__(256) void k_rank(const RankArgs a, int64_t n_items)
         count = total + extra;
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        // float64 distances, then drop rows provably beyond tau
-        const double cut = tau / a.c.scale2;
+        // float64 distances, 16 lanes per candidate (any summation order: the
+        // interval tests use the order-free bound g)
+        const double cut = (double)tau / a.c.scale2;
         int kept = 0;
-        for (int j = lane; j < count; j += 64) {
-            const double s = canonical_sq(qv, a.X + (size_t)li[j] * d, d);
-            const bool ok = s * (1.0 - g) <= cut;
-            ls[j] = ok ? s : INFINITY;
-            kept += ok;
+        for (int b = 0; b < count; b += 4) {
+            const int j = b + (lane >> 4);
+            const int gl = lane & 15;
+            double sq = 0.0;
+            if (j < count) {
+                const float* xv = a.X + (size_t)li[j] * d;
+                for (int e = gl; e < d; e += 16) {
+                    const double t = (double)qv[e] - (double)xv[e];
+                    sq += t * t;
+                }
+            }
+            for (int o = 8; o > 0; o >>= 1) sq += __shfl_xor(sq, o, 16);
+            if (j < count && gl == 0) {
+                const bool ok = sq * (1.0 - g) <= cut;
+                ls[j] = ok ? sq : INFINITY;
+                kept += ok;
+            }
         }
         for (int o = 32; o > 0; o >>= 1) {
             kept += __shfl_xor(kept, o);
@@ -365,7 +408,7 @@ __global__ __launch_bounds__(256) void k_rank(const RankArgs a, int64_t n_items)
         if (rank < k) {
             a.out_idx[q * k + rank] = ij;
             if (a.out_dist) {
-                const double sc = a.mode == 0 ? sj : canonical_sq(qv, a.X + (size_t)ij * d, d);
+                const double sc = canonical_sq(qv, a.X + (size_t)ij * d, d);
                 a.out_dist[q * k + rank] = a.squared ? sc : sqrt(sc);
             }
         }
@@ -457,6 +500,11 @@ __global__ void k_qprep_status(int64_t m, const int* __restrict__ status, int* _
     }
 }
 
+__global__ void k_fill_f32(float* __restrict__ p, int64_t n, float v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 __global__ void k_force(int64_t m, const int64_t* __restrict__ rows, int64_t count, int* __restrict__ status,
                         float* __restrict__ thr) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -469,21 +517,29 @@ __global__ void k_force(int64_t m, const int64_t* __restrict__ rows, int64_t cou
 // ---------------------------------------------------------------- host
 typedef void (*ScreenFn)(const ScreenArgs);
 
-template <int PREC, int QT>
+template <int PREC, int QT, int RG>
 static ScreenFn pick_dp(int dp) {
     switch (dp) {
-        case 32: return k_screen<PREC, 32, QT, 2, SCREEN_LISTS>;
-        case 64: return k_screen<PREC, 64, QT, 2, SCREEN_LISTS>;
-        case 96: return k_screen<PREC, 96, QT, 2, SCREEN_LISTS>;
-        case 128: return k_screen<PREC, 128, QT, 2, SCREEN_LISTS>;
+        case 32: return k_screen<PREC, 32, QT, RG, SCREEN_LISTS>;
+        case 64: return k_screen<PREC, 64, QT, RG, SCREEN_LISTS>;
+        case 96: return k_screen<PREC, 96, QT, RG, SCREEN_LISTS>;
+        case 128: return k_screen<PREC, 128, QT, RG, SCREEN_LISTS>;
         default: return nullptr;
     }
 }
 
-static ScreenFn pick_screen(int prec, int dp, int qt) {
-    if (prec == PREC_FP32) return qt == 1 ? pick_dp<PREC_FP32, 1>(dp) : pick_dp<PREC_FP32, 2>(dp);
-    if (prec == PREC_BF16) return qt == 1 ? pick_dp<PREC_BF16, 1>(dp) : pick_dp<PREC_BF16, 2>(dp);
-    return qt == 1 ? pick_dp<PREC_FP16, 1>(dp) : pick_dp<PREC_FP16, 2>(dp);
+template <int PREC>
+static ScreenFn pick_variant(int dp, int qt, int rg) {
+    if (qt == 1) return pick_dp<PREC, 1, 1>(dp);
+    if (rg == 1) return pick_dp<PREC, 2, 1>(dp);
+    return pick_dp<PREC, 2, 2>(dp);
+}
+
+// Screen variants: (query tiles per wave, row groups) in {(1,1), (2,1), (2,2)}.
+static ScreenFn pick_screen(int prec, int dp, int qt, int rg) {
+    if (prec == PREC_FP32) return pick_variant<PREC_FP32>(dp, qt, rg);
+    if (prec == PREC_BF16) return pick_variant<PREC_BF16>(dp, qt, rg);
+    return pick_variant<PREC_FP16>(dp, qt, rg);
 }
 
 static double unit_roundoff(int prec) {
@@ -507,21 +563,24 @@ Consts make_consts(int prec, int dp, int d, bool exact_inputs, double scale) {
 }
 
 struct Plan {
-    int qt, splits, cap;
+    int qt, rg, splits, cap;
     std::vector<double> rounds;  // cumulative fractions of each split, ending at 1
 };
 
 static Plan make_plan(int prec, int dp, int64_t n, int64_t m, int k) {
     Plan p;
     const Tuning& t = tuning();
-    p.qt = t.query_tiles ? t.query_tiles : 2;
+    // measured default: one query tile per wave in 4-wave blocks (highest
+    // occupancy) for every precision; the other variants stay selectable
+    p.qt = t.query_tiles ? t.query_tiles : 1;
+    p.rg = p.qt == 1 ? 1 : (t.row_groups ? t.row_groups : 2);
     const int64_t BQ = 4 * p.qt * 32;
     const int64_t qb = (m + BQ - 1) / BQ;
     if (t.splits) {
         p.splits = t.splits;
     } else {
-        // about 4 blocks per CU, at least 128 rows per split, multiple of 8
-        int64_t want = (1024 + qb - 1) / qb;
+        // about 16 blocks per CU, at least 128 rows per split, multiple of 8
+        int64_t want = (4096 + qb - 1) / qb;
         int64_t most = std::max<int64_t>(1, n / 128);
         want = std::min(want, most);
         want = std::max<int64_t>(want, 1);
@@ -531,7 +590,8 @@ static Plan make_plan(int prec, int dp, int64_t n, int64_t m, int k) {
     // slots per (query, split, lane half) sub-list
     // (at least 6k slots per query over its 2 * splits sub-lists)
     const int base = prec == PREC_FP32 ? 24 : 48;
-    const int need = (6 * k + 2 * p.splits - 1) / (2 * p.splits) + 4;
+    const int lists = 2 * p.rg * p.splits;
+    const int need = (6 * k + lists - 1) / lists + 4;
     p.cap = t.capacity ? t.capacity : std::max(base, need);
     if (!t.rounds.empty()) {
         p.rounds = t.rounds;
@@ -571,7 +631,7 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
     const int dp = mr.dp;
     ScreenFn screen = nullptr;
     Plan plan = make_plan(prec, dp, n, m, k);
-    screen = pick_screen(prec, dp, plan.qt);
+    screen = pick_screen(prec, dp, plan.qt, plan.rg);
     const bool all_fallback = !screen || k > 64 || mr.n_unsafe > 64 || (hooks().hooks & MLK_HOOK_FORCE_FALLBACK);
     if (hooks().hooks & MLK_HOOK_CAPACITY) plan.cap = (int)std::max<int64_t>(1, hooks().param);
 
@@ -625,19 +685,27 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
     MLK_CHECK_HIP(hipEventRecord(ev1, stream));
     std::vector<hipEvent_t> round_events;
     if (!all_fallback) {
-        const int S2 = 2 * S;  // candidate sub-lists per query (split x lane half)
-        MLK_TRY(ws[8].reserve((size_t)m_pad * S2 * sizeof(int)));
-        MLK_TRY(ws[9].reserve((size_t)m_pad * S2 * cap * sizeof(int)));
-        MLK_TRY(ws[10].reserve((size_t)m_pad * S2 * cap * sizeof(float)));
+        const int S2 = 2 * plan.rg * S;  // candidate sub-lists per query (split x row group x lane half)
+        MLK_TRY(ws[8].reserve((size_t)m_pad * S2 * sizeof(int)));          // counts
+        MLK_TRY(ws[9].reserve((size_t)m_pad * S2 * cap * sizeof(int)));    // rows
+        MLK_TRY(ws[10].reserve((size_t)m_pad * S2 * cap * sizeof(float))); // dot-product estimates
+        MLK_TRY(ws[16].reserve((size_t)m_pad * S2 * cap * sizeof(float))); // row norms
+        MLK_TRY(ws[17].reserve((size_t)m_pad * S2 * sizeof(int)));         // merged counts
+        MLK_TRY(ws[18].reserve((size_t)m_pad * k * sizeof(float)));        // k smallest upper bounds
         MLK_CHECK_HIP(hipMemsetAsync(ws[8].ptr, 0, (size_t)m_pad * S2 * sizeof(int), stream));
-        ScreenArgs sa;
+        MLK_CHECK_HIP(hipMemsetAsync(ws[17].ptr, 0, (size_t)m_pad * S2 * sizeof(int), stream));
+        hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)((m_pad * k + 255) / 256)), dim3(256), 0, stream,
+                           ws[18].as<float>(), (int64_t)(m_pad * k), INFINITY);
+        ScreenArgs sa{};
         sa.xs = mr.xs.ptr;
         sa.bias = mr.bias.as<float>();
+        sa.nx32 = mr.nx32.as<float>();
         sa.qs = ws[0].ptr;
         sa.thr = thr;
         sa.cnt = ws[8].as<int>();
         sa.cidx = ws[9].as<int>();
-        sa.cacc = ws[10].as<float>();
+        sa.cP = ws[10].as<float>();
+        sa.cN = ws[16].as<float>();
         sa.n = n;
         sa.split_rows = (n + S - 1) / S;
         sa.S = S;
@@ -661,28 +729,30 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
             round_ev.push_back(ea);
             round_ev.push_back(eb);
             MLK_CHECK_HIP(hipEventRecord(ea, stream));
-            hipLaunchKernelGGL(screen, dim3(blocks), dim3(256), 0, stream, sa);
+            hipLaunchKernelGGL(screen, dim3(blocks), dim3(256 * plan.rg), 0, stream, sa);
             MLK_CHECK_HIP(hipGetLastError());
             MLK_CHECK_HIP(hipEventRecord(eb, stream));
             ++st.rounds;
             if (r + 1 < plan.rounds.size()) {
                 hipLaunchKernelGGL(k_merge, dim3((unsigned)((m + 3) / 4)), dim3(256), 0, stream, m, k, sa.cnt,
-                                   sa.cidx, sa.cacc, S2, cap, ws[1].as<double>(), mr.nx.as<double>(), sa.bias, c, tau,
-                                   thr, status);
+                                   ws[17].as<int>(), sa.cP, sa.cN, S2, cap, ws[18].as<float>(), ws[1].as<double>(), c, tau, thr,
+                                   status);
                 MLK_CHECK_HIP(hipGetLastError());
             }
         }
         round_events = std::move(round_ev);
         // final overflow check (the last round has no merge)
-        hipLaunchKernelGGL(k_merge, dim3((unsigned)((m + 3) / 4)), dim3(256), 0, stream, m, k, sa.cnt, sa.cidx,
-                           sa.cacc, S2, cap, ws[1].as<double>(), mr.nx.as<double>(), sa.bias, c, tau, thr, status);
+        hipLaunchKernelGGL(k_merge, dim3((unsigned)((m + 3) / 4)), dim3(256), 0, stream, m, k, sa.cnt,
+                           ws[17].as<int>(), sa.cP, sa.cN, S2, cap, ws[18].as<float>(), ws[1].as<double>(), c, tau, thr,
+                                   status);
         MLK_CHECK_HIP(hipGetLastError());
         MLK_CHECK_HIP(hipEventRecord(ev2, stream));
 
         RankArgs ra{};
         ra.X = X; ra.n = n; ra.d = d; ra.Q = Q; ra.m = m; ra.k = k;
-        ra.cnt = sa.cnt; ra.cidx = sa.cidx; ra.cacc = sa.cacc; ra.S = S2; ra.cap = cap;
-        ra.nq = ws[1].as<double>(); ra.nx = mr.nx.as<double>(); ra.bias = sa.bias; ra.tau = tau; ra.c = c;
+        ra.cnt = sa.cnt; ra.cidx = sa.cidx; ra.cP = sa.cP; ra.cN = sa.cN; ra.nq = ws[1].as<double>();
+        ra.S = S2; ra.cap = cap;
+        ra.tau = tau; ra.c = c;
         ra.unsafe = mr.unsafe.as<int>(); ra.n_unsafe = (int)mr.n_unsafe;
         ra.mode = 0; ra.self_off = q_offset; ra.exclude = exclude; ra.status = status;
         ra.out_idx = idx; ra.out_dist = dist; ra.squared = (flags & MLK_SQUARED_DIST) != 0;

@@ -47,7 +47,7 @@ SIGNATURES = {
     "mlk_get_stats": ([ctypes.POINTER(Stats)], ctypes.c_int),
     "mlk_set_test_hooks": ([_u32, _i64], ctypes.c_int),
     "mlk_set_forced_rows": ([_i64p, _i64], ctypes.c_int),
-    "mlk_set_tuning": ([_i32, _i32, _i32, _i32, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    "mlk_set_tuning": ([_i32, _i32, _i32, _i32, _i32, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "mlk_index_create": ([_vp, _i64, _i32, _u32, _vp, ctypes.POINTER(_vp)], ctypes.c_int),
     "mlk_index_destroy": ([_vp], ctypes.c_int),
     "mlk_index_info": ([_vp, _i64p, ctypes.POINTER(_i32), ctypes.POINTER(_i32)], ctypes.c_int),
@@ -110,7 +110,8 @@ def set_forced_rows(rows) -> None:
     check(load().mlk_set_forced_rows(arr.ctypes.data_as(_i64p), arr.size), "mlk_set_forced_rows")
 
 
-def set_tuning(splits: int = 0, capacity: int = 0, query_tiles: int = 0, rounds=None) -> None:
+def set_tuning(splits: int = 0, capacity: int = 0, query_tiles: int = 0, row_groups: int = 0, rounds=None) -> None:
     rounds = list(rounds or [])
     arr = (ctypes.c_double * max(len(rounds), 1))(*rounds)
-    check(load().mlk_set_tuning(int(splits), int(capacity), int(query_tiles), len(rounds), arr), "mlk_set_tuning")
+    check(load().mlk_set_tuning(int(splits), int(capacity), int(query_tiles), int(row_groups), len(rounds), arr),
+          "mlk_set_tuning")
