@@ -22,11 +22,14 @@ int create_index(const float* X, int64_t n, int d, hipStream_t stream, mlk_index
 
 static inline unsigned nblk(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
 
-__global__ void k_labels_u32(const int64_t* __restrict__ labels, int64_t n, unsigned* __restrict__ keys,
+// Labels outside [0, kc) are counted in counts[kc] and reported as an input
+// error by the host; their rows go to the sentinel key kc and are not summed.
+__global__ void k_labels_u32(const int64_t* __restrict__ labels, int64_t n, int64_t kc, unsigned* __restrict__ keys,
                              int* __restrict__ rows, unsigned long long* __restrict__ counts) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const unsigned l = (unsigned)labels[i];
+    const int64_t raw = labels[i];
+    const unsigned l = (raw >= 0 && raw < kc) ? (unsigned)raw : (unsigned)kc;
     keys[i] = l;
     rows[i] = (int)i;
     atomicAdd(&counts[l], 1ull);
@@ -152,10 +155,17 @@ extern "C" int mlk_kmeans_update(const float* X_dev, int64_t n, int32_t d, const
     MLK_TRY(off.reserve((size_t)(kc + 1) * 8));
     MLK_CHECK_HIP(hipMemsetAsync(cnt.ptr, 0, (size_t)(kc + 1) * 8, stream));
     if (n) {
-        hipLaunchKernelGGL(k_labels_u32, dim3(nblk(n)), dim3(256), 0, stream, labels_dev, n, keys.as<unsigned>(),
+        hipLaunchKernelGGL(k_labels_u32, dim3(nblk(n)), dim3(256), 0, stream, labels_dev, n, kc, keys.as<unsigned>(),
                            rows.as<int>(), cnt.as<unsigned long long>());
+        unsigned long long bad = 0;
+        MLK_CHECK_HIP(hipMemcpyAsync(&bad, cnt.as<unsigned long long>() + kc, sizeof(bad), hipMemcpyDeviceToHost, stream));
+        MLK_CHECK_HIP(hipStreamSynchronize(stream));
+        if (bad) {
+            set_error("kmeans_update: " + std::to_string(bad) + " labels outside [0, n_clusters)");
+            return MLK_EINVAL;
+        }
         int bits = 1;
-        while ((1ll << bits) < kc) ++bits;
+        while ((1ll << bits) <= kc) ++bits;
         size_t tb = 0;
         MLK_CHECK_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, keys.as<unsigned>(), keys2.as<unsigned>(),
                                                          rows.as<int>(), rows2.as<int>(), (int)n, 0, bits, stream));

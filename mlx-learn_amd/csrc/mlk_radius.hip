@@ -77,8 +77,10 @@ __global__ void k_radius_thr(int64_t m, int64_t m_pad, const double* __restrict_
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= m_pad) return;
     if (q >= m) { thr[q] = INFINITY; return; }
-    // unsafe query: every row is a candidate
-    thr[q] = qflags[q] ? thr_from_tau(tau, nq[q], c) : -INFINITY;
+    // Unsafe query (screen values zero): every safe row is a candidate. A safe
+    // row's accumulator is -a_x > -2^120, an unsafe row's is -3e38, so -1e38
+    // passes exactly the safe rows; k_unsafe_pairs adds the unsafe ones once.
+    thr[q] = qflags[q] ? thr_from_tau(tau, nq[q], c) : -1e38f;
 }
 
 // Unsafe database rows never hit the screen; they are candidates of every query.
@@ -183,6 +185,35 @@ static ScreenFn pick_pairs(int prec, int dp) {
 
 static inline unsigned blocks_for(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
 
+// Stable segmented sort of every CSR row by its distance bits (non-negative
+// doubles order like their bit patterns); rows arrive in ascending index
+// order, so equal distances stay in index order.
+static int sort_rows_by_distance(mlk_radius_result* res, const int64_t* indptr, int64_t m, int64_t nnz,
+                                 DevBuf& tmp, hipStream_t stream) {
+    if (!nnz) return MLK_OK;
+    DevBuf bits_in, bits_out, idx_out;
+    MLK_TRY(bits_in.reserve((size_t)nnz * 8));
+    MLK_TRY(bits_out.reserve((size_t)nnz * 8));
+    MLK_TRY(idx_out.reserve((size_t)nnz * 8));
+    hipLaunchKernelGGL(k_dist_bits, dim3(blocks_for(nnz)), dim3(256), 0, stream, res->dist.as<double>(), nnz,
+                       bits_in.as<unsigned long long>());
+    size_t tmp_bytes = 0;
+    MLK_CHECK_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(
+        nullptr, tmp_bytes, bits_in.as<unsigned long long>(), bits_out.as<unsigned long long>(),
+        res->indices.as<int64_t>(), idx_out.as<int64_t>(), (int)nnz, (int)m, indptr, indptr + 1, 0, 64, stream));
+    MLK_TRY(tmp.reserve(tmp_bytes + 16));
+    MLK_CHECK_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(
+        tmp.ptr, tmp_bytes, bits_in.as<unsigned long long>(), bits_out.as<unsigned long long>(),
+        res->indices.as<int64_t>(), idx_out.as<int64_t>(), (int)nnz, (int)m, indptr, indptr + 1, 0, 64, stream));
+    MLK_CHECK_HIP(hipMemcpyAsync(res->indices.ptr, idx_out.ptr, nnz * 8, hipMemcpyDeviceToDevice, stream));
+    hipLaunchKernelGGL(k_bits_dist, dim3(blocks_for(nnz)), dim3(256), 0, stream, bits_out.as<unsigned long long>(),
+                       nnz, res->dist.as<double>());
+    MLK_CHECK_HIP(hipGetLastError());
+    // the temporaries are freed on return: wait for the kernels that use them
+    MLK_CHECK_HIP(hipStreamSynchronize(stream));
+    return MLK_OK;
+}
+
 int radius_query_impl(mlk_index* index, const float* Q, int64_t m, double R2, int64_t q_offset, uint32_t flags,
                       int64_t* indptr, int64_t* nnz_out, mlk_radius_result** out, hipStream_t stream) {
     mlk_stats& st = stats();
@@ -223,6 +254,7 @@ int radius_query_impl(mlk_index* index, const float* Q, int64_t m, double R2, in
                            (int)exclude, R2, res->indices.as<int64_t>(), res->has_dist ? res->dist.as<double>() : nullptr);
         hipLaunchKernelGGL(k_everything_indptr, dim3(blocks_for(m + 1)), dim3(256), 0, stream, m, per, indptr);
         MLK_CHECK_HIP(hipGetLastError());
+        if (sort_rows) MLK_TRY(sort_rows_by_distance(res, indptr, m, total, index->ws[14], stream));
         MLK_CHECK_HIP(hipStreamSynchronize(stream));
         res->nnz = total;
         *nnz_out = total;
@@ -399,30 +431,10 @@ int radius_query_impl(mlk_index* index, const float* Q, int64_t m, double R2, in
                            res->indices.as<int64_t>());
     if (res->has_dist) {
         MLK_TRY(res->dist.reserve((size_t)std::max<int64_t>(nnz, 1) * sizeof(double)));
-        if (nnz && !sort_rows)
+        if (nnz)
             MLK_CHECK_HIP(hipMemcpyAsync(res->dist.ptr, mdist.ptr, nnz * sizeof(double), hipMemcpyDeviceToDevice, stream));
     }
-    if (sort_rows && nnz) {
-        // stable segmented sort of each row by the distance bits (non-negative doubles)
-        DevBuf bits_in, bits_out, idx_out;
-        MLK_TRY(bits_in.reserve((size_t)nnz * 8));
-        MLK_TRY(bits_out.reserve((size_t)nnz * 8));
-        MLK_TRY(idx_out.reserve((size_t)nnz * 8));
-        hipLaunchKernelGGL(k_dist_bits, dim3(blocks_for(nnz)), dim3(256), 0, stream, mdist.as<double>(), nnz,
-                           bits_in.as<unsigned long long>());
-        size_t tmp_bytes = 0;
-        MLK_CHECK_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(
-            nullptr, tmp_bytes, bits_in.as<unsigned long long>(), bits_out.as<unsigned long long>(),
-            res->indices.as<int64_t>(), idx_out.as<int64_t>(), (int)nnz, (int)m, indptr, indptr + 1, 0, 64, stream));
-        MLK_TRY(ws[14].reserve(tmp_bytes + 16));
-        MLK_CHECK_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(
-            ws[14].ptr, tmp_bytes, bits_in.as<unsigned long long>(), bits_out.as<unsigned long long>(),
-            res->indices.as<int64_t>(), idx_out.as<int64_t>(), (int)nnz, (int)m, indptr, indptr + 1, 0, 64, stream));
-        MLK_CHECK_HIP(hipMemcpyAsync(res->indices.ptr, idx_out.ptr, nnz * 8, hipMemcpyDeviceToDevice, stream));
-        hipLaunchKernelGGL(k_bits_dist, dim3(blocks_for(nnz)), dim3(256), 0, stream, bits_out.as<unsigned long long>(),
-                           nnz, res->dist.as<double>());
-        MLK_CHECK_HIP(hipStreamSynchronize(stream));
-    }
+    if (sort_rows) MLK_TRY(sort_rows_by_distance(res, indptr, m, nnz, ws[14], stream));
     MLK_CHECK_HIP(hipGetLastError());
     MLK_CHECK_HIP(hipStreamSynchronize(stream));
     res->nnz = nnz;

@@ -1,0 +1,137 @@
+"""Radius neighbourhoods on the GPU vs the exact oracle, through the C ABI.
+
+Membership is exact (S <= fl64(r*r)), so the CSR row pointers and column
+indices must be bit-identical to the oracle's; distances are
+sqrt(min(s, R2)) of the same canonical float64 sum (rtol 1e-12).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+import radius_cases
+
+pytestmark = pytest.mark.gpu
+
+PRECISIONS = ["float32", "bfloat16", "float16"]
+CASES = radius_cases.all_cases()
+
+
+@pytest.fixture(scope="module")
+def mlk():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import mlxlearn_amd
+    from mlxlearn_amd import _lib
+    _lib.load()
+    return mlxlearn_amd
+
+
+def gpu_radius(mlk, X, Q, R2, precision="float32", **kw):
+    index = mlk.DeviceNeighborIndex(X)
+    try:
+        indptr, indices, dist = index.radius(Q, R2, precision=precision, **kw)
+        return (indptr.cpu().numpy(), indices.cpu().numpy(), None if dist is None else dist.cpu().numpy())
+    finally:
+        index.close()
+
+
+def assert_csr(got, want, case=""):
+    gp, gi, gd = got
+    wp, wi, wd = want
+    np.testing.assert_array_equal(gp, wp, err_msg=f"{case}: row pointers")
+    if not np.array_equal(gi, wi):
+        rows = np.flatnonzero([not np.array_equal(gi[gp[r]:gp[r + 1]], wi[wp[r]:wp[r + 1]]) for r in range(len(wp) - 1)])
+        raise AssertionError(f"{case}: {rows.size} rows differ, first {rows[:5]}")
+    if wd is not None:
+        assert gd is not None
+        np.testing.assert_allclose(gd, wd, rtol=1e-12, atol=0, err_msg=f"{case}: distances")
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize("case", CASES, ids=[c.name for c in CASES])
+def test_adversarial_cases(mlk, case, precision):
+    """tests/_radius_oracle.py families, unsorted and sorted rows (radius_result.py:130-147)."""
+    X = np.ascontiguousarray(case.X, np.float32)
+    Q = np.ascontiguousarray(case.Q, np.float32)
+    R2 = float(case.radius) * float(case.radius)
+    want = oracle.radius(X, Q, R2)
+    assert_csr(gpu_radius(mlk, X, Q, R2, precision), want, case.name)
+    want_sorted = oracle.radius(X, Q, R2, sort_results=True)
+    assert_csr(gpu_radius(mlk, X, Q, R2, precision, sort_results=True), want_sorted, case.name + " sorted")
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_exclude_self_and_no_distances(mlk, precision, rng):
+    X = rng.normal(size=(3000, 12)).astype(np.float32)
+    X[100:110] = X[99]
+    R2 = 9.0
+    want = oracle.radius(X, X, R2, exclude_self=True)
+    got = gpu_radius(mlk, X, X, R2, precision, exclude_self=True)
+    assert_csr(got, want)
+    gp, gi, _ = got
+    rows = np.repeat(np.arange(3000), np.diff(gp))
+    assert not np.any(gi == rows)
+    got = gpu_radius(mlk, X, X[:500], R2, precision, need_distance=False)
+    assert got[2] is None
+    assert_csr(got, oracle.radius(X, X[:500], R2, need_distance=False))
+
+
+def test_empty_queries_and_offsets(mlk, rng):
+    X = rng.normal(size=(500, 5)).astype(np.float32)
+    p, i, d = gpu_radius(mlk, X, np.empty((0, 5), np.float32), 1.0)
+    np.testing.assert_array_equal(p, [0])
+    assert i.size == 0
+    Q = X[200:260]
+    want = oracle.radius(X, Q, 2.0, exclude_self=True, q_offset=200)
+    assert_csr(gpu_radius(mlk, X, Q, 2.0, exclude_self=True, q_offset=200), want)
+
+
+def test_candidate_buffer_regrows(mlk, rng):
+    """More than 2^20 candidate pairs: the pair buffer is regrown and the screen rerun."""
+    X = rng.normal(size=(24_000, 8)).astype(np.float32)
+    Q = rng.normal(size=(200, 8)).astype(np.float32)
+    R2 = 40.0
+    want = oracle.radius(X, Q, R2)
+    assert want[0][-1] > (1 << 20)
+    for p in PRECISIONS:
+        assert_csr(gpu_radius(mlk, X, Q, R2, p), want)
+
+
+def test_clumped_workload_batches(mlk):
+    """Workload c4 generator at reduced size: uniform cube plus 20% tight clumps, calibrated radius."""
+    X, Q, sizes, radius = oracle.gen_c4(seed=7, n=200_000, n_batches=6, max_batch=256)
+    R2 = radius * radius
+    index = mlk.DeviceNeighborIndex(X)
+    try:
+        start = 0
+        for s in sizes:
+            q = Q[start:start + s]
+            start += s
+            indptr, indices, dist = index.radius(q, R2, sort_results=True)
+            assert_csr((indptr.cpu().numpy(), indices.cpu().numpy(), dist.cpu().numpy()),
+                       oracle.radius(X, q, R2, sort_results=True))
+    finally:
+        index.close()
+
+
+def test_estimator_radius_neighbors(mlk, rng):
+    """radius_neighbors / radius_neighbors_graph (reference _base.py:634-675)."""
+    X = rng.normal(size=(2000, 6))
+    Q = rng.normal(size=(40, 6))
+    nn = mlk.NearestNeighbors(radius=1.5).fit(X)
+    d, i = nn.radius_neighbors(Q, sort_results=True)
+    wp, wi, wd = oracle.radius(X.astype(np.float32), Q.astype(np.float32), 1.5 * 1.5, sort_results=True)
+    for r in range(40):
+        np.testing.assert_array_equal(i[r], wi[wp[r]:wp[r + 1]])
+        np.testing.assert_allclose(d[r], wd[wp[r]:wp[r + 1]], rtol=1e-12)
+        assert i[r].dtype == np.int64 and d[r].dtype == np.float64
+    g = nn.radius_neighbors_graph(Q, mode="distance")
+    assert g.shape == (40, 2000) and g.nnz == wp[-1]
+    i_self = nn.radius_neighbors(return_distance=False)
+    wp, wi, _ = oracle.radius(X.astype(np.float32), X.astype(np.float32), 2.25, exclude_self=True)
+    for r in (0, 7, 1999):
+        np.testing.assert_array_equal(i_self[r], wi[wp[r]:wp[r + 1]])
+    assert math.isclose(float(np.mean([len(v) for v in i_self])), (wp[-1]) / 2000)

@@ -175,3 +175,37 @@ def test_generators_are_seeded_and_shaped():
     zeros = (X == 0).mean()
     assert 0.7 < zeros < 0.9
     assert set(np.unique(y)) <= set(range(10))
+
+
+def _small_radius_cases():
+    import radius_cases
+    return [c for c in radius_cases.all_cases() if c.X.shape[0] * c.Q.shape[0] <= 60_000]
+
+
+@pytest.mark.parametrize("case", _small_radius_cases(), ids=lambda c: c.name)
+def test_radius_oracle_matches_integer_brute_force(case):
+    """Membership S * 2^298 <= floor(R2 * 2^298) with Python integers
+    (radius_contract.py:169-217), ascending index rows, (distance, index) when sorted."""
+    from fractions import Fraction
+    X = np.ascontiguousarray(case.X, np.float32)
+    Q = np.ascontiguousarray(case.Q, np.float32)
+    R2 = float(case.radius) * float(case.radius)
+    limit = math.floor(Fraction(R2) * 2 ** 298) if math.isfinite(R2) else None
+    xs = [oracle.integer_row(x) for x in X]
+    indptr, indices, dist = oracle.radius(X, Q, R2)
+    for r, q in enumerate(Q):
+        qi = oracle.integer_row(q)
+        keys = [sum((a - b) ** 2 for a, b in zip(qi, xi)) for xi in xs]
+        want = [i for i, s in enumerate(keys) if limit is None or s <= limit]
+        got = indices[indptr[r]:indptr[r + 1]]
+        assert list(got) == want, f"{case.name} row {r}"
+        for i, dv in zip(got, dist[indptr[r]:indptr[r + 1]]):
+            exact = keys[i] / 2.0 ** 298 if keys[i] < 2 ** 1300 else math.inf
+            assert math.isclose(dv, math.sqrt(min(exact, R2)), rel_tol=1e-14, abs_tol=0) or dv == math.sqrt(R2)
+    sp, si, sd = oracle.radius(X, Q, R2, sort_results=True)
+    np.testing.assert_array_equal(sp, indptr)
+    for r in range(len(Q)):
+        seg = slice(sp[r], sp[r + 1])
+        pairs = list(zip(sd[seg], si[seg]))
+        assert pairs == sorted(pairs)
+        assert sorted(si[seg]) == list(indices[seg])
