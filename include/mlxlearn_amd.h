@@ -79,9 +79,11 @@ int mlk_set_test_hooks(uint32_t hooks, int64_t param);
 int mlk_set_forced_rows(const int64_t* rows_host, int64_t count);
 
 /* Tuning overrides (0 = automatic): database splits, candidate slots per
- * sub-list, query tiles per wave (1-2), row groups per block (1-2) and the
- * cumulative round fractions of each split. Results never depend on them. */
-int mlk_set_tuning(int32_t splits, int32_t capacity, int32_t query_tiles, int32_t row_groups,
+ * sub-list, query tiles of 32 per screen wave (1-2), stages of the screen's
+ * LDS tile ring (2-4) and the cumulative round fractions of each split.
+ * Results never depend on them; a shape that is not built falls back to the
+ * exact scan. */
+int mlk_set_tuning(int32_t splits, int32_t capacity, int32_t query_tiles, int32_t stages,
                    int32_t n_rounds, const double* round_fractions);
 
 /*

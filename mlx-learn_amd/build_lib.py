@@ -16,7 +16,10 @@ CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "mlxlearn_amd", "_lib")
 OUT = os.path.join(OUT_DIR, "libmlxlearn_amd.so")
 OBJ_DIR = os.path.join(HERE, "_build")
-SOURCES = ["mlk_capi.hip", "mlk_index.hip", "mlk_knn.hip", "mlk_radius.hip", "mlk_kmeans.hip"]
+SOURCES = ["mlk_capi.hip", "mlk_index.hip", "mlk_screen.hip", "mlk_knn.hip", "mlk_radius.hip", "mlk_kmeans.hip"]
+# per-source extra flags: the screen's values are finite by construction, so
+# its max-tree needs no NaN canonicalisation
+EXTRA = {"mlk_screen.hip": ["-fno-honor-nans"]}
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
@@ -30,7 +33,7 @@ FLAGS = [
 
 def _compile(src: str, verbose: bool) -> str:
     obj = os.path.join(OBJ_DIR, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [HIPCC, *FLAGS, *EXTRA.get(src, []), "-c", os.path.join(CSRC, src), "-o", obj]
     if verbose:
         print(" ".join(cmd), flush=True)
     proc = subprocess.run(cmd, capture_output=True, text=True)

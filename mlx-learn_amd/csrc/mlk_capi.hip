@@ -69,17 +69,17 @@ int mlk_set_forced_rows(const int64_t* rows, int64_t count) {
     return MLK_OK;
 }
 
-int mlk_set_tuning(int32_t splits, int32_t capacity, int32_t query_tiles, int32_t row_groups, int32_t n_rounds,
+int mlk_set_tuning(int32_t splits, int32_t capacity, int32_t query_tiles, int32_t stages, int32_t n_rounds,
                    const double* round_fractions) {
     std::lock_guard<std::mutex> g(g_config_lock);
-    if (splits < 0 || capacity < 0 || query_tiles < 0 || query_tiles > 2 || row_groups < 0 || row_groups > 2 ||
+    if (splits < 0 || capacity < 0 || query_tiles < 0 || query_tiles > 2 || stages < 0 || stages == 1 || stages > 4 ||
         n_rounds < 0)
-        return invalid("tuning values must be non-negative (query_tiles, row_groups <= 2)");
+        return invalid("tuning values must be non-negative (query_tiles <= 2, stages 2-4)");
     Tuning t;
     t.splits = splits;
     t.capacity = capacity;
     t.query_tiles = query_tiles;
-    t.row_groups = row_groups;
+    t.stages = stages;
     if (n_rounds) {
         if (!round_fractions) return invalid("round fractions missing");
         double prev = 0.0;

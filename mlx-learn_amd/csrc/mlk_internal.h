@@ -70,7 +70,7 @@ struct Tuning {
     int splits = 0;
     int capacity = 0;
     int query_tiles = 0;
-    int row_groups = 0;
+    int stages = 0;
     std::vector<double> rounds;
 };
 Tuning& tuning();

@@ -515,33 +515,6 @@ __global__ void k_force(int64_t m, const int64_t* __restrict__ rows, int64_t cou
 }
 
 // ---------------------------------------------------------------- host
-typedef void (*ScreenFn)(const ScreenArgs);
-
-template <int PREC, int QT, int RG>
-static ScreenFn pick_dp(int dp) {
-    switch (dp) {
-        case 32: return k_screen<PREC, 32, QT, RG, SCREEN_LISTS>;
-        case 64: return k_screen<PREC, 64, QT, RG, SCREEN_LISTS>;
-        case 96: return k_screen<PREC, 96, QT, RG, SCREEN_LISTS>;
-        case 128: return k_screen<PREC, 128, QT, RG, SCREEN_LISTS>;
-        default: return nullptr;
-    }
-}
-
-template <int PREC>
-static ScreenFn pick_variant(int dp, int qt, int rg) {
-    if (qt == 1) return pick_dp<PREC, 1, 1>(dp);
-    if (rg == 1) return pick_dp<PREC, 2, 1>(dp);
-    return pick_dp<PREC, 2, 2>(dp);
-}
-
-// Screen variants: (query tiles per wave, row groups) in {(1,1), (2,1), (2,2)}.
-static ScreenFn pick_screen(int prec, int dp, int qt, int rg) {
-    if (prec == PREC_FP32) return pick_variant<PREC_FP32>(dp, qt, rg);
-    if (prec == PREC_BF16) return pick_variant<PREC_BF16>(dp, qt, rg);
-    return pick_variant<PREC_FP16>(dp, qt, rg);
-}
-
 static double unit_roundoff(int prec) {
     switch (prec) {
         case PREC_FP32: return 0x1p-24;
@@ -563,18 +536,18 @@ Consts make_consts(int prec, int dp, int d, bool exact_inputs, double scale) {
 }
 
 struct Plan {
-    int qt, rg, splits, cap;
+    ScreenShape shape;
+    int splits, cap;
     std::vector<double> rounds;  // cumulative fractions of each split, ending at 1
 };
 
 static Plan make_plan(int prec, int dp, int64_t n, int64_t m, int k) {
     Plan p;
     const Tuning& t = tuning();
-    // measured default: one query tile per wave in 4-wave blocks (highest
-    // occupancy) for every precision; the other variants stay selectable
-    p.qt = t.query_tiles ? t.query_tiles : 1;
-    p.rg = p.qt == 1 ? 1 : (t.row_groups ? t.row_groups : 2);
-    const int64_t BQ = 4 * p.qt * 32;
+    p.shape = default_shape(prec);
+    if (t.query_tiles) p.shape.qt = t.query_tiles;
+    if (t.stages) p.shape.ns = t.stages;
+    const int64_t BQ = 4 * p.shape.qt * 32;
     const int64_t qb = (m + BQ - 1) / BQ;
     if (t.splits) {
         p.splits = t.splits;
@@ -590,7 +563,7 @@ static Plan make_plan(int prec, int dp, int64_t n, int64_t m, int k) {
     // slots per (query, split, lane half) sub-list
     // (at least 6k slots per query over its 2 * splits sub-lists)
     const int base = prec == PREC_FP32 ? 24 : 48;
-    const int lists = 2 * p.rg * p.splits;
+    const int lists = 2 * p.splits;
     const int need = (6 * k + lists - 1) / lists + 4;
     p.cap = t.capacity ? t.capacity : std::max(base, need);
     if (!t.rounds.empty()) {
@@ -631,11 +604,11 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
     const int dp = mr.dp;
     ScreenFn screen = nullptr;
     Plan plan = make_plan(prec, dp, n, m, k);
-    screen = pick_screen(prec, dp, plan.qt, plan.rg);
+    screen = pick_screen(SCREEN_LISTS, prec, dp, plan.shape);
     const bool all_fallback = !screen || k > 64 || mr.n_unsafe > 64 || (hooks().hooks & MLK_HOOK_FORCE_FALLBACK);
     if (hooks().hooks & MLK_HOOK_CAPACITY) plan.cap = (int)std::max<int64_t>(1, hooks().param);
 
-    const int BQ = 4 * plan.qt * 32;
+    const int BQ = 4 * plan.shape.qt * 32;
     const int64_t QB = (m + BQ - 1) / BQ;
     const int64_t m_pad = QB * BQ;
     const int S = plan.splits;
@@ -685,7 +658,7 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
     MLK_CHECK_HIP(hipEventRecord(ev1, stream));
     std::vector<hipEvent_t> round_events;
     if (!all_fallback) {
-        const int S2 = 2 * plan.rg * S;  // candidate sub-lists per query (split x row group x lane half)
+        const int S2 = 2 * S;  // candidate sub-lists per query (split x lane half)
         MLK_TRY(ws[8].reserve((size_t)m_pad * S2 * sizeof(int)));          // counts
         MLK_TRY(ws[9].reserve((size_t)m_pad * S2 * cap * sizeof(int)));    // rows
         MLK_TRY(ws[10].reserve((size_t)m_pad * S2 * cap * sizeof(float))); // dot-product estimates
@@ -729,7 +702,7 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
             round_ev.push_back(ea);
             round_ev.push_back(eb);
             MLK_CHECK_HIP(hipEventRecord(ea, stream));
-            hipLaunchKernelGGL(screen, dim3(blocks), dim3(256 * plan.rg), 0, stream, sa);
+            hipLaunchKernelGGL(screen, dim3(blocks), dim3(256), 0, stream, sa);
             MLK_CHECK_HIP(hipGetLastError());
             MLK_CHECK_HIP(hipEventRecord(eb, stream));
             ++st.rounds;

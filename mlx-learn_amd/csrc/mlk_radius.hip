@@ -162,27 +162,6 @@ __global__ void k_everything_indptr(int64_t m, int64_t per, int64_t* __restrict_
     if (q <= m) indptr[q] = q * per;
 }
 
-typedef void (*ScreenFn)(const ScreenArgs);
-
-template <int PREC, int QT, int RG>
-static ScreenFn pick_dp_pairs(int dp) {
-    switch (dp) {
-        case 32: return k_screen<PREC, 32, QT, RG, SCREEN_PAIRS>;
-        case 64: return k_screen<PREC, 64, QT, RG, SCREEN_PAIRS>;
-        case 96: return k_screen<PREC, 96, QT, RG, SCREEN_PAIRS>;
-        case 128: return k_screen<PREC, 128, QT, RG, SCREEN_PAIRS>;
-        default: return nullptr;
-    }
-}
-
-// float32: one query tile per wave, 4-wave blocks; bf16 / fp16: two query
-// tiles per wave, 8-wave blocks (two row groups).
-static ScreenFn pick_pairs(int prec, int dp) {
-    if (prec == PREC_FP32) return pick_dp_pairs<PREC_FP32, 1, 1>(dp);
-    if (prec == PREC_BF16) return pick_dp_pairs<PREC_BF16, 2, 2>(dp);
-    return pick_dp_pairs<PREC_FP16, 2, 2>(dp);
-}
-
 static inline unsigned blocks_for(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
 
 // Stable segmented sort of every CSR row by its distance bits (non-negative
@@ -268,9 +247,9 @@ int radius_query_impl(mlk_index* index, const float* Q, int64_t m, double R2, in
     MLK_TRY(prepare_mirror(index, prec, stream));
     Mirror& mr = index->mirror[prec];
     const int dp = mr.dp;
-    ScreenFn screen = pick_pairs(prec, dp);
-    const int qt = prec == PREC_FP32 ? 1 : 2;
-    const int rg = prec == PREC_FP32 ? 1 : 2;
+    const ScreenShape shape = default_shape(prec);
+    ScreenFn screen = pick_screen(SCREEN_PAIRS, prec, dp, shape);
+    const int qt = shape.qt;
     const int BQ = 4 * qt * 32;
     const int64_t QB = (m + BQ - 1) / BQ;
     const int64_t m_pad = QB * BQ;
@@ -338,7 +317,7 @@ int radius_query_impl(mlk_index* index, const float* Q, int64_t m, double R2, in
             sa.pairs = ws[9].as<int2>();
             sa.pair_count = dcount;
             sa.pair_cap = cap;
-            hipLaunchKernelGGL(screen, dim3((unsigned)(QB * S)), dim3(256 * rg), 0, stream, sa);
+            hipLaunchKernelGGL(screen, dim3((unsigned)(QB * S)), dim3(256), 0, stream, sa);
             MLK_CHECK_HIP(hipGetLastError());
         } else {
             set_error("radius search supports at most 128 features");

@@ -1,0 +1,428 @@
+// The fused MFMA screen kernel (k-NN candidate lists and radius pairs).
+//
+// Reference counterpart: gram_lower + select_topk / compact_gram
+// (src/mlxlearn_device/_neighbors_general.py:188-223, _neighbors_fused.py:421-506,
+// _radius.py:128-185), which write the rows x N Gram tile to memory and
+// re-read it for selection; here the tile stays in MFMA accumulators and only
+// threshold hits leave the kernel.
+//
+// This translation unit is compiled with -fno-honor-nans: every screen value
+// is finite (rows outside the safe range are zero in the mirror), so the
+// max-tree hit test needs no NaN canonicalisation.
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "mlk_device.h"
+#include "mlk_internal.h"
+#include "mlk_screen.h"
+
+namespace mlk {
+
+template <int PREC> struct PT;
+template <> struct PT<PREC_FP32> { typedef float T; };
+template <> struct PT<PREC_BF16> { typedef __bf16 T; };
+template <> struct PT<PREC_FP16> { typedef _Float16 T; };
+
+template <int PREC>
+__device__ __forceinline__ f32x16 mfma16(const i32x4& a, const i32x4& b, const f32x16& c) {
+    if constexpr (PREC == PREC_BF16)
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+    else
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, a), __builtin_bit_cast(f16x8, b), c, 0, 0, 0);
+}
+
+// Work item (query block, split) of a block; the QB blocks of one split are
+// consecutive slots of one XCD (blocks are dispatched round-robin over the 8
+// XCDs), so they stream the same rows through that XCD's L2 together.
+__device__ __forceinline__ void block_item(const ScreenArgs& a, int& qb, int& split) {
+    const int b = blockIdx.x;
+    if (a.xcd_map) {
+        const int xcd = b & 7, slot = b >> 3;
+        qb = slot % a.QB;
+        split = (slot / a.QB) * 8 + xcd;
+    } else {
+        qb = b % a.QB;
+        split = b / a.QB;
+    }
+}
+
+typedef __attribute__((address_space(3))) char lds_char;
+
+#if defined(__HIP_DEVICE_COMPILE__)
+// Raw buffer resource over [base, base + bytes): loads past `bytes` return zero
+// (buffer range check), so tile loads need no clamping.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void* base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
+}
+#endif
+
+__device__ __forceinline__ float max16(const f32x16& v) {
+    float a = fmaxf(fmaxf(v[0], v[1]), v[2]);
+    float b = fmaxf(fmaxf(v[3], v[4]), v[5]);
+    float c = fmaxf(fmaxf(v[6], v[7]), v[8]);
+    float d = fmaxf(fmaxf(v[9], v[10]), v[11]);
+    float e = fmaxf(fmaxf(v[12], v[13]), v[14]);
+    return fmaxf(fmaxf(fmaxf(a, b), fmaxf(c, d)), fmaxf(e, v[15]));
+}
+
+// s_waitcnt with vmcnt <= N and lgkmcnt = 0 (expcnt unconstrained); gfx9
+// encoding: vmcnt[3:0] in bits 3:0, vmcnt[5:4] in bits 15:14.
+template <int N>
+__device__ __forceinline__ void wait_vm_lgkm0() {
+    static_assert(N >= 0 && N < 64, "vmcnt range");
+    __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (0 << 8) | ((N >> 4) << 14));
+}
+
+// Workgroup barrier that the compiler may not move memory accesses across.
+// The waits that make the tile data valid are explicit (wait_vm_lgkm0).
+__device__ __forceinline__ void tile_barrier() { asm volatile("s_barrier" ::: "memory"); }
+
+// The screen. A block of 4 waves walks BN-row tiles (BN = 32 * RTW) of its
+// split for BQ = 4 * QT * 32 queries: each wave keeps the operand fragments of
+// its QT x 32 queries in registers and multiplies them with every row of the
+// tile, so every A-operand byte read from LDS feeds QT MFMAs. Tiles stream
+// from memory straight into an NS-stage LDS ring (buffer loads with the LDS
+// flag, no staging registers; rows padded by one 16-byte chunk so the 32 rows
+// of an operand read hit distinct banks): while a wave computes tile t, tiles
+// t+1 .. t+NS-1 are in flight, and at the end of each step a wave waits only
+// for tile t+1 (vmcnt counts its outstanding tile loads in issue order).
+// Per tile a wave issues its MFMAs (accumulators start at the rows' biases
+// -a_x), then one branch-free max-tree test of every accumulator against its
+// query's threshold; only a hit enters the append path.
+template <int PREC, int DP, int QT, int NS, int MODE>
+__global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
+    // the body uses device-only types (buffer resources): compile it in the
+    // device pass only, so the host pass still emits the kernel's launch stub
+#if defined(__HIP_DEVICE_COMPILE__)
+    typedef typename PT<PREC>::T T;
+    constexpr int NW = 4;                     // waves per block
+    constexpr int RTW = QT == 1 ? 2 : 1;      // 32-row MFMA tiles per step
+    constexpr int BN = 32 * RTW;              // rows per tile
+    constexpr int EB = sizeof(T);
+    constexpr int RB = DP * EB;               // bytes per database row
+    constexpr int CPR = RB / 16;              // 16-byte chunks per row
+    constexpr int RS = RB + 16;               // LDS row stride: one pad chunk per row
+    constexpr int NCH = BN * (CPR + 1);       // LDS chunks per tile
+    constexpr int NLOAD = (NCH + 63) / 64;    // 1 KiB load instructions per tile
+    constexpr int ABYTES = NLOAD * 1024;
+    constexpr int LPW = (NLOAD + NW - 1) / NW;
+    constexpr int BQ = NW * QT * 32;
+    static_assert(NS >= 2 && NS <= 4, "stages");
+    static_assert(QT == 1 || QT == 2, "query tiles");
+    static_assert(BN <= 64, "bias staging holds 64 rows");
+    // distinct buffers (not one array) so the compiler can tell the tile
+    // being read from the tiles being loaded and keeps those loads in flight
+    __shared__ __attribute__((aligned(1024))) char sA0[ABYTES];
+    __shared__ __attribute__((aligned(1024))) char sA1[ABYTES];
+    __shared__ __attribute__((aligned(1024))) char sA2[NS > 2 ? ABYTES : 16];
+    __shared__ __attribute__((aligned(1024))) char sA3[NS > 3 ? ABYTES : 16];
+    // [0, 64): -a_x of the tile rows, [64, 128): their norms (float32)
+    __shared__ __attribute__((aligned(256))) float sB0[128];
+    __shared__ __attribute__((aligned(256))) float sB1[128];
+    __shared__ __attribute__((aligned(256))) float sB2[NS > 2 ? 128 : 4];
+    __shared__ __attribute__((aligned(256))) float sB3[NS > 3 ? 128 : 4];
+    __shared__ int2 sPair[MODE == SCREEN_PAIRS ? NW : 1][MODE == SCREEN_PAIRS ? PAIR_STAGE : 1];
+    __shared__ int sPairN[NW];
+
+    int qb, split;
+    block_item(a, qb, split);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, h = lane >> 5, col = lane & 31;
+    const int64_t s_lo = (int64_t)split * a.split_rows;
+    const int64_t r_begin = s_lo + a.off_lo;
+    int64_t r_end = s_lo + (a.off_hi < a.split_rows ? a.off_hi : a.split_rows);
+    if (r_end > a.n) r_end = a.n;
+    if (r_begin >= r_end) return;
+
+    if constexpr (MODE == SCREEN_PAIRS) {
+        if (threadIdx.x < NW) sPairN[threadIdx.x] = 0;
+    }
+    // Radius mode: copy this wave's staged pairs to the global buffer.
+    auto flush_pairs = [&]() {
+        const int nbuf = sPairN[wave];
+        unsigned long long base = 0;
+        if (lane == 0 && nbuf) base = atomicAdd(a.pair_count, (unsigned long long)nbuf);
+        base = __shfl(base, 0);
+        for (int j = lane; j < nbuf; j += 64)
+            if (base + j < a.pair_cap) a.pairs[base + j] = sPair[wave][j];
+        __builtin_amdgcn_wave_barrier();
+        if (lane == 0) sPairN[wave] = 0;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    };
+
+    // Query operand fragments: lane (col, h) holds features h*DP/2 + [8j, 8j+8)
+    // (float32: h*DP/2 + j) of query col of each tile.
+    const int64_t qrow0 = (int64_t)qb * BQ + wave * QT * 32;
+    float thr[QT];
+#pragma unroll
+    for (int qt = 0; qt < QT; ++qt) thr[qt] = a.thr[qrow0 + qt * 32 + col];
+    // k-NN mode: each lane owns the candidate sub-list (query col, split, lane
+    // half h), so appends need no atomics; counts live in registers.
+    int cnt_r[QT];
+    size_t list_base[QT];
+    if constexpr (MODE == SCREEN_LISTS) {
+#pragma unroll
+        for (int qt = 0; qt < QT; ++qt) {
+            const size_t sub = (((size_t)(qrow0 + qt * 32 + col)) * a.S + split) * 2 + h;
+            cnt_r[qt] = a.cnt[sub];
+            list_base[qt] = sub * a.cap;
+        }
+    }
+    bool live = false;
+#pragma unroll
+    for (int qt = 0; qt < QT; ++qt) live |= thr[qt] < INFINITY;
+    const bool wave_live = ballot(live) != 0;
+
+    constexpr int NB = PREC == PREC_FP32 ? DP / 2 : DP / 16;
+    typedef typename std::conditional<PREC == PREC_FP32, float, i32x4>::type BFrag;
+    BFrag bq[QT][NB];
+#pragma unroll
+    for (int qt = 0; qt < QT; ++qt) {
+        const T* qp = (const T*)a.qs + (size_t)(qrow0 + qt * 32 + col) * DP + h * (DP / 2);
+        if constexpr (PREC == PREC_FP32) {
+#pragma unroll
+            for (int j = 0; j < NB; j += 4) {
+                f32x4 v = *(const f32x4*)(qp + j);
+                bq[qt][j] = v[0]; bq[qt][j + 1] = v[1]; bq[qt][j + 2] = v[2]; bq[qt][j + 3] = v[3];
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < NB; ++j) bq[qt][j] = *(const i32x4*)(qp + 8 * j);
+        }
+    }
+
+    const int64_t ntiles = (r_end - r_begin + BN - 1) / BN;
+    // Load plan: LDS chunk L (of NLOAD * 64 per tile) is row L / (CPR + 1),
+    // position L % (CPR + 1); the last position of each row and every chunk
+    // past the tile are padding (loaded out of range, i.e. zero). Source
+    // offsets within a tile are lane constants; the tile base goes into the
+    // buffer resource (SGPRs). Waves 1 and 2 also load the rows' biases and
+    // norms.
+    uint32_t voff[LPW];
+#pragma unroll
+    for (int i = 0; i < LPW; ++i) {
+        const int L = (wave + i * NW) * 64 + lane;
+        const int row = L / (CPR + 1), pos = L % (CPR + 1);
+        voff[i] = (pos < CPR && row < BN) ? (uint32_t)(row * RB + pos * 16) : 0x7ffffff0u;
+    }
+    // tile loads this wave issues (uniform): its share of the NLOAD chunks, plus one
+    const int my_loads = (NLOAD > wave ? (NLOAD - wave + NW - 1) / NW : 0) + (wave == 1 || wave == 2);
+
+    auto bufA = [&](auto I) -> char* {
+        constexpr int i = decltype(I)::value;
+        if constexpr (i == 0) return sA0; else if constexpr (i == 1) return sA1;
+        else if constexpr (i == 2) return sA2; else return sA3;
+    };
+    auto bufB = [&](auto I) -> float* {
+        constexpr int i = decltype(I)::value;
+        if constexpr (i == 0) return sB0; else if constexpr (i == 1) return sB1;
+        else if constexpr (i == 2) return sB2; else return sB3;
+    };
+    auto load_tile = [&](int64_t t, char* dstA, float* dstB) {
+        const int64_t r0 = r_begin + t * BN;
+        const int64_t rows = r_end - r0 < BN ? r_end - r0 : BN;
+        const __amdgpu_buffer_rsrc_t ra = buffer_rsrc((const char*)a.xs + (size_t)r0 * RB, (uint32_t)(rows * RB));
+#pragma unroll
+        for (int i = 0; i < LPW; ++i) {
+            const int g = wave + i * NW;
+            if (g < NLOAD)
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(ra, (lds_char*)(dstA + g * 1024), 16, voff[i], 0, 0, 0);
+        }
+        if (wave == 1 || wave == 2) {
+            const float* src = wave == 1 ? a.bias + r0 : a.nx32 + r0;
+            const __amdgpu_buffer_rsrc_t rb = buffer_rsrc(src, (uint32_t)(rows * 4));
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rb, (lds_char*)(dstB + (wave - 1) * 64), 4, lane * 4, 0, 0, 0);
+        }
+    };
+    // wait until at most `stages` tiles of this wave's loads are outstanding
+    // (and every LDS read has returned), then meet the other waves
+    auto sync_tiles = [&](int stages) {
+        const int allowed = stages * my_loads;
+        switch (allowed) {
+#define MLK_WAIT_CASE(n) case n: wait_vm_lgkm0<n>(); break;
+            MLK_WAIT_CASE(1) MLK_WAIT_CASE(2) MLK_WAIT_CASE(3) MLK_WAIT_CASE(4) MLK_WAIT_CASE(5)
+            MLK_WAIT_CASE(6) MLK_WAIT_CASE(7) MLK_WAIT_CASE(8) MLK_WAIT_CASE(9) MLK_WAIT_CASE(10)
+            MLK_WAIT_CASE(11) MLK_WAIT_CASE(12) MLK_WAIT_CASE(13) MLK_WAIT_CASE(14) MLK_WAIT_CASE(15)
+            MLK_WAIT_CASE(16) MLK_WAIT_CASE(17) MLK_WAIT_CASE(18) MLK_WAIT_CASE(19) MLK_WAIT_CASE(20)
+#undef MLK_WAIT_CASE
+            default: wait_vm_lgkm0<0>(); break;
+        }
+        tile_barrier();
+    };
+
+    // one step: prefetch tile t + NS - 1 into the buffer freed by step t - 1,
+    // compute tile t from buffer CUR, wait for tile t + 1
+    auto step = [&](auto CUR, int64_t t) {
+        constexpr int c = decltype(CUR)::value;
+        constexpr int nxt = (c + NS - 1) % NS;
+        const bool full = t + NS - 1 < ntiles;
+        if (full) load_tile(t + NS - 1, bufA(std::integral_constant<int, nxt>{}), bufB(std::integral_constant<int, nxt>{}));
+        const char* cA = bufA(CUR);
+        const float* cB = bufB(CUR);
+        bool stored = false;
+        if (wave_live) {
+            const int64_t r0 = r_begin + t * BN;
+            f32x16 acc[RTW][QT];
+#pragma unroll
+            for (int rt = 0; rt < RTW; ++rt) {
+                // accumulator start = -a_x of this lane's 16 rows (4 runs of 4 rows)
+                const float* nb = cB + rt * 32 + (h << 2);
+                const f32x4 b0 = *(const f32x4*)(nb), b1 = *(const f32x4*)(nb + 8);
+                const f32x4 b2 = *(const f32x4*)(nb + 16), b3 = *(const f32x4*)(nb + 24);
+                const f32x16 init = {b0[0], b0[1], b0[2], b0[3], b1[0], b1[1], b1[2], b1[3],
+                                     b2[0], b2[1], b2[2], b2[3], b3[0], b3[1], b3[2], b3[3]};
+                const char* arow = cA + (rt * 32 + col) * RS + h * (CPR / 2) * 16;
+                if constexpr (PREC == PREC_FP32) {
+#pragma unroll
+                    for (int qt = 0; qt < QT; ++qt) acc[rt][qt] = init;
+#pragma unroll
+                    for (int j = 0; j < NB; j += 4) {
+                        const f32x4 av = *(const f32x4*)(arow + (j / 4) * 16);
+#pragma unroll
+                        for (int e = 0; e < 4; ++e)
+#pragma unroll
+                            for (int qt = 0; qt < QT; ++qt)
+                                acc[rt][qt] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[e], bq[qt][j + e], acc[rt][qt], 0, 0, 0);
+                    }
+                } else {
+                    // the first k-step reads the bias registers as its C operand
+                    const i32x4 a0 = *(const i32x4*)(arow);
+#pragma unroll
+                    for (int qt = 0; qt < QT; ++qt) acc[rt][qt] = mfma16<PREC>(a0, bq[qt][0], init);
+#pragma unroll
+                    for (int j = 1; j < NB; ++j) {
+                        const i32x4 av = *(const i32x4*)(arow + j * 16);
+#pragma unroll
+                        for (int qt = 0; qt < QT; ++qt) acc[rt][qt] = mfma16<PREC>(av, bq[qt][j], acc[rt][qt]);
+                    }
+                }
+            }
+            // Hit test: acc >= thr  <=>  L(x) <= tau. Branch-free over the tile.
+            bool hit_rq[RTW][QT];
+            bool any = false;
+#pragma unroll
+            for (int rt = 0; rt < RTW; ++rt)
+#pragma unroll
+                for (int qt = 0; qt < QT; ++qt) {
+                    hit_rq[rt][qt] = max16(acc[rt][qt]) >= thr[qt];
+                    any |= hit_rq[rt][qt];
+                }
+            if (ballot(any)) {
+                stored = true;
+#pragma unroll
+                for (int rt = 0; rt < RTW; ++rt)
+#pragma unroll
+                for (int qt = 0; qt < QT; ++qt) {
+                    if (!ballot(hit_rq[rt][qt])) continue;
+                    const int64_t qg = qrow0 + qt * 32 + col;
+                    const int64_t self = a.exclude ? a.self_off + qg : -1;
+                    // rolled loop: the register index i is uniform (v_movrels)
+#pragma unroll 1
+                    for (int i = 0; i < 16; ++i) {
+                        const float v = acc[rt][qt][i];
+                        const int rl = rt * 32 + mfma32_row(i, lane);
+                        const int64_t row = r0 + rl;
+                        const bool hit = v >= thr[qt] && row < r_end && row != self;
+                        if constexpr (MODE == SCREEN_LISTS) {
+                            if (hit) {
+                                const int slot = cnt_r[qt]++;
+                                if (slot < a.cap) {
+                                    const size_t o = list_base[qt] + slot;
+                                    a.cidx[o] = (int)row;
+                                    a.cP[o] = v - cB[rl];
+                                    a.cN[o] = cB[64 + rl];
+                                }
+                            }
+                        } else {
+                            if (ballot(hit)) {
+                                if (sPairN[wave] + 64 > PAIR_STAGE) flush_pairs();
+                                if (hit) {
+                                    const int slot = atomicAdd(&sPairN[wave], 1);
+                                    sPair[wave][slot] = int2{(int)qg, (int)row};
+                                }
+                                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                                __builtin_amdgcn_wave_barrier();
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        // Stores of the hit path also count in vmcnt: after one, wait for
+        // everything rather than rely on the order of loads and stores.
+        sync_tiles(full && !stored ? NS - 2 : 0);
+    };
+
+    // prologue: tiles 0 .. NS-2 in flight, wait for tile 0
+    load_tile(0, sA0, sB0);
+    if constexpr (NS > 2) if (1 < ntiles) load_tile(1, sA1, sB1);
+    if constexpr (NS > 3) if (2 < ntiles) load_tile(2, sA2, sB2);
+    sync_tiles(ntiles >= NS - 1 ? NS - 2 : 0);
+    for (int64_t t = 0; t < ntiles; t += NS) {
+        step(std::integral_constant<int, 0>{}, t);
+        if (t + 1 < ntiles) step(std::integral_constant<int, 1>{}, t + 1);
+        if constexpr (NS > 2) if (t + 2 < ntiles) step(std::integral_constant<int, 2>{}, t + 2);
+        if constexpr (NS > 3) if (t + 3 < ntiles) step(std::integral_constant<int, 3>{}, t + 3);
+    }
+    if constexpr (MODE == SCREEN_LISTS) {
+#pragma unroll
+        for (int qt = 0; qt < QT; ++qt)
+            a.cnt[(((size_t)(qrow0 + qt * 32 + col)) * a.S + split) * 2 + h] = cnt_r[qt];
+    } else {
+        flush_pairs();
+    }
+#else
+    (void)a;
+#endif
+}
+
+template <int PREC, int QT, int NS, int MODE>
+static ScreenFn pick_dp(int dp) {
+    switch (dp) {
+        case 32: return k_screen<PREC, 32, QT, NS, MODE>;
+        case 64: return k_screen<PREC, 64, QT, NS, MODE>;
+        case 96: return k_screen<PREC, 96, QT, NS, MODE>;
+        case 128: return k_screen<PREC, 128, QT, NS, MODE>;
+        default: return nullptr;
+    }
+}
+
+template <int PREC, int MODE>
+static ScreenFn pick_shape(int dp, int qt, int ns) {
+    if constexpr (PREC == PREC_FP32) {
+        // float32 operands: two query tiles would need 128 registers of fragments
+        if (qt == 1 && ns == 2) return pick_dp<PREC, 1, 2, MODE>(dp);
+        if constexpr (MODE == SCREEN_LISTS)
+            if (qt == 1 && ns == 3) return pick_dp<PREC, 1, 3, MODE>(dp);
+        return nullptr;
+    } else {
+        if (qt == 1 && ns == 2) return pick_dp<PREC, 1, 2, MODE>(dp);
+        if constexpr (MODE == SCREEN_LISTS) {
+            if (qt == 2 && ns == 4) return pick_dp<PREC, 2, 4, MODE>(dp);
+            if (qt == 2 && ns == 3) return pick_dp<PREC, 2, 3, MODE>(dp);
+            if (qt == 2 && ns == 2) return pick_dp<PREC, 2, 2, MODE>(dp);
+        }
+        return nullptr;
+    }
+}
+
+// Measured on the 1M x 128 mixture workload: with the candidate appends of a
+// real query, one query tile per wave (occupancy 4) beats two (occupancy 3).
+ScreenShape default_shape(int prec) {
+    (void)prec;
+    return ScreenShape{1, 2};
+}
+
+ScreenFn pick_screen(int mode, int prec, int dp, ScreenShape shape) {
+    if (mode == SCREEN_LISTS) {
+        if (prec == PREC_FP32) return pick_shape<PREC_FP32, SCREEN_LISTS>(dp, shape.qt, shape.ns);
+        if (prec == PREC_BF16) return pick_shape<PREC_BF16, SCREEN_LISTS>(dp, shape.qt, shape.ns);
+        return pick_shape<PREC_FP16, SCREEN_LISTS>(dp, shape.qt, shape.ns);
+    }
+    if (prec == PREC_FP32) return pick_shape<PREC_FP32, SCREEN_PAIRS>(dp, shape.qt, shape.ns);
+    if (prec == PREC_BF16) return pick_shape<PREC_BF16, SCREEN_PAIRS>(dp, shape.qt, shape.ns);
+    return pick_shape<PREC_FP16, SCREEN_PAIRS>(dp, shape.qt, shape.ns);
+}
+
+}  // namespace mlk

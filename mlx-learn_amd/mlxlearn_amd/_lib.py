@@ -110,8 +110,8 @@ def set_forced_rows(rows) -> None:
     check(load().mlk_set_forced_rows(arr.ctypes.data_as(_i64p), arr.size), "mlk_set_forced_rows")
 
 
-def set_tuning(splits: int = 0, capacity: int = 0, query_tiles: int = 0, row_groups: int = 0, rounds=None) -> None:
+def set_tuning(splits: int = 0, capacity: int = 0, query_tiles: int = 0, stages: int = 0, rounds=None) -> None:
     rounds = list(rounds or [])
     arr = (ctypes.c_double * max(len(rounds), 1))(*rounds)
-    check(load().mlk_set_tuning(int(splits), int(capacity), int(query_tiles), int(row_groups), len(rounds), arr),
+    check(load().mlk_set_tuning(int(splits), int(capacity), int(query_tiles), int(stages), len(rounds), arr),
           "mlk_set_tuning")

@@ -37,6 +37,7 @@ def test_library_validation_without_device():
     rc = lib.mlk_knn_query(None, None, 0, 1, 0, 0, None, None, None)
     assert rc == 1 and b"index is NULL" in lib.mlk_last_error()
     assert lib.mlk_set_tuning(-1, 0, 0, 0, 0, None) == 1
+    assert lib.mlk_set_tuning(0, 0, 0, 1, 0, None) == 1  # a one-stage ring does not exist
     with pytest.raises(ValueError):
         _lib.check(1)
     with pytest.raises(MemoryError):
