@@ -209,3 +209,25 @@ def test_radius_oracle_matches_integer_brute_force(case):
         pairs = list(zip(sd[seg], si[seg]))
         assert pairs == sorted(pairs)
         assert sorted(si[seg]) == list(indices[seg])
+
+
+def test_kmeans_assign_is_exact_nearest_centre():
+    """Labels = the exact 1-NN among the centres (integer brute force), residual = canonical s."""
+    r = np.random.default_rng(31)
+    X = (r.normal(size=(60, 3)) * 2.0 ** r.integers(-20, 20, (60, 1))).astype(np.float32)
+    C = np.concatenate([X[:5], X[:5] * np.float32(1.0000001), r.normal(size=(10, 3)).astype(np.float32)])
+    labels, resid = oracle.kmeans_assign(X, C)
+    np.testing.assert_array_equal(labels, oracle.integer_knn(C, X, 1)[:, 0])
+    for i in range(60):
+        assert resid[i] == oracle.canonical_sq(X[i], C[labels[i]])
+
+
+def test_cpu_baseline_port_is_exact(rng):
+    """The timed CPU baseline (CPUIndex._gram restatement) returns the oracle's neighbours."""
+    X = rng.normal(size=(3000, 16)).astype(np.float32)
+    X[100:110] = X[5]
+    Q = np.concatenate([X[:20], rng.normal(size=(50, 16)).astype(np.float32)])
+    d, i = oracle.GramPort(X).query(Q, 7, block=32)
+    wd, wi = oracle.knn(X, Q, 7)
+    np.testing.assert_array_equal(i, wi)
+    np.testing.assert_allclose(d, wd, rtol=1e-12)
