@@ -101,33 +101,61 @@ __device__ __forceinline__ int sacc_sign(long long* w) {
     return 0;
 }
 
-// Float64 sum that reports whether every operation was exact.
+// Float64 sum that reports whether every operation was exact. Features are
+// read sixteen at a time so their loads are in flight together.
 __device__ __forceinline__ double exact_flag_sum(const float* __restrict__ q, const float* __restrict__ x, int d,
                                                  bool& exact) {
     double s = 0.0;
     bool ok = true;
-    for (int i = 0; i < d; ++i) {
-        const double A = (double)q[i], B = -(double)x[i];
-        const double t = A + B;
-        const double bb = t - A;
-        ok &= ((A - (t - bb)) + (B - bb)) == 0.0;
-        const double p = t * t;
-        ok &= __fma_rn(t, t, -p) == 0.0;
-        const double s2 = s + p;
-        const double b2 = s2 - s;
-        ok &= ((s - (s2 - b2)) + (p - b2)) == 0.0;
-        s = s2;
+    for (int i0 = 0; i0 < d; i0 += 16) {
+        float qa[16], xa[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {  // unconditional loads (clamped index)
+            const int i = i0 + u < d ? i0 + u : d - 1;
+            qa[u] = q[i];
+            xa[u] = x[i];
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            if (i0 + u >= d) break;
+            const double A = (double)qa[u], B = -(double)xa[u];
+            const double t = A + B;
+            const double bb = t - A;
+            ok &= ((A - (t - bb)) + (B - bb)) == 0.0;
+            const double p = t * t;
+            ok &= __fma_rn(t, t, -p) == 0.0;
+            const double s2 = s + p;
+            const double b2 = s2 - s;
+            ok &= ((s - (s2 - b2)) + (p - b2)) == 0.0;
+            s = s2;
+        }
     }
     exact = ok;
     return s;
 }
 
+// Bitwise equality of two rows, sixteen features per batch of loads.
+__device__ __forceinline__ bool rows_identical(const float* __restrict__ a, const float* __restrict__ b, int d) {
+    for (int i0 = 0; i0 < d; i0 += 16) {
+        uint32_t va[16], vb[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {  // unconditional loads (clamped index)
+            const int i = i0 + u < d ? i0 + u : d - 1;
+            va[u] = __float_as_uint(a[i]);
+            vb[u] = __float_as_uint(b[i]);
+        }
+        bool same = true;
+#pragma unroll
+        for (int u = 0; u < 16; ++u) same &= va[u] == vb[u];
+        if (!same) return false;
+    }
+    return true;
+}
+
 // sign(S(q, a) - S(q, b)), exactly.
 __device__ __noinline__ int exact_cmp(const float* __restrict__ q, const float* __restrict__ a,
                                       const float* __restrict__ b, int d) {
-    bool same = true;
-    for (int i = 0; i < d && same; ++i) same = __float_as_uint(a[i]) == __float_as_uint(b[i]);
-    if (same) return 0;
+    if (rows_identical(a, b, d)) return 0;
     bool ea, eb;
     const double sa = exact_flag_sum(q, a, d, ea);
     const double sb = exact_flag_sum(q, b, d, eb);

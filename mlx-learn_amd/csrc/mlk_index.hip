@@ -130,7 +130,9 @@ __global__ void k_mirror(const float* __restrict__ X, int64_t n, int64_t n_pad, 
     if (lane == 0) {
         norms[row] = row < n ? s : 0.0;
         flags[row] = (row < n && safe) ? 1 : 0;
-        if (row < n && safe && !exact) atomicOr(inexact, 1u);
+        // one flag for the whole mirror: skip the (same-address) atomic once it is set
+        if (row < n && safe && !exact && __hip_atomic_load(inexact, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u)
+            atomicOr(inexact, 1u);
     }
 }
 

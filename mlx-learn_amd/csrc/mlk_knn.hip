@@ -75,7 +75,9 @@ __device__ __forceinline__ double fast_sq(const float* __restrict__ q, const flo
 }
 
 // One wave per query: the k-th smallest certified upper bound over up to 256
-// evenly spaced rows is an upper bound of the query's k-th smallest S.
+// evenly spaced rows is an upper bound of the query's k-th smallest S. Sixteen
+// lanes share a sample row (coalesced 64-byte segments), four rows per step;
+// the float64 partial sums may be added in any order (bound g).
 __global__ void k_tau0(const float* __restrict__ X, int64_t n, int d, const float* __restrict__ Q, int64_t m,
                        int64_t m_pad, int k, int64_t self_off, int exclude, const double* __restrict__ nq,
                        const uint8_t* __restrict__ qflags, Consts c, float* __restrict__ tau,
@@ -90,16 +92,51 @@ __global__ void k_tau0(const float* __restrict__ X, int64_t n, int d, const floa
     }
     const int samples = n < TAU0_SAMPLES ? (int)n : TAU0_SAMPLES;
     const double g = f64_interval(d);
-    for (int j = lane; j < TAU0_SAMPLES; j += 64) {
-        uint32_t bits = 0x7f800000u;
-        if (j < samples) {
-            const int64_t row = (int64_t)j * n / samples;
-            if (!(exclude && row == self_off + q)) {
-                const double s = fast_sq(Q + q * d, X + row * d, d);
-                bits = __float_as_uint(__double2float_ru(s * (1.0 + g) * c.scale2));
+    const float* qv = Q + q * d;
+    const int gl = lane & 15;
+    const int64_t stride = n / samples;  // sample j is row j * stride
+    // four sample rows per 16-lane group and pass; every load of a
+    // 128-feature slice is issued before the arithmetic
+    for (int j0 = 0; j0 < TAU0_SAMPLES; j0 += 16) {
+        int64_t row[4];
+        const float* xp[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int j = j0 + (lane >> 4) + 4 * u;
+            row[u] = j < samples ? (int64_t)j * stride : -1;
+            xp[u] = X + (row[u] < 0 ? 0 : row[u]) * d;
+        }
+        double part[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int e0 = 0; e0 < d; e0 += 128) {
+            // unconditional loads from clamped addresses (issued together)
+            float qq[8], xx[4][8];
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const int e = e0 + gl + 16 * t;
+                const int ec = e < d ? e : d - 1;
+                qq[t] = qv[ec];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) xx[u][t] = xp[u][ec];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int t = 0; t < 8; ++t) {
+                    const double dt = e0 + gl + 16 * t < d ? (double)qq[t] - (double)xx[u][t] : 0.0;
+                    part[u] += dt * dt;
+                }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            for (int o = 8; o > 0; o >>= 1) part[u] += __shfl_xor(part[u], o, 16);
+            const int j = j0 + (lane >> 4) + 4 * u;
+            if (gl == 0) {
+                uint32_t bits = 0x7f800000u;
+                if (row[u] >= 0 && !(exclude && row[u] == self_off + q))
+                    bits = __float_as_uint(__double2float_ru(part[u] * (1.0 + g) * c.scale2));
+                sb[w][j] = bits;
             }
         }
-        sb[w][j] = bits;
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -114,6 +151,25 @@ __global__ void k_tau0(const float* __restrict__ X, int64_t n, int d, const floa
     }
 }
 
+// Sums of the per-query rank statistics (three counts per query) in a fixed order.
+__global__ void k_sum_qstats(const int* __restrict__ qs, int64_t m, unsigned long long* __restrict__ out) {
+    __shared__ unsigned long long sh[3][256];
+    unsigned long long a0 = 0, a1 = 0, a2 = 0;
+    for (int64_t q = threadIdx.x; q < m; q += blockDim.x) {
+        a0 += (unsigned)qs[3 * q];
+        a1 += (unsigned)qs[3 * q + 1];
+        a2 += (unsigned)qs[3 * q + 2];
+    }
+    sh[0][threadIdx.x] = a0; sh[1][threadIdx.x] = a1; sh[2][threadIdx.x] = a2;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o)
+            for (int c = 0; c < 3; ++c) sh[c][threadIdx.x] += sh[c][threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) out[threadIdx.x] = sh[threadIdx.x][0];
+}
+
 // ---------------------------------------------------------------- merge
 // Incremental merge after a round. Per query, kbest holds the k smallest
 // certified upper bounds over every candidate merged so far; the entries
@@ -122,7 +178,7 @@ __global__ void k_tau0(const float* __restrict__ X, int64_t n, int d, const floa
 // (an upper bound of the query's k-th smallest S), and kbest is rewritten with
 // the k smallest values of the union.
 __global__ void k_merge(int64_t m, int k, const int* __restrict__ cnt, int* __restrict__ seen,
-                        const float* __restrict__ cP, const float* __restrict__ cN, int S2, int cap,
+                        const int2* __restrict__ cand, const float* __restrict__ nx32, int S2, int cap,
                         float* __restrict__ kbest, const double* __restrict__ nq, Consts c, float* __restrict__ tau,
                         float* __restrict__ thr, int* __restrict__ status) {
     __shared__ uint32_t sU[4][MCAP];
@@ -166,11 +222,22 @@ __global__ void k_merge(int64_t m, int k, const int* __restrict__ cnt, int* __re
         }
         int at = base + incl - mine;
         const double nqq = nq[q];
-        for (int j = lo; j < hi; ++j) {
-            const size_t off = ((size_t)q * S2 + s) * cap + j;
-            double L, U;
-            hit_bounds(cP[off], cN[off], nqq, c, L, U);
-            buf[at++] = __float_as_uint(fmaxf(__double2float_ru(U), 0.0f));
+        const size_t off = ((size_t)q * S2 + s) * cap;
+        for (int j0 = lo; j0 < hi; j0 += 8) {
+            // eight entries per batch: loads and norm gathers issued together
+            int2 e[8];
+            float nxv[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) e[u] = cand[off + (j0 + u < hi ? j0 + u : hi - 1)];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) nxv[u] = nx32[e[u].x];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                if (j0 + u >= hi) continue;
+                double L, U;
+                hit_bounds(__int_as_float(e[u].y), nxv[u], nqq, c, L, U);
+                buf[at++] = __float_as_uint(fmaxf(__double2float_ru(U), 0.0f));
+            }
         }
         base += __shfl(incl, 63);
     }
@@ -206,9 +273,8 @@ struct RankArgs {
     int k;
     // screen candidate lists
     const int* cnt;
-    const int* cidx;
-    const float* cP;
-    const float* cN;
+    const int2* cand;
+    const float* nx32;
     const double* nq;
     int S, cap;
     const float* tau;
@@ -228,7 +294,7 @@ struct RankArgs {
     int64_t* out_idx;
     double* out_dist;
     int squared;
-    unsigned long long* counters;  // [0] list entries, [1] after the bound filter, [2] ranked
+    int* qstats;  // per query: list entries, after the bound filter, ranked
 };
 
 struct Cand {
@@ -247,6 +313,8 @@ __device__ __forceinline__ Cand shfl_cand(Cand v, int o) {
 __global__ __launch_bounds__(256) void k_rank(const RankArgs a, int64_t n_items) {
     __shared__ int sI[4][LCAP];
     __shared__ double sS[4][LCAP];
+    __shared__ int sN[4];
+    __shared__ int sO[4][64];  // result rows by rank
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int64_t item = (int64_t)blockIdx.x * 4 + w;
     if (item >= n_items) return;
@@ -267,9 +335,13 @@ __global__ __launch_bounds__(256) void k_rank(const RankArgs a, int64_t n_items)
         const float tau = a.tau[q];
         const double nqq = a.nq[q];
         const int64_t self = a.exclude ? a.self_off + q : -1;
-        // Rows whose certified lower bound does not exceed tau, compacted in
-        // two passes (count per lane, exclusive scan, place).
-        int mine = 0, listed = 0;
+        // Rows whose certified lower bound does not exceed tau, appended in one
+        // pass through an LDS counter (the rank below does not depend on the
+        // order of the candidates).
+        if (lane == 0) sN[w] = 0;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        int listed = 0;
         bool over = false;
         for (int s0 = 0; s0 < a.S; s0 += 64) {
             const int s = s0 + lane;
@@ -281,36 +353,38 @@ __global__ __launch_bounds__(256) void k_rank(const RankArgs a, int64_t n_items)
             }
             listed += cs;
             const size_t off = ((size_t)q * a.S + s) * a.cap;
-            for (int j = 0; j < cs; ++j) {
-                double L, U;
-                hit_bounds(a.cP[off + j], a.cN[off + j], nqq, a.c, L, U);
-                mine += L <= (double)tau;
+            // eight entries at a time: their loads and norm gathers are issued
+            // together, so a batch costs one memory latency per step
+            for (int j0 = 0; j0 < cs; j0 += 8) {
+                // unconditional loads from clamped (valid) slots, so the
+                // compiler issues them together; slots past cs are ignored
+                int2 e[8];
+                float nxv[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) e[u] = a.cand[off + (j0 + u < cs ? j0 + u : cs - 1)];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) nxv[u] = a.nx32[e[u].x];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    if (j0 + u >= cs) continue;
+                    double L, U;
+                    hit_bounds(__int_as_float(e[u].y), nxv[u], nqq, a.c, L, U);
+                    if (L <= (double)tau) {
+                        const int at = atomicAdd(&sN[w], 1);
+                        if (at < LCAP) li[at] = e[u].x;
+                    }
+                }
             }
         }
-        int incl = mine;
-        for (int o = 1; o < 64; o <<= 1) {
-            const int v = __shfl_up(incl, o);
-            if (lane >= o) incl += v;
-        }
-        const int total = __shfl(incl, 63);
         for (int o = 32; o > 0; o >>= 1) over = __shfl_xor((int)over, o) | over;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        const int total = sN[w];
         int extra = 0;
         for (int u = 0; u < a.n_unsafe; ++u) extra += a.unsafe[u] != self;
         if (over || total + extra > LCAP) {
             if (lane == 0) a.status[q] = ST_OVERFLOW;
             return;
-        }
-        int at = incl - mine;
-        for (int s0 = 0; s0 < a.S; s0 += 64) {
-            const int s = s0 + lane;
-            int cs = 0;
-            if (s < a.S) { cs = a.cnt[q * a.S + s]; cs = cs < a.cap ? cs : a.cap; }
-            const size_t off = ((size_t)q * a.S + s) * a.cap;
-            for (int j = 0; j < cs; ++j) {
-                double L, U;
-                hit_bounds(a.cP[off + j], a.cN[off + j], nqq, a.c, L, U);
-                if (L <= (double)tau) li[at++] = a.cidx[off + j];
-            }
         }
         if (lane == 0) {
             int u2 = total;
@@ -324,32 +398,60 @@ __global__ __launch_bounds__(256) void k_rank(const RankArgs a, int64_t n_items)
         // interval tests use the order-free bound g)
         const double cut = (double)tau / a.c.scale2;
         int kept = 0;
-        for (int b = 0; b < count; b += 4) {
-            const int j = b + (lane >> 4);
-            const int gl = lane & 15;
-            double sq = 0.0;
-            if (j < count) {
-                const float* xv = a.X + (size_t)li[j] * d;
-                for (int e = gl; e < d; e += 16) {
-                    const double t = (double)qv[e] - (double)xv[e];
-                    sq += t * t;
+        const int gl = lane & 15;
+        for (int b = 0; b < count; b += 8) {
+            // 16 lanes per candidate, two candidates per lane group; all loads
+            // of a 128-feature slice are issued before the arithmetic
+            const int j0 = b + (lane >> 4), j1 = j0 + 4;
+            const float* x0 = a.X + (size_t)(j0 < count ? li[j0] : 0) * d;
+            const float* x1 = a.X + (size_t)(j1 < count ? li[j1] : 0) * d;
+            double s0 = 0.0, s1 = 0.0;
+            for (int e0 = 0; e0 < d; e0 += 128) {
+                // unconditional loads (features clamped to d - 1, rows to row 0
+                // past the list); padded terms are zeroed after the loads
+                float qq[8], xa[8], xb[8];
+#pragma unroll
+                for (int t = 0; t < 8; ++t) {
+                    const int e = e0 + gl + 16 * t;
+                    const int ec = e < d ? e : d - 1;
+                    qq[t] = qv[ec];
+                    xa[t] = x0[ec];
+                    xb[t] = x1[ec];
+                }
+#pragma unroll
+                for (int t = 0; t < 8; ++t) {
+                    const bool in = e0 + gl + 16 * t < d;
+                    const double ta = in ? (double)qq[t] - (double)xa[t] : 0.0;
+                    const double tb = in ? (double)qq[t] - (double)xb[t] : 0.0;
+                    s0 += ta * ta;
+                    s1 += tb * tb;
                 }
             }
-            for (int o = 8; o > 0; o >>= 1) sq += __shfl_xor(sq, o, 16);
-            if (j < count && gl == 0) {
-                const bool ok = sq * (1.0 - g) <= cut;
-                ls[j] = ok ? sq : INFINITY;
-                kept += ok;
+            for (int o = 8; o > 0; o >>= 1) {
+                s0 += __shfl_xor(s0, o, 16);
+                s1 += __shfl_xor(s1, o, 16);
+            }
+            if (gl == 0) {
+                if (j0 < count) {
+                    const bool ok = s0 * (1.0 - g) <= cut;
+                    ls[j0] = ok ? s0 : INFINITY;
+                    kept += ok;
+                }
+                if (j1 < count) {
+                    const bool ok = s1 * (1.0 - g) <= cut;
+                    ls[j1] = ok ? s1 : INFINITY;
+                    kept += ok;
+                }
             }
         }
         for (int o = 32; o > 0; o >>= 1) {
             kept += __shfl_xor(kept, o);
             listed += __shfl_xor(listed, o);
         }
-        if (lane == 0 && a.counters) {
-            atomicAdd(&a.counters[0], (unsigned long long)listed);
-            atomicAdd(&a.counters[1], (unsigned long long)count);
-            atomicAdd(&a.counters[2], (unsigned long long)kept);
+        if (lane == 0 && a.qstats) {
+            a.qstats[3 * q] = listed;
+            a.qstats[3 * q + 1] = count;
+            a.qstats[3 * q + 2] = kept;
         }
         if (kept < k) {
             if (lane == 0) a.status[q] = ST_OVERFLOW;
@@ -395,6 +497,10 @@ __global__ __launch_bounds__(256) void k_rank(const RankArgs a, int64_t n_items)
         }
     }
     // rank counting: position of each kept candidate in the exact (S, index) order
+    int* orow = sO[w];
+    if (lane < 64) orow[lane] = 0;  // every slot holds a valid row even if a rank stays unset
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     for (int j = lane; j < count; j += 64) {
         const double sj = ls[j];
         if (!(sj < INFINITY)) continue;
@@ -407,11 +513,52 @@ __global__ __launch_bounds__(256) void k_rank(const RankArgs a, int64_t n_items)
         }
         if (rank < k) {
             a.out_idx[q * k + rank] = ij;
-            if (a.out_dist) {
+            if (k <= 64) {
+                orow[rank] = (int)ij;
+            } else if (a.out_dist) {
+                // fallback route with k > 64: no staging, the lane reads its row
                 const double sc = canonical_sq(qv, a.X + (size_t)ij * d, d);
                 a.out_dist[q * k + rank] = a.squared ? sc : sqrt(sc);
             }
         }
+    }
+    if (!a.out_dist || k > 64) return;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    // Canonical distances of the k results: the query and a chunk of result
+    // rows are copied into LDS (the candidate distance array is free now) with
+    // coalesced loads; then one lane per row forms the left-to-right sum.
+    float* stage = (float*)ls;
+    const int rs = d + 1;  // padded row stride
+    const int ch = d < LCAP * 2 ? (LCAP * 2 - d) / rs : 0;
+    if (ch == 0) {
+        for (int r = lane; r < k; r += 64) {
+            const double sc = canonical_sq(qv, a.X + (size_t)orow[r] * d, d);
+            a.out_dist[q * k + r] = a.squared ? sc : sqrt(sc);
+        }
+        return;
+    }
+    float* sq_ = stage + ch * rs;
+    for (int e = lane; e < d; e += 64) sq_[e] = qv[e];
+    for (int r0 = 0; r0 < k; r0 += ch) {
+        const int nr = k - r0 < ch ? k - r0 : ch;
+        for (int rr = 0; rr < nr; ++rr) {
+            const float* xv = a.X + (size_t)orow[r0 + rr] * d;
+            for (int e = lane; e < d; e += 64) stage[rr * rs + e] = xv[e];
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        if (lane < nr) {
+            const float* xr = stage + lane * rs;
+            double sc = 0.0;
+            for (int e = 0; e < d; ++e) {
+                const double t = __dsub_rn((double)sq_[e], (double)xr[e]);
+                sc = __dadd_rn(sc, __dmul_rn(t, t));
+            }
+            a.out_dist[q * k + r0 + lane] = a.squared ? sc : sqrt(sc);
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     }
 }
 
@@ -660,9 +807,7 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
     if (!all_fallback) {
         const int S2 = 2 * S;  // candidate sub-lists per query (split x lane half)
         MLK_TRY(ws[8].reserve((size_t)m_pad * S2 * sizeof(int)));          // counts
-        MLK_TRY(ws[9].reserve((size_t)m_pad * S2 * cap * sizeof(int)));    // rows
-        MLK_TRY(ws[10].reserve((size_t)m_pad * S2 * cap * sizeof(float))); // dot-product estimates
-        MLK_TRY(ws[16].reserve((size_t)m_pad * S2 * cap * sizeof(float))); // row norms
+        MLK_TRY(ws[9].reserve((size_t)m_pad * S2 * cap * sizeof(int2)));   // (row, dot-product estimate)
         MLK_TRY(ws[17].reserve((size_t)m_pad * S2 * sizeof(int)));         // merged counts
         MLK_TRY(ws[18].reserve((size_t)m_pad * k * sizeof(float)));        // k smallest upper bounds
         MLK_CHECK_HIP(hipMemsetAsync(ws[8].ptr, 0, (size_t)m_pad * S2 * sizeof(int), stream));
@@ -676,9 +821,7 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
         sa.qs = ws[0].ptr;
         sa.thr = thr;
         sa.cnt = ws[8].as<int>();
-        sa.cidx = ws[9].as<int>();
-        sa.cP = ws[10].as<float>();
-        sa.cN = ws[16].as<float>();
+        sa.cand = ws[9].as<int2>();
         sa.n = n;
         sa.split_rows = (n + S - 1) / S;
         sa.S = S;
@@ -708,7 +851,7 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
             ++st.rounds;
             if (r + 1 < plan.rounds.size()) {
                 hipLaunchKernelGGL(k_merge, dim3((unsigned)((m + 3) / 4)), dim3(256), 0, stream, m, k, sa.cnt,
-                                   ws[17].as<int>(), sa.cP, sa.cN, S2, cap, ws[18].as<float>(), ws[1].as<double>(), c, tau, thr,
+                                   ws[17].as<int>(), sa.cand, sa.nx32, S2, cap, ws[18].as<float>(), ws[1].as<double>(), c, tau, thr,
                                    status);
                 MLK_CHECK_HIP(hipGetLastError());
             }
@@ -716,22 +859,26 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
         round_events = std::move(round_ev);
         // final overflow check (the last round has no merge)
         hipLaunchKernelGGL(k_merge, dim3((unsigned)((m + 3) / 4)), dim3(256), 0, stream, m, k, sa.cnt,
-                           ws[17].as<int>(), sa.cP, sa.cN, S2, cap, ws[18].as<float>(), ws[1].as<double>(), c, tau, thr,
+                           ws[17].as<int>(), sa.cand, sa.nx32, S2, cap, ws[18].as<float>(), ws[1].as<double>(), c, tau, thr,
                                    status);
         MLK_CHECK_HIP(hipGetLastError());
         MLK_CHECK_HIP(hipEventRecord(ev2, stream));
 
         RankArgs ra{};
         ra.X = X; ra.n = n; ra.d = d; ra.Q = Q; ra.m = m; ra.k = k;
-        ra.cnt = sa.cnt; ra.cidx = sa.cidx; ra.cP = sa.cP; ra.cN = sa.cN; ra.nq = ws[1].as<double>();
+        ra.cnt = sa.cnt; ra.cand = sa.cand; ra.nx32 = sa.nx32; ra.nq = ws[1].as<double>();
         ra.S = S2; ra.cap = cap;
         ra.tau = tau; ra.c = c;
         ra.unsafe = mr.unsafe.as<int>(); ra.n_unsafe = (int)mr.n_unsafe;
         ra.mode = 0; ra.self_off = q_offset; ra.exclude = exclude; ra.status = status;
         ra.out_idx = idx; ra.out_dist = dist; ra.squared = (flags & MLK_SQUARED_DIST) != 0;
-        ra.counters = (unsigned long long*)(dflag + 4);
-        MLK_CHECK_HIP(hipMemsetAsync(ra.counters, 0, 3 * sizeof(unsigned long long), stream));
+        MLK_TRY(ws[19].reserve((size_t)m * 3 * sizeof(int)));
+        ra.qstats = ws[19].as<int>();
+        MLK_CHECK_HIP(hipMemsetAsync(ra.qstats, 0, (size_t)m * 3 * sizeof(int), stream));
         hipLaunchKernelGGL(k_rank, dim3((unsigned)((m + 3) / 4)), dim3(256), 0, stream, ra, m);
+        MLK_CHECK_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_sum_qstats, dim3(1), dim3(256), 0, stream, ra.qstats, m,
+                           (unsigned long long*)(dflag + 4));
         MLK_CHECK_HIP(hipGetLastError());
     } else {
         MLK_CHECK_HIP(hipEventRecord(ev2, stream));

@@ -40,19 +40,40 @@ __device__ __forceinline__ void hit_bounds(float p32, float nx32, double nq, con
 }
 
 // k-th smallest (1-based) of non-negative float values spread over a wave:
-// binary search on the bit pattern. Returns +inf bits if fewer than k values.
+// binary search on the bit pattern, counting with ballots (no shuffles).
+// Up to 256 values stay in registers for the whole search. Returns the
+// largest pattern with fewer than k values strictly below it (+inf bits if
+// there are fewer than k values).
 template <class GetFn>
 __device__ uint32_t wave_kth_bits(int count, int k, GetFn get) {
     const int lane = threadIdx.x & 63;
     uint32_t ans = 0;
+    if (count <= 256) {
+        uint32_t v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int j = lane + 64 * u;
+            v[u] = j < count ? get(j) : 0xffffffffu;
+        }
+        for (int b = 31; b >= 0; --b) {
+            const uint32_t trial = ans | (1u << b);
+            int below = 0;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) below += __popcll(ballot(v[u] < trial));
+            if (below < k) ans = trial;
+        }
+        return ans;
+    }
     for (int b = 31; b >= 0; --b) {
         const uint32_t trial = ans | (1u << b);
         int below = 0;
-        for (int j = lane; j < count; j += 64) below += get(j) < trial;
-        for (int o = 32; o > 0; o >>= 1) below += __shfl_xor(below, o);
+        for (int j0 = 0; j0 < count; j0 += 64) {
+            const int j = j0 + lane;
+            below += __popcll(ballot(j < count && get(j) < trial));
+        }
         if (below < k) ans = trial;
     }
-    return ans;  // largest pattern with fewer than k values strictly below it
+    return ans;
 }
 
 // ---------------------------------------------------------------- screen
@@ -63,9 +84,7 @@ struct ScreenArgs {
     const void* qs;      // query screen mirror, m_pad x DP
     const float* thr;    // per query: hit threshold on the accumulator
     int* cnt;            // per (query, split, lane half): appended count
-    int* cidx;           // candidate rows
-    float* cP;           // acc + a_x (the screen's dot product estimate), float32
-    float* cN;           // the row's squared norm, float32
+    int2* cand;          // candidates: (row, float32 bits of p = acc + a_x, the screen's dot product)
     int64_t n;
     int64_t split_rows;
     int64_t off_lo, off_hi;

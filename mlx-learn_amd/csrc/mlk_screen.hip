@@ -317,24 +317,34 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
                     if (!ballot(hit_rq[rt][qt])) continue;
                     const int64_t qg = qrow0 + qt * 32 + col;
                     const int64_t self = a.exclude ? a.self_off + qg : -1;
-                    // rolled loop: the register index i is uniform (v_movrels)
+                    if constexpr (MODE == SCREEN_LISTS) {
+                        // wave-uniform mask of the accumulator registers with a hit
+                        // in some lane; only those registers are visited (uniform
+                        // register index, v_movrels)
+                        uint32_t regs = 0;
+#pragma unroll
+                        for (int i = 0; i < 16; ++i) regs |= (ballot(acc[rt][qt][i] >= thr[qt]) != 0 ? 1u : 0u) << i;
 #pragma unroll 1
-                    for (int i = 0; i < 16; ++i) {
-                        const float v = acc[rt][qt][i];
-                        const int rl = rt * 32 + mfma32_row(i, lane);
-                        const int64_t row = r0 + rl;
-                        const bool hit = v >= thr[qt] && row < r_end && row != self;
-                        if constexpr (MODE == SCREEN_LISTS) {
-                            if (hit) {
+                        while (regs) {
+                            const int i = __builtin_ctz(regs);
+                            regs &= regs - 1;
+                            const float v = acc[rt][qt][i];
+                            const int rl = rt * 32 + mfma32_row(i, lane);
+                            const int64_t row = r0 + rl;
+                            if (v >= thr[qt] && row < r_end && row != self) {
                                 const int slot = cnt_r[qt]++;
-                                if (slot < a.cap) {
-                                    const size_t o = list_base[qt] + slot;
-                                    a.cidx[o] = (int)row;
-                                    a.cP[o] = v - cB[rl];
-                                    a.cN[o] = cB[64 + rl];
-                                }
+                                if (slot < a.cap)
+                                    a.cand[list_base[qt] + slot] = int2{(int)row, __float_as_int(v - cB[rl])};
                             }
-                        } else {
+                        }
+                    } else {
+                        // rolled loop: the register index i is uniform (v_movrels)
+#pragma unroll 1
+                        for (int i = 0; i < 16; ++i) {
+                            const float v = acc[rt][qt][i];
+                            const int rl = rt * 32 + mfma32_row(i, lane);
+                            const int64_t row = r0 + rl;
+                            const bool hit = v >= thr[qt] && row < r_end && row != self;
                             if (ballot(hit)) {
                                 if (sPairN[wave] + 64 > PAIR_STAGE) flush_pairs();
                                 if (hit) {
