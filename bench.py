@@ -100,6 +100,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5"],
+                    help="c2 (default) is the headline; c1, c3, c4, c5 are the other configurations")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -109,9 +111,14 @@ def main():
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} (WORLD_SIZE={world}) is not supported. "
-              "The database would shard by contiguous row ranges with an exact (distance, index) merge; "
-              "see DESIGN.md.", file=sys.stderr)
+              "The database would shard by contiguous row ranges with an exact (distance, index) merge.",
+              file=sys.stderr)
         sys.exit(2)
+    if args.workload != "c2":
+        import bench_workloads
+        line = bench_workloads.run(args)
+        print(json.dumps(line), flush=True)
+        return
 
     import numpy as np
     import torch

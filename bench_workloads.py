@@ -1,0 +1,274 @@
+"""The other benchmark configurations of bench.py (``--workload c1|c3|c4|c5``).
+
+Each prints one JSON line like the headline (c2): whole-job throughput with
+inputs resident in HBM, the screen kernel's roofline fraction (algorithmic
+flops over the summed device time of its launches, from HIP events on the
+library's stream), an oracle check on a sample, and a CPU baseline timed on a
+bounded sample on the host cores.
+
+* c1  NearestNeighbors(k=5) on 10,000 x 64 N(0,1), 1,000 queries, plain fp32 MFMA screen.
+* c3  KNeighborsClassifier(k=3, uniform) on 60,000 x 784 integer pixels, 10,000
+      queries: fp16 screen (integer data are exact in fp16) + exact rank + device votes.
+* c4  radius_neighbors on 2,000,000 x 32 (uniform cube + 20% clumps), 256
+      batches of 1..4096 queries in one call (CSR split per batch), fp32 screen,
+      rows sorted by (distance, index).
+* c5  KMeans: one exact assignment of 10,000,000 x 96 rows to 65,536 centroids
+      (bf16 screen, k = 1) + one float64 centroid update.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "mlx-learn_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+PEAK = {"float32": 157.3, "bfloat16": 2500.0, "float16": 2500.0}  # TFLOP/s, MI355X dense
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def timed(fn, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = None
+    for _ in range(steps):
+        out = fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps, out
+
+
+def base_line(args, metric, value, unit, ms, dtype, data, config):
+    return {"metric": metric, "value": round(value, 1), "unit": unit, "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": dtype, "data": data, "config": config}
+
+
+def roofline(flops, screen_ms, precision, kernel, launches):
+    ach = flops / (screen_ms * 1e-3) / 1e12
+    return {"bound": "mfma", "achieved": round(ach, 2), "peak": PEAK[precision], "unit": "TFLOP/s",
+            "frac": round(ach / PEAK[precision], 4), "traffic": None, "kernel": kernel,
+            "launches_per_step": launches, "screen_ms_per_step": round(screen_ms, 3)}
+
+
+def knn_step(index, Qd, k, precision):
+    from mlxlearn_amd import _lib
+    def step():
+        d, i = index.query(Qd, k, precision=precision)
+        return d, i, _lib.stats()
+    return step
+
+
+def run_c1(args):
+    import torch
+    import mlxlearn_amd
+    import oracle
+    X, Q = oracle.gen_c1(seed=args.seed)
+    index = mlxlearn_amd.DeviceNeighborIndex(torch.from_numpy(X).cuda())
+    Qd = torch.from_numpy(Q).cuda()
+    res = {}
+    for prec in ("float32", "bfloat16"):
+        t, (d, i, st) = timed(knn_step(index, Qd, 5, prec), args.steps, args.warmup)
+        res[prec] = (t, d.cpu().numpy(), i.cpu().numpy(), st)
+        log(f"c1 {prec}: {t * 1e3:.3f} ms/step, stats {st}")
+    wd, wi = oracle.knn(X, Q, 5)
+    t, d, i, st = res["float32"]
+    match = float((i == wi).all(axis=1).mean())
+    match_bf = float((res["bfloat16"][2] == wi).all(axis=1).mean())
+    flops = 2.0 * Q.shape[0] * X.shape[0] * X.shape[1]
+    line = base_line(args, "exact kNN queries/s 10k x 64 k=5 (plain fp32 path)", Q.shape[0] / t, "queries/s",
+                     t * 1e3, "fp32", "synthetic (seeded N(0,1))",
+                     {"workload": "c1: NearestNeighbors(k=5), database 10,000x64 float32, 1,000 queries, fp32 MFMA "
+                                  "screen + fp64 exact rerank", "n": 10000, "d": 64, "queries": 1000, "k": 5})
+    line["roofline"] = roofline(flops, st["screen_ms"], "float32", "k_screen<fp32,64>", st["rounds"])
+    line["bf16_path"] = {"value": round(Q.shape[0] / res["bfloat16"][0], 1),
+                         "ms_per_step": round(res["bfloat16"][0] * 1e3, 3)}
+    line["index_match"] = {"fp32_vs_oracle": match, "bf16_vs_oracle": match_bf, "oracle_sample": Q.shape[0],
+                           "max_rel_dist_err": float(np.max(np.abs(d - wd) / np.maximum(wd, 1e-300)))}
+    if not args.no_cpu:
+        port = oracle.GramPort(X)
+        t0 = time.perf_counter()
+        port.query(Q, 5, block=256)
+        tc = time.perf_counter() - t0
+        line["cpu_baseline"] = {"value": round(Q.shape[0] / tc, 1), "unit": "queries/s",
+                                "cores": len(os.sched_getaffinity(0)), "kind": "port",
+                                "sample": "all 1,000 queries (numpy float64 BLAS CPUIndex._gram restatement)"}
+    return line
+
+
+def numpy_votes(idx, codes, n_classes):
+    votes = np.zeros((idx.shape[0], n_classes))
+    for j in range(idx.shape[1]):
+        np.add.at(votes, (np.arange(idx.shape[0]), codes[idx[:, j]]), 1.0)
+    return votes.argmax(1)
+
+
+def run_c3(args):
+    import torch
+    import mlxlearn_amd
+    import oracle
+    from mlxlearn_amd import _lib
+    X, y, Q = oracle.gen_c3(seed=args.seed)
+    k, n_classes = 3, 10
+    index = mlxlearn_amd.DeviceNeighborIndex(torch.from_numpy(X).cuda())
+    Qd = torch.from_numpy(Q).cuda()
+    codes = torch.as_tensor(y.astype(np.int32)).cuda()
+    lib = _lib.load()
+    m = Q.shape[0]
+    pred = torch.empty(m, dtype=torch.int64, device="cuda")
+    ptr = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+
+    def step(prec):
+        def f():
+            _, i = index.query(Qd, k, return_distance=False, precision=prec)
+            st = _lib.stats()
+            _lib.check(lib.mlk_knn_votes(ptr(i), ctypes.c_void_p(0), m, k, ptr(codes), n_classes,
+                                         _lib.WEIGHTS_UNIFORM, ptr(pred), ctypes.c_void_p(0),
+                                         ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+            return i, st
+        return f
+
+    res = {}
+    for prec in ("float16", "float32"):
+        t, (i, st) = timed(step(prec), args.steps, args.warmup)
+        res[prec] = (t, i.cpu().numpy(), pred.cpu().numpy(), st)
+        log(f"c3 {prec}: {t * 1e3:.3f} ms/step, stats {st}")
+    sample = min(args.oracle_sample, m)
+    _, wi = oracle.knn(X, Q[:sample], k)
+    want_pred = numpy_votes(wi, y, n_classes)
+    t, i, p, st = res["float16"]
+    flops = 2.0 * m * X.shape[0] * X.shape[1]
+    line = base_line(args, "KNeighborsClassifier.predict queries/s 60k x 784 k=3", m / t, "queries/s", t * 1e3,
+                     "fp16", "synthetic (seeded 28x28 integer blob images, 10 labels)",
+                     {"workload": "c3: KNeighborsClassifier(k=3, uniform), database 60,000x784 integer pixels, "
+                                  "10,000 queries, fp16 MFMA screen + exact rank + device votes",
+                      "n": X.shape[0], "d": X.shape[1], "queries": m, "k": k})
+    line["roofline"] = roofline(flops, max(st["screen_ms"], 1e-9), "float16", "k_screen<fp16,784>",
+                                st["rounds"]) if st["rounds"] else None
+    line["fp32_path"] = {"value": round(m / res["float32"][0], 1), "ms_per_step": round(res["float32"][0] * 1e3, 3)}
+    line["index_match"] = {"fp16_vs_oracle": float((i[:sample] == wi).all(axis=1).mean()),
+                           "fp16_vs_fp32": float((i == res["float32"][1]).all(axis=1).mean()),
+                           "pred_vs_oracle": float((p[:sample] == want_pred).mean()), "oracle_sample": sample}
+    line["screen_stats"] = {key: st[key] for key in ("candidates", "reranked", "fallback_queries", "rounds")}
+    if not args.no_cpu:
+        port = oracle.GramPort(X)
+        nq = min(args.cpu_sample, m)
+        t0 = time.perf_counter()
+        port.query(Q[:nq], k, block=128)
+        tc = time.perf_counter() - t0
+        line["cpu_baseline"] = {"value": round(nq / tc, 1), "unit": "queries/s", "cores": len(os.sched_getaffinity(0)),
+                                "kind": "port", "sample": f"{nq} queries x 60,000 rows x 784 (numpy float64 BLAS "
+                                                         "CPUIndex._gram restatement; votes excluded)"}
+    return line
+
+
+def run_c4(args):
+    import torch
+    import mlxlearn_amd
+    import oracle
+    from mlxlearn_amd import _lib
+    X, Q, sizes, radius = oracle.gen_c4(seed=args.seed)
+    R2 = radius * radius
+    index = mlxlearn_amd.DeviceNeighborIndex(torch.from_numpy(X).cuda())
+    Qd = torch.from_numpy(Q).cuda()
+    offsets = np.concatenate([[0], np.cumsum(sizes)])
+
+    def step():
+        # every batch in one call: the CSR row pointers split it per batch
+        indptr, indices, dist = index.radius(Qd, R2, sort_results=True, precision="float32")
+        return indptr, indices, dist, _lib.stats()
+
+    t, (indptr, indices, dist, st) = timed(step, args.steps, args.warmup)
+    indptr = indptr.cpu().numpy()
+    log(f"c4: {t * 1e3:.3f} ms/step, nnz {indptr[-1]}, stats {st}")
+    # oracle check: the first query of a spread of batches
+    picks = [int(offsets[b]) for b in range(0, len(sizes), max(1, len(sizes) // 16))]
+    ind_h, dist_h = indices.cpu().numpy(), dist.cpu().numpy()
+    ok = 0
+    for q in picks:
+        wp, wi, wd = oracle.radius(X, Q[q:q + 1], R2, sort_results=True)
+        got_i = ind_h[indptr[q]:indptr[q + 1]]
+        got_d = dist_h[indptr[q]:indptr[q + 1]]
+        ok += int(np.array_equal(got_i, wi) and np.allclose(got_d, wd, rtol=1e-12, atol=0))
+    m = Q.shape[0]
+    flops = 2.0 * m * X.shape[0] * X.shape[1]
+    line = base_line(args, "radius_neighbors queries/s 2M x 32, 256 batches, sorted CSR", m / t, "queries/s", t * 1e3,
+                     "fp32", "synthetic (seeded uniform cube + 20% Gaussian clumps, calibrated radius)",
+                     {"workload": f"c4: radius_neighbors(r={radius}, sort_results=True), database 2,000,000x32 float32, "
+                                  f"{len(sizes)} batches ({m} queries) in one call, fp32 MFMA screen + exact band",
+                      "n": X.shape[0], "d": X.shape[1], "queries": m, "batches": len(sizes)})
+    line["roofline"] = roofline(flops, st["screen_ms"], "float32", "k_screen<fp32,32,pairs>", 1)
+    line["result"] = {"nnz": int(indptr[-1]), "median_per_query": float(np.median(np.diff(indptr))),
+                      "oracle_rows_checked": len(picks), "oracle_rows_equal": ok, "candidates": st["candidates"]}
+    if not args.no_cpu:
+        nq = min(64, m)
+        t0 = time.perf_counter()
+        oracle.radius(X, Q[:nq], R2, sort_results=True)
+        tc = time.perf_counter() - t0
+        line["cpu_baseline"] = {"value": round(nq / tc, 1), "unit": "queries/s",
+                                "cores": oracle.default_threads(), "kind": "port",
+                                "sample": f"{nq} queries x 2,000,000 rows (C restatement of the canonical float64 "
+                                          "brute force with the exact band, all cores)"}
+    return line
+
+
+def run_c5(args):
+    import torch
+    import oracle
+    from mlxlearn_amd.cluster._k_means import assign_device, update_device
+    n = args.n if args.n != 1_000_000 else 10_000_000
+    X, C = oracle.gen_c5(seed=args.seed, n=n)
+    kc = C.shape[0]
+    Xd = torch.from_numpy(X).cuda()
+    Cd = torch.from_numpy(C).cuda()
+    from mlxlearn_amd import _lib
+
+    def step():
+        labels, resid = assign_device(Xd, Cd, precision="bfloat16")
+        st = _lib.stats()
+        sums, counts, inertia = update_device(Xd, labels, kc, resid)
+        return labels, resid, counts, inertia, st
+
+    t, (labels, resid, counts, inertia, st) = timed(step, args.steps, args.warmup)
+    log(f"c5: {t * 1e3:.3f} ms/step, stats {st}")
+    sample = min(4096, n)
+    rows = np.random.default_rng(1).choice(n, sample, replace=False)
+    rows.sort()
+    wl, wr = oracle.kmeans_assign(X[rows], C)
+    lab = labels.cpu().numpy()
+    res = resid.cpu().numpy()
+    flops = 2.0 * n * kc * X.shape[1]
+    line = base_line(args, "KMeans assignment + update rows/s 10M x 96, 65,536 centroids", n / t, "rows/s", t * 1e3,
+                     "bf16", "synthetic (seeded mixture of 4,096 Gaussians, centroids sampled from rows)",
+                     {"workload": f"c5: KMeans one exact assignment (bf16 MFMA screen, k=1) + one float64 update, "
+                                  f"{n}x96 float32 rows, {kc} centroids", "n": n, "d": X.shape[1], "clusters": kc})
+    line["roofline"] = roofline(flops, st["screen_ms"], "bfloat16", "k_screen<bf16,96>", st["rounds"])
+    line["check"] = {"labels_vs_oracle": float((lab[rows] == wl).mean()), "oracle_sample_rows": sample,
+                     "max_rel_residual_err": float(np.max(np.abs(res[rows] - wr) / np.maximum(wr, 1e-300))),
+                     "inertia": float(inertia.item()), "inertia_fsum_rel_err":
+                         abs(float(inertia.item()) - math.fsum(res.tolist())) / max(math.fsum(res.tolist()), 1e-300)}
+    if not args.no_cpu:
+        nr = 256
+        t0 = time.perf_counter()
+        oracle.kmeans_assign(X[:nr], C)
+        tc = time.perf_counter() - t0
+        line["cpu_baseline"] = {"value": round(nr / tc, 1), "unit": "rows/s", "cores": oracle.default_threads(),
+                                "kind": "port", "sample": f"{nr} rows x {kc} centroids (C exact nearest-centre "
+                                                          "restatement, all cores; update excluded)"}
+    return line
+
+
+def run(args):
+    return {"c1": run_c1, "c3": run_c3, "c4": run_c4, "c5": run_c5}[args.workload](args)

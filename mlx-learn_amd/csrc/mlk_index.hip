@@ -16,6 +16,7 @@
 
 #include "mlk_device.h"
 #include "mlk_internal.h"
+#include "mlk_screen.h"
 
 namespace mlk {
 
@@ -182,14 +183,14 @@ int mirror_rows(int prec, const float* X, int64_t n, int64_t n_pad, int d, int d
     return MLK_OK;
 }
 
-int pad32(int d) { return (d + 31) / 32 * 32; }
+
 
 int prepare_mirror(mlk_index* index, int prec, hipStream_t stream) {
     Mirror& mr = index->mirror[prec];
     if (mr.ready) return MLK_OK;
     const int64_t n = index->n;
     const int d = index->d;
-    mr.dp = pad32(d);
+    mr.dp = screen_dp(prec, d);
     double scale = 1.0;
     DevBuf tmp;
     MLK_TRY(tmp.reserve(16));

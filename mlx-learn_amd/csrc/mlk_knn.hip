@@ -691,9 +691,11 @@ struct Plan {
 static Plan make_plan(int prec, int dp, int64_t n, int64_t m, int k) {
     Plan p;
     const Tuning& t = tuning();
-    p.shape = default_shape(prec);
-    if (t.query_tiles) p.shape.qt = t.query_tiles;
-    if (t.stages) p.shape.ns = t.stages;
+    p.shape = default_shape(prec, dp);
+    if (dp <= 128) {  // wide rows have one shape: its query block size must match
+        if (t.query_tiles) p.shape.qt = t.query_tiles;
+        if (t.stages) p.shape.ns = t.stages;
+    }
     const int64_t BQ = 4 * p.shape.qt * 32;
     const int64_t qb = (m + BQ - 1) / BQ;
     if (t.splits) {
@@ -734,9 +736,9 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
     if (m == 0) return MLK_OK;
     const int64_t n = index->n;
     const int d = index->d;
-    const int prec = (int)((flags & MLK_PREC_MASK) >> 4);
+    const int prec = effective_prec((int)((flags & MLK_PREC_MASK) >> 4), d);
     const bool exclude = (flags & MLK_EXCLUDE_SELF) != 0;
-    st.screen_precision = flags & MLK_PREC_MASK;
+    st.screen_precision = (uint32_t)prec << 4;  // the precision actually screened
 
     hipEvent_t ev0, ev1, ev2;
     MLK_CHECK_HIP(hipEventCreate(&ev0));

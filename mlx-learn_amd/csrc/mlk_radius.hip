@@ -242,12 +242,12 @@ int radius_query_impl(mlk_index* index, const float* Q, int64_t m, double R2, in
         return MLK_OK;
     }
 
-    const int prec = (int)((flags & MLK_PREC_MASK) >> 4);
-    st.screen_precision = flags & MLK_PREC_MASK;
+    const int prec = effective_prec((int)((flags & MLK_PREC_MASK) >> 4), d);
+    st.screen_precision = (uint32_t)prec << 4;  // the precision actually screened
     MLK_TRY(prepare_mirror(index, prec, stream));
     Mirror& mr = index->mirror[prec];
     const int dp = mr.dp;
-    const ScreenShape shape = default_shape(prec);
+    const ScreenShape shape = default_shape(prec, dp);
     ScreenFn screen = pick_screen(SCREEN_PAIRS, prec, dp, shape);
     const int qt = shape.qt;
     const int BQ = 4 * qt * 32;
@@ -320,7 +320,7 @@ int radius_query_impl(mlk_index* index, const float* Q, int64_t m, double R2, in
             hipLaunchKernelGGL(screen, dim3((unsigned)(QB * S)), dim3(256), 0, stream, sa);
             MLK_CHECK_HIP(hipGetLastError());
         } else {
-            set_error("radius search supports at most 128 features");
+            set_error("radius search supports at most 800 features");
             return MLK_EINVAL;
         }
         MLK_CHECK_HIP(hipEventRecord(e1, stream));

@@ -110,8 +110,16 @@ struct ScreenShape {
     int qt, ns;
 };
 
-// Measured default shape of each precision.
-ScreenShape default_shape(int prec);
+// Measured default shape for a precision and padded width.
+ScreenShape default_shape(int prec, int dp);
+
+// Padded width of the screen mirror: a multiple of 32 up to 128, then the
+// wide-kernel widths 256, 384, 512, 640, 800 (bf16 / fp16 only).
+int screen_dp(int prec, int d);
+
+// Screen precision actually used: float32 has no wide kernel, so rows wider
+// than 128 features are screened in bf16 (results do not depend on it).
+inline int effective_prec(int prec, int d) { return (prec == PREC_FP32 && d > 128 && d <= 800) ? PREC_BF16 : prec; }
 
 // Launch function of the screen for a padded width dp in {32, 64, 96, 128};
 // nullptr when the shape is not built for that mode and precision. Blocks

@@ -90,15 +90,14 @@ __device__ __forceinline__ void tile_barrier() { asm volatile("s_barrier" ::: "m
 // Per tile a wave issues its MFMAs (accumulators start at the rows' biases
 // -a_x), then one branch-free max-tree test of every accumulator against its
 // query's threshold; only a hit enters the append path.
-template <int PREC, int DP, int QT, int NS, int MODE>
+template <int PREC, int DP, int QT, int RTW, int NS, int MODE>
 __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
     // the body uses device-only types (buffer resources): compile it in the
     // device pass only, so the host pass still emits the kernel's launch stub
 #if defined(__HIP_DEVICE_COMPILE__)
     typedef typename PT<PREC>::T T;
     constexpr int NW = 4;                     // waves per block
-    constexpr int RTW = QT == 1 ? 2 : 1;      // 32-row MFMA tiles per step
-    constexpr int BN = 32 * RTW;              // rows per tile
+    constexpr int BN = 32 * RTW;              // rows per tile (RTW 32-row MFMA tiles per step)
     constexpr int EB = sizeof(T);
     constexpr int RB = DP * EB;               // bytes per database row
     constexpr int CPR = RB / 16;              // 16-byte chunks per row
@@ -110,7 +109,9 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
     constexpr int BQ = NW * QT * 32;
     static_assert(NS >= 2 && NS <= 4, "stages");
     static_assert(QT == 1 || QT == 2, "query tiles");
-    static_assert(BN <= 64, "bias staging holds 64 rows");
+    static_assert(RTW == 1 || RTW == 2, "row tiles per step");
+    static_assert(NS * (ABYTES + 512) + (MODE == SCREEN_PAIRS ? NW * PAIR_STAGE * 8 : 0) <= 160 * 1024,
+                  "LDS of one block");
     // distinct buffers (not one array) so the compiler can tell the tile
     // being read from the tiles being loaded and keeps those loads in flight
     __shared__ __attribute__((aligned(1024))) char sA0[ABYTES];
@@ -199,12 +200,17 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
     // offsets within a tile are lane constants; the tile base goes into the
     // buffer resource (SGPRs). Waves 1 and 2 also load the rows' biases and
     // norms.
-    uint32_t voff[LPW];
-#pragma unroll
-    for (int i = 0; i < LPW; ++i) {
+    auto chunk_off = [&](int i) -> uint32_t {
         const int L = (wave + i * NW) * 64 + lane;
         const int row = L / (CPR + 1), pos = L % (CPR + 1);
-        voff[i] = (pos < CPR && row < BN) ? (uint32_t)(row * RB + pos * 16) : 0x7ffffff0u;
+        return (pos < CPR && row < BN) ? (uint32_t)(row * RB + pos * 16) : 0x7ffffff0u;
+    };
+    // narrow rows: the few offsets stay in registers; wide rows recompute them
+    constexpr bool KEEP_OFF = LPW <= 4;
+    uint32_t voff[KEEP_OFF ? LPW : 1];
+    if constexpr (KEEP_OFF) {
+#pragma unroll
+        for (int i = 0; i < LPW; ++i) voff[i] = chunk_off(i);
     }
     // tile loads this wave issues (uniform): its share of the NLOAD chunks, plus one
     const int my_loads = (NLOAD > wave ? (NLOAD - wave + NW - 1) / NW : 0) + (wave == 1 || wave == 2);
@@ -226,8 +232,11 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
 #pragma unroll
         for (int i = 0; i < LPW; ++i) {
             const int g = wave + i * NW;
-            if (g < NLOAD)
-                __builtin_amdgcn_raw_ptr_buffer_load_lds(ra, (lds_char*)(dstA + g * 1024), 16, voff[i], 0, 0, 0);
+            if (g < NLOAD) {
+                uint32_t off;
+                if constexpr (KEEP_OFF) off = voff[i]; else off = chunk_off(i);
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(ra, (lds_char*)(dstA + g * 1024), 16, off, 0, 0, 0);
+            }
         }
         if (wave == 1 || wave == 2) {
             const float* src = wave == 1 ? a.bias + r0 : a.nx32 + r0;
@@ -245,6 +254,9 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
             MLK_WAIT_CASE(6) MLK_WAIT_CASE(7) MLK_WAIT_CASE(8) MLK_WAIT_CASE(9) MLK_WAIT_CASE(10)
             MLK_WAIT_CASE(11) MLK_WAIT_CASE(12) MLK_WAIT_CASE(13) MLK_WAIT_CASE(14) MLK_WAIT_CASE(15)
             MLK_WAIT_CASE(16) MLK_WAIT_CASE(17) MLK_WAIT_CASE(18) MLK_WAIT_CASE(19) MLK_WAIT_CASE(20)
+            MLK_WAIT_CASE(21) MLK_WAIT_CASE(22) MLK_WAIT_CASE(23) MLK_WAIT_CASE(24) MLK_WAIT_CASE(25)
+            MLK_WAIT_CASE(26) MLK_WAIT_CASE(27) MLK_WAIT_CASE(28) MLK_WAIT_CASE(29) MLK_WAIT_CASE(30)
+            MLK_WAIT_CASE(31) MLK_WAIT_CASE(32)
 #undef MLK_WAIT_CASE
             default: wait_vm_lgkm0<0>(); break;
         }
@@ -389,17 +401,38 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
 
 template <int PREC, int QT, int NS, int MODE>
 static ScreenFn pick_dp(int dp) {
+    constexpr int RTW = QT == 1 ? 2 : 1;
     switch (dp) {
-        case 32: return k_screen<PREC, 32, QT, NS, MODE>;
-        case 64: return k_screen<PREC, 64, QT, NS, MODE>;
-        case 96: return k_screen<PREC, 96, QT, NS, MODE>;
-        case 128: return k_screen<PREC, 128, QT, NS, MODE>;
+        case 32: return k_screen<PREC, 32, QT, RTW, NS, MODE>;
+        case 64: return k_screen<PREC, 64, QT, RTW, NS, MODE>;
+        case 96: return k_screen<PREC, 96, QT, RTW, NS, MODE>;
+        case 128: return k_screen<PREC, 128, QT, RTW, NS, MODE>;
+        default: return nullptr;
+    }
+}
+
+// Wide rows (bf16 / fp16, padded width in {256, 384, 512, 640, 800}): one
+// query tile per wave (its fragments for the whole width stay in registers),
+// 32-row tiles; three LDS stages for k-NN lists, two beside the pair staging.
+template <int PREC, int MODE>
+static ScreenFn pick_wide(int dp) {
+    constexpr int NS = MODE == SCREEN_PAIRS ? 2 : 3;
+    switch (dp) {
+        case 256: return k_screen<PREC, 256, 1, 1, NS, MODE>;
+        case 384: return k_screen<PREC, 384, 1, 1, NS, MODE>;
+        case 512: return k_screen<PREC, 512, 1, 1, NS, MODE>;
+        case 640: return k_screen<PREC, 640, 1, 1, NS, MODE>;
+        case 800: return k_screen<PREC, 800, 1, 1, NS, MODE>;
         default: return nullptr;
     }
 }
 
 template <int PREC, int MODE>
 static ScreenFn pick_shape(int dp, int qt, int ns) {
+    if (dp > 128) {
+        if constexpr (PREC == PREC_FP32) return nullptr;
+        else return pick_wide<PREC, MODE>(dp);
+    }
     if constexpr (PREC == PREC_FP32) {
         // float32 operands: two query tiles would need 128 registers of fragments
         if (qt == 1 && ns == 2) return pick_dp<PREC, 1, 2, MODE>(dp);
@@ -417,11 +450,19 @@ static ScreenFn pick_shape(int dp, int qt, int ns) {
     }
 }
 
+int screen_dp(int prec, int d) {
+    const int p32 = (d + 31) / 32 * 32;
+    if (p32 <= 128 || prec == PREC_FP32) return p32;
+    for (int w : {256, 384, 512, 640, 800})
+        if (d <= w) return w;
+    return p32;  // wider than any screen: exact fallback
+}
+
 // Measured on the 1M x 128 mixture workload: with the candidate appends of a
 // real query, one query tile per wave (occupancy 4) beats two (occupancy 3).
-ScreenShape default_shape(int prec) {
+ScreenShape default_shape(int prec, int dp) {
     (void)prec;
-    return ScreenShape{1, 2};
+    return dp > 128 ? ScreenShape{1, 3} : ScreenShape{1, 2};
 }
 
 ScreenFn pick_screen(int mode, int prec, int dp, ScreenShape shape) {

@@ -266,3 +266,21 @@ def test_estimator_kneighbors(mlk, rng):
     assert_parity((d2, i2), oracle.knn(X.astype(np.float32), X.astype(np.float32), 4, exclude_self=True))
     g = nn.kneighbors_graph(q, mode="distance")
     assert g.shape == (25, 3000)
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize("d", [129, 200, 384, 500, 640, 784, 800])
+def test_wide_screen_matches_oracle(mlk, precision, d):
+    """Rows wider than 128 features: the wide MFMA screen (float32 requests are screened in bf16)."""
+    r = np.random.default_rng(7000 + d)
+    X = r.normal(size=(3000, d)).astype(np.float32)
+    X[50:55] = X[49]
+    Q = np.concatenate([X[:3], r.normal(size=(157, d)).astype(np.float32)])
+    assert_parity(gpu_knn(mlk, X, Q, 7, precision, screened=True), oracle.knn(X, Q, 7))
+
+
+def test_wide_pixels_exact_screen(mlk):
+    """Workload c3 generator (784 integer pixels) through the wide fp16 screen, no fallback."""
+    X, y, Q = oracle.gen_c3(seed=13, n=8000, m=400)
+    for p in PRECISIONS:
+        assert_parity(gpu_knn(mlk, X, Q, 3, p), oracle.knn(X, Q, 3))

@@ -135,3 +135,13 @@ def test_estimator_radius_neighbors(mlk, rng):
     for r in (0, 7, 1999):
         np.testing.assert_array_equal(i_self[r], wi[wp[r]:wp[r + 1]])
     assert math.isclose(float(np.mean([len(v) for v in i_self])), (wp[-1]) / 2000)
+
+
+@pytest.mark.parametrize("d", [200, 784])
+def test_wide_rows(mlk, d):
+    r = np.random.default_rng(8100 + d)
+    X = r.normal(size=(2500, d)).astype(np.float32)
+    Q = np.concatenate([X[:5], r.normal(size=(60, d)).astype(np.float32)])
+    R2 = float(radius_cases.typical_radius(X, Q, 12)) ** 2
+    for p in PRECISIONS:
+        assert_csr(gpu_radius(mlk, X, Q, R2, p, sort_results=True), oracle.radius(X, Q, R2, sort_results=True))
