@@ -79,7 +79,7 @@ __device__ __forceinline__ double fast_sq(const float* __restrict__ q, const flo
 // lanes share a sample row (coalesced 64-byte segments), four rows per step;
 // the float64 partial sums may be added in any order (bound g).
 __global__ void k_tau0(const float* __restrict__ X, int64_t n, int d, const float* __restrict__ Q, int64_t m,
-                       int64_t m_pad, int k, int64_t self_off, int exclude, const double* __restrict__ nq,
+                       int64_t m_pad, int k, int max_samples, int64_t self_off, int exclude, const double* __restrict__ nq,
                        const uint8_t* __restrict__ qflags, Consts c, float* __restrict__ tau,
                        float* __restrict__ thr, int* __restrict__ status) {
     __shared__ uint32_t sb[4][TAU0_SAMPLES];
@@ -90,14 +90,15 @@ __global__ void k_tau0(const float* __restrict__ X, int64_t n, int d, const floa
         if (lane == 0) { tau[q] = -1.0f; thr[q] = INFINITY; status[q] = 0; }
         return;
     }
-    const int samples = n < TAU0_SAMPLES ? (int)n : TAU0_SAMPLES;
+    const int samples = n < max_samples ? (int)n : max_samples;
+    const int span = (samples + 15) / 16 * 16;  // entries past `samples` hold +inf
     const double g = f64_interval(d);
     const float* qv = Q + q * d;
     const int gl = lane & 15;
     const int64_t stride = n / samples;  // sample j is row j * stride
     // four sample rows per 16-lane group and pass; every load of a
     // 128-feature slice is issued before the arithmetic
-    for (int j0 = 0; j0 < TAU0_SAMPLES; j0 += 16) {
+    for (int j0 = 0; j0 < span; j0 += 16) {
         int64_t row[4];
         const float* xp[4];
 #pragma unroll
@@ -141,7 +142,7 @@ __global__ void k_tau0(const float* __restrict__ X, int64_t n, int d, const floa
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     const uint32_t* mine = sb[w];
-    const uint32_t kb = wave_kth_bits(TAU0_SAMPLES, k, [&](int j) { return mine[j]; });
+    const uint32_t kb = wave_kth_bits(span, k, [&](int j) { return mine[j]; });
     const float t0 = __uint_as_float(kb);
     if (lane == 0) {
         const int st = qflags[q] ? 0 : ST_UNSAFE;
@@ -151,11 +152,12 @@ __global__ void k_tau0(const float* __restrict__ X, int64_t n, int d, const floa
     }
 }
 
-// Sums of the per-query rank statistics (three counts per query) in a fixed order.
+// Sums of the per-query rank statistics (three counts per query): block
+// partials added into out (zeroed by the caller); integer sums, any order.
 __global__ void k_sum_qstats(const int* __restrict__ qs, int64_t m, unsigned long long* __restrict__ out) {
     __shared__ unsigned long long sh[3][256];
     unsigned long long a0 = 0, a1 = 0, a2 = 0;
-    for (int64_t q = threadIdx.x; q < m; q += blockDim.x) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < m; q += (int64_t)gridDim.x * blockDim.x) {
         a0 += (unsigned)qs[3 * q];
         a1 += (unsigned)qs[3 * q + 1];
         a2 += (unsigned)qs[3 * q + 2];
@@ -167,7 +169,7 @@ __global__ void k_sum_qstats(const int* __restrict__ qs, int64_t m, unsigned lon
             for (int c = 0; c < 3; ++c) sh[c][threadIdx.x] += sh[c][threadIdx.x + o];
         __syncthreads();
     }
-    if (threadIdx.x < 3) out[threadIdx.x] = sh[threadIdx.x][0];
+    if (threadIdx.x < 3 && sh[threadIdx.x][0]) atomicAdd(&out[threadIdx.x], sh[threadIdx.x][0]);
 }
 
 // ---------------------------------------------------------------- merge
@@ -261,6 +263,66 @@ __global__ void k_merge(int64_t m, int k, const int* __restrict__ cnt, int* __re
         tau[q] = t_new;
         thr[q] = thr_from_tau((double)t_new, nq[q], c);
     }
+}
+
+// Merge with one lane per query, for few sub-lists per query and k <= KM
+// (k-means assignment, small k): the k smallest upper bounds stay sorted in
+// registers and every new entry goes through a compare-exchange chain. Same
+// result as k_merge (kbest = the k smallest bounds of the union, tau = min of
+// the old tau and the k-th).
+template <int KM>
+__global__ void k_merge_lane(int64_t m, int k, const int* __restrict__ cnt, int* __restrict__ seen,
+                             const int2* __restrict__ cand, const float* __restrict__ nx32, int S2, int cap,
+                             float* __restrict__ kbest, const double* __restrict__ nq, Consts c,
+                             float* __restrict__ tau, float* __restrict__ thr, int* __restrict__ status) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= m || status[q]) return;
+    float best[KM];
+#pragma unroll
+    for (int i = 0; i < KM; ++i) best[i] = INFINITY;
+    auto insert = [&](float v) {
+#pragma unroll
+        for (int i = 0; i < KM; ++i) {
+            const float lo = fminf(v, best[i]);
+            v = fmaxf(v, best[i]);
+            best[i] = lo;
+        }
+    };
+    for (int i = 0; i < k; ++i) insert(kbest[q * k + i]);
+    const double nqq = nq[q];
+    bool over = false;
+    int fresh = 0;
+    for (int s = 0; s < S2; ++s) {
+        const int cs = cnt[q * S2 + s];
+        over |= cs > cap;
+        const int hi = cs < cap ? cs : cap;
+        const int lo = seen[q * S2 + s];
+        seen[q * S2 + s] = hi;
+        const size_t off = ((size_t)q * S2 + s) * cap;
+        for (int j = lo; j < hi; ++j) {
+            const int2 e = cand[off + j];
+            double L, U;
+            hit_bounds(__int_as_float(e.y), nx32[e.x], nqq, c, L, U);
+            insert(fmaxf(__double2float_ru(U), 0.0f));
+            ++fresh;
+        }
+    }
+    if (over) {
+        status[q] = ST_OVERFLOW;
+        thr[q] = INFINITY;
+        return;
+    }
+    if (fresh == 0) return;
+#pragma unroll
+    for (int i = 0; i < KM; ++i)
+        if (i < k) kbest[q * k + i] = best[i];
+    float kth = best[0];
+#pragma unroll
+    for (int i = 0; i < KM; ++i)
+        if (i == k - 1) kth = best[i];
+    const float t_new = fminf(tau[q], kth);
+    tau[q] = t_new;
+    thr[q] = thr_from_tau((double)t_new, nqq, c);
 }
 
 // ---------------------------------------------------------------- rank
@@ -711,8 +773,9 @@ static Plan make_plan(int prec, int dp, int64_t n, int64_t m, int k) {
     }
     // slots per (query, split, lane half) sub-list
     // (at least 6k slots per query over its 2 * splits sub-lists)
-    const int base = prec == PREC_FP32 ? 24 : 48;
     const int lists = 2 * p.splits;
+    // few lists per query (small databases, k-means centroids): more slots each
+    const int base = (prec == PREC_FP32 ? 24 : 48) * (lists <= 16 ? 2 : 1);
     const int need = (6 * k + lists - 1) / lists + 4;
     p.cap = t.capacity ? t.capacity : std::max(base, need);
     if (!t.rounds.empty()) {
@@ -792,8 +855,11 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
     float* tau = ws[3].as<float>();
     float* thr = ws[4].as<float>();
     int* status = ws[5].as<int>();
+    // sampled start threshold: 32 rows per neighbour asked for, 64 to 256 rows
+    const int tau_samples = std::min(TAU0_SAMPLES, std::max(64, 32 * k));
     hipLaunchKernelGGL(k_tau0, dim3((unsigned)((m_pad + 3) / 4)), dim3(256), 0, stream, X, n, d, Q, m, m_pad, k,
-                       q_offset, (int)exclude, ws[1].as<double>(), ws[2].as<uint8_t>(), c, tau, thr, status);
+                       tau_samples, q_offset, (int)exclude, ws[1].as<double>(), ws[2].as<uint8_t>(), c, tau, thr,
+                       status);
     MLK_CHECK_HIP(hipGetLastError());
 
     if ((hooks().hooks & MLK_HOOK_FORCE_UNCERTIFIED) && !hooks().forced_rows.empty()) {
@@ -835,6 +901,21 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
         sa.exclude = exclude;
         sa.xcd_map = (S % 8) == 0;
         const unsigned blocks = (unsigned)(QB * S);
+        // one wave per query in general; one lane per query when a query has
+        // few sub-lists and k is small
+        const bool lane_merge = S2 <= 16 && k <= 8;
+        auto merge = [&]() -> int {
+            if (lane_merge)
+                hipLaunchKernelGGL(k_merge_lane<8>, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, stream, m, k,
+                                   sa.cnt, ws[17].as<int>(), sa.cand, sa.nx32, S2, cap, ws[18].as<float>(),
+                                   ws[1].as<double>(), c, tau, thr, status);
+            else
+                hipLaunchKernelGGL(k_merge, dim3((unsigned)((m + 3) / 4)), dim3(256), 0, stream, m, k, sa.cnt,
+                                   ws[17].as<int>(), sa.cand, sa.nx32, S2, cap, ws[18].as<float>(),
+                                   ws[1].as<double>(), c, tau, thr, status);
+            MLK_CHECK_HIP(hipGetLastError());
+            return MLK_OK;
+        };
         double prev = 0.0;
         std::vector<hipEvent_t> round_ev;
         for (size_t r = 0; r < plan.rounds.size(); ++r) {
@@ -853,19 +934,11 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
             MLK_CHECK_HIP(hipGetLastError());
             MLK_CHECK_HIP(hipEventRecord(eb, stream));
             ++st.rounds;
-            if (r + 1 < plan.rounds.size()) {
-                hipLaunchKernelGGL(k_merge, dim3((unsigned)((m + 3) / 4)), dim3(256), 0, stream, m, k, sa.cnt,
-                                   ws[17].as<int>(), sa.cand, sa.nx32, S2, cap, ws[18].as<float>(), ws[1].as<double>(), c, tau, thr,
-                                   status);
-                MLK_CHECK_HIP(hipGetLastError());
-            }
+            if (r + 1 < plan.rounds.size()) MLK_TRY(merge());
         }
         round_events = std::move(round_ev);
         // final overflow check (the last round has no merge)
-        hipLaunchKernelGGL(k_merge, dim3((unsigned)((m + 3) / 4)), dim3(256), 0, stream, m, k, sa.cnt,
-                           ws[17].as<int>(), sa.cand, sa.nx32, S2, cap, ws[18].as<float>(), ws[1].as<double>(), c, tau, thr,
-                                   status);
-        MLK_CHECK_HIP(hipGetLastError());
+        MLK_TRY(merge());
         MLK_CHECK_HIP(hipEventRecord(ev2, stream));
 
         RankArgs ra{};
@@ -881,8 +954,9 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
         MLK_CHECK_HIP(hipMemsetAsync(ra.qstats, 0, (size_t)m * 3 * sizeof(int), stream));
         hipLaunchKernelGGL(k_rank, dim3((unsigned)((m + 3) / 4)), dim3(256), 0, stream, ra, m);
         MLK_CHECK_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_sum_qstats, dim3(1), dim3(256), 0, stream, ra.qstats, m,
-                           (unsigned long long*)(dflag + 4));
+        MLK_CHECK_HIP(hipMemsetAsync(dflag + 4, 0, 3 * sizeof(unsigned long long), stream));
+        hipLaunchKernelGGL(k_sum_qstats, dim3((unsigned)std::min<int64_t>(1024, (m + 255) / 256)), dim3(256), 0,
+                           stream, ra.qstats, m, (unsigned long long*)(dflag + 4));
         MLK_CHECK_HIP(hipGetLastError());
     } else {
         MLK_CHECK_HIP(hipEventRecord(ev2, stream));

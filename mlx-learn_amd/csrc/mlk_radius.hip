@@ -120,9 +120,19 @@ __global__ void k_classify(const unsigned long long* __restrict__ keys, int64_t 
     dist[i] = sqrt(s < rc.R2 ? s : rc.R2);
 }
 
-__global__ void k_row_counts(const unsigned long long* __restrict__ keys, int64_t nnz, unsigned long long* __restrict__ counts) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nnz) atomicAdd(&counts[keys[i] >> 32], 1ull);
+// CSR row pointers from the sorted member keys: indptr[q] is the first key
+// whose query part is >= q (binary search; indptr[m] = nnz).
+__global__ void k_indptr_search(const unsigned long long* __restrict__ keys, int64_t nnz, int64_t m,
+                                int64_t* __restrict__ indptr) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q > m) return;
+    const unsigned long long target = (unsigned long long)q << 32;
+    int64_t lo = 0, hi = nnz;
+    while (lo < hi) {
+        const int64_t mid = lo + ((hi - lo) >> 1);
+        if (keys[mid] < target) lo = mid + 1; else hi = mid;
+    }
+    indptr[q] = lo;
 }
 
 __global__ void k_key_to_index(const unsigned long long* __restrict__ keys, int64_t nnz, int64_t* __restrict__ idx) {
@@ -390,20 +400,9 @@ int radius_query_impl(mlk_index* index, const float* Q, int64_t m, double R2, in
     MLK_CHECK_HIP(hipStreamSynchronize(stream));
 
     // CSR row pointers
-    DevBuf counts;
-    MLK_TRY(counts.reserve((size_t)(m + 1) * sizeof(unsigned long long)));
-    MLK_CHECK_HIP(hipMemsetAsync(counts.ptr, 0, (size_t)(m + 1) * sizeof(unsigned long long), stream));
-    if (nnz)
-        hipLaunchKernelGGL(k_row_counts, dim3(blocks_for(nnz)), dim3(256), 0, stream, mkeys.as<unsigned long long>(), nnz,
-                           counts.as<unsigned long long>());
-    {
-        size_t tmp_bytes = 0;
-        unsigned long long* cptr = counts.as<unsigned long long>();
-        unsigned long long* optr = (unsigned long long*)indptr;
-        MLK_CHECK_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, cptr, optr, (int)(m + 1), stream));
-        MLK_TRY(ws[14].reserve(tmp_bytes + 16));
-        MLK_CHECK_HIP(hipcub::DeviceScan::ExclusiveSum(ws[14].ptr, tmp_bytes, cptr, optr, (int)(m + 1), stream));
-    }
+    hipLaunchKernelGGL(k_indptr_search, dim3(blocks_for(m + 1)), dim3(256), 0, stream, mkeys.as<unsigned long long>(),
+                       nnz, m, indptr);
+    MLK_CHECK_HIP(hipGetLastError());
     MLK_TRY(res->indices.reserve((size_t)std::max<int64_t>(nnz, 1) * sizeof(int64_t)));
     if (nnz)
         hipLaunchKernelGGL(k_key_to_index, dim3(blocks_for(nnz)), dim3(256), 0, stream, mkeys.as<unsigned long long>(), nnz,

@@ -350,22 +350,32 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
                             }
                         }
                     } else {
-                        // rolled loop: the register index i is uniform (v_movrels)
+                        // registers with a hit in some lane, visited in order;
+                        // each one's hits are compacted into the wave's LDS
+                        // staging with one ballot (no per-lane atomics)
+                        uint32_t regs = 0;
+#pragma unroll
+                        for (int i = 0; i < 16; ++i) regs |= (ballot(acc[rt][qt][i] >= thr[qt]) != 0 ? 1u : 0u) << i;
 #pragma unroll 1
-                        for (int i = 0; i < 16; ++i) {
+                        while (regs) {
+                            const int i = __builtin_ctz(regs);
+                            regs &= regs - 1;
                             const float v = acc[rt][qt][i];
                             const int rl = rt * 32 + mfma32_row(i, lane);
                             const int64_t row = r0 + rl;
                             const bool hit = v >= thr[qt] && row < r_end && row != self;
-                            if (ballot(hit)) {
-                                if (sPairN[wave] + 64 > PAIR_STAGE) flush_pairs();
-                                if (hit) {
-                                    const int slot = atomicAdd(&sPairN[wave], 1);
-                                    sPair[wave][slot] = int2{(int)qg, (int)row};
-                                }
-                                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                                __builtin_amdgcn_wave_barrier();
-                            }
+                            const unsigned long long mask = ballot(hit);
+                            if (!mask) continue;
+                            if (sPairN[wave] + 64 > PAIR_STAGE) flush_pairs();
+                            const int base = sPairN[wave];
+                            const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                                                        __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0));
+                            if (hit) sPair[wave][base + below] = int2{(int)qg, (int)row};
+                            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                            __builtin_amdgcn_wave_barrier();
+                            if (lane == 0) sPairN[wave] = base + __popcll(mask);
+                            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                            __builtin_amdgcn_wave_barrier();
                         }
                     }
                 }
