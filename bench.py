@@ -15,6 +15,11 @@ Also reported:
                 from the rocprofv3 FETCH_SIZE/WRITE_SIZE summary in profiles/.
   cpu_baseline  numpy float64 restatement of the reference CPU path
                 (CPUIndex._gram) on a bounded query sample, host cores.
+  autotune      before timing, each precision's plan is autotuned on the
+                queries (mlk_autotune; --no-autotune keeps the heuristic plan);
+                the heuristic plan's time is reported beside the tuned one.
+  step_ms       median and interquartile range of the timed steps, and the
+                first (cold) call, which builds the screen mirrors.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2]
 """
@@ -102,6 +107,8 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5"],
                     help="c2 (default) is the headline; c1, c3, c4, c5 are the other configurations")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="time the heuristic plans instead of autotuned ones")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -114,8 +121,8 @@ def main():
               "The database would shard by contiguous row ranges with an exact (distance, index) merge.",
               file=sys.stderr)
         sys.exit(2)
+    import bench_workloads
     if args.workload != "c2":
-        import bench_workloads
         line = bench_workloads.run(args)
         print(json.dumps(line), flush=True)
         return
@@ -141,20 +148,25 @@ def main():
         dist, idx = index.query(Qd, k, precision=precision)
         return dist, idx, _lib.stats()
 
-    results = {}
+    results, tuned = {}, {}
     for precision in ("bfloat16", "float32"):
-        for _ in range(args.warmup):
-            run(precision)
-        torch.cuda.synchronize()
-        screen_ms, stats = [], None
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            dist, idx, stats = run(precision)
-            screen_ms.append(stats["screen_ms"])
-        torch.cuda.synchronize()
-        wall = (time.perf_counter() - t0) / args.steps
+        first = bench_workloads.cold(lambda: run(precision))
+        if args.autotune:
+            # plan search on these queries: untimed preparation, like fit
+            tuned[precision] = bench_workloads.tune(index, Qd, k, precision=precision, max_trials=16)
+        screen_ms = []
+
+        def step():
+            out = run(precision)
+            screen_ms.append(out[2]["screen_ms"])
+            return out
+
+        wall, (dist, idx, stats), spread = bench_workloads.timed(step, args.steps, args.warmup)
+        screen_ms = screen_ms[-args.steps:]
+        spread["first_call_ms"] = first
         results[precision] = {"ms_per_step": wall * 1e3, "qps": m / wall, "idx": idx.cpu().numpy(),
-                              "dist": dist.cpu().numpy(), "screen_ms": float(np.mean(screen_ms)), "stats": stats}
+                              "dist": dist.cpu().numpy(), "screen_ms": float(np.mean(screen_ms)), "stats": stats,
+                              "spread": spread}
         log(f"{precision}: {wall * 1e3:.2f} ms/step, {m / wall:,.0f} q/s, screen {np.mean(screen_ms):.2f} ms, "
             f"stats {stats}")
 
@@ -199,6 +211,8 @@ def main():
                         "oracle_sample": sample, "max_rel_dist_err": float(rel.max())},
         "screen_stats": {key: bf["stats"][key] for key in ("candidates", "reranked", "fallback_queries", "rounds",
                                                            "splits", "capacity")},
+        "step_ms": {"bf16": bf["spread"], "fp32": fp["spread"]},
+        "autotune": {"bf16": tuned.get("bfloat16"), "fp32": tuned.get("float32")} if tuned else None,
     }
     index.close()
     del Xd, Qd

@@ -37,17 +37,45 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def cold(fn):
+    """Milliseconds of a first call (it builds the screen mirrors and workspace)."""
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fn()
+    torch.cuda.synchronize()
+    return round((time.perf_counter() - t0) * 1e3, 3)
+
+
 def timed(fn, steps, warmup):
+    """Mean seconds per step over `steps` timed steps (bracketed by device
+    synchronisation), the last output, and the spread of the per-step times
+    (median and interquartile range)."""
     import torch
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
+    per = []
     out = None
+    t0 = time.perf_counter()
     for _ in range(steps):
+        ts = time.perf_counter()
         out = fn()
-    torch.cuda.synchronize()
-    return (time.perf_counter() - t0) / steps, out
+        torch.cuda.synchronize()
+        per.append(time.perf_counter() - ts)
+    total = time.perf_counter() - t0
+    q1, med, q3 = np.percentile(np.array(per) * 1e3, [25, 50, 75])
+    spread = {"median_ms": round(float(med), 3), "iqr_ms": round(float(q3 - q1), 3), "runs": steps}
+    return total / steps, out, spread
+
+
+def tune(index, Q, k=None, **kw):
+    """Autotune the plan for these queries (untimed preparation); a summary for the JSON line."""
+    plan = index.autotune(Q, k, **kw)
+    log(f"autotune {kw}: {plan}")
+    return {"tuned_ms": round(plan["tuned_ms"], 3), "default_ms": round(plan["default_ms"], 3),
+            "trials": plan["trials"], "plan": {key: plan[key] for key in ("splits", "capacity", "query_tiles",
+                                                                          "stages", "rounds")}}
 
 
 def base_line(args, metric, value, unit, ms, dtype, data, config):
@@ -78,9 +106,13 @@ def run_c1(args):
     X, Q = oracle.gen_c1(seed=args.seed)
     index = mlxlearn_amd.DeviceNeighborIndex(torch.from_numpy(X).cuda())
     Qd = torch.from_numpy(Q).cuda()
-    res = {}
+    res, spread, tuned = {}, {}, {}
     for prec in ("float32", "bfloat16"):
-        t, (d, i, st) = timed(knn_step(index, Qd, 5, prec), args.steps, args.warmup)
+        first = cold(knn_step(index, Qd, 5, prec))
+        if args.autotune:
+            tuned[prec] = tune(index, Qd, 5, precision=prec, max_trials=16)
+        t, (d, i, st), spread[prec] = timed(knn_step(index, Qd, 5, prec), args.steps, args.warmup)
+        spread[prec]["first_call_ms"] = first
         res[prec] = (t, d.cpu().numpy(), i.cpu().numpy(), st)
         log(f"c1 {prec}: {t * 1e3:.3f} ms/step, stats {st}")
     wd, wi = oracle.knn(X, Q, 5)
@@ -93,6 +125,8 @@ def run_c1(args):
                      {"workload": "c1: NearestNeighbors(k=5), database 10,000x64 float32, 1,000 queries, fp32 MFMA "
                                   "screen + fp64 exact rerank", "n": 10000, "d": 64, "queries": 1000, "k": 5})
     line["roofline"] = roofline(flops, st["screen_ms"], "float32", "k_screen<fp32,64>", st["rounds"])
+    line["step_ms"] = spread
+    line["autotune"] = tuned or None
     line["bf16_path"] = {"value": round(Q.shape[0] / res["bfloat16"][0], 1),
                          "ms_per_step": round(res["bfloat16"][0] * 1e3, 3)}
     line["index_match"] = {"fp32_vs_oracle": match, "bf16_vs_oracle": match_bf, "oracle_sample": Q.shape[0],
@@ -140,9 +174,13 @@ def run_c3(args):
             return i, st
         return f
 
-    res = {}
+    res, spread, tuned = {}, {}, {}
     for prec in ("float16", "float32"):
-        t, (i, st) = timed(step(prec), args.steps, args.warmup)
+        first = cold(step(prec))
+        if args.autotune:
+            tuned[prec] = tune(index, Qd, k, precision=prec, max_trials=16)
+        t, (i, st), spread[prec] = timed(step(prec), args.steps, args.warmup)
+        spread[prec]["first_call_ms"] = first
         res[prec] = (t, i.cpu().numpy(), pred.cpu().numpy(), st)
         log(f"c3 {prec}: {t * 1e3:.3f} ms/step, stats {st}")
     sample = min(args.oracle_sample, m)
@@ -158,6 +196,8 @@ def run_c3(args):
     line["roofline"] = roofline(flops, max(st["screen_ms"], 1e-9), "float16", "k_screen<fp16,784>",
                                 st["rounds"]) if st["rounds"] else None
     line["fp32_path"] = {"value": round(m / res["float32"][0], 1), "ms_per_step": round(res["float32"][0] * 1e3, 3)}
+    line["step_ms"] = spread
+    line["autotune"] = tuned or None
     line["index_match"] = {"fp16_vs_oracle": float((i[:sample] == wi).all(axis=1).mean()),
                            "fp16_vs_fp32": float((i == res["float32"][1]).all(axis=1).mean()),
                            "pred_vs_oracle": float((p[:sample] == want_pred).mean()), "oracle_sample": sample}
@@ -190,7 +230,10 @@ def run_c4(args):
         indptr, indices, dist = index.radius(Qd, R2, sort_results=True, precision="float32")
         return indptr, indices, dist, _lib.stats()
 
-    t, (indptr, indices, dist, st) = timed(step, args.steps, args.warmup)
+    first = cold(step)
+    tuned = tune(index, Qd, radius_sq=R2, sort_results=True, max_trials=10) if args.autotune else None
+    t, (indptr, indices, dist, st), spread = timed(step, args.steps, args.warmup)
+    spread["first_call_ms"] = first
     indptr = indptr.cpu().numpy()
     log(f"c4: {t * 1e3:.3f} ms/step, nnz {indptr[-1]}, stats {st}")
     # oracle check: the first query of a spread of batches
@@ -210,6 +253,8 @@ def run_c4(args):
                                   f"{len(sizes)} batches ({m} queries) in one call, fp32 MFMA screen + exact band",
                       "n": X.shape[0], "d": X.shape[1], "queries": m, "batches": len(sizes)})
     line["roofline"] = roofline(flops, st["screen_ms"], "float32", "k_screen<fp32,32,pairs>", 1)
+    line["step_ms"] = spread
+    line["autotune"] = tuned
     line["result"] = {"nnz": int(indptr[-1]), "median_per_query": float(np.median(np.diff(indptr))),
                       "oracle_rows_checked": len(picks), "oracle_rows_equal": ok, "candidates": st["candidates"]}
     if not args.no_cpu:
@@ -241,7 +286,16 @@ def run_c5(args):
         sums, counts, inertia = update_device(Xd, labels, kc, resid)
         return labels, resid, counts, inertia, st
 
-    t, (labels, resid, counts, inertia, st) = timed(step, args.steps, args.warmup)
+    first = cold(step)
+    tuned = None
+    if args.autotune:
+        # a plan tuned on a centres index serves every assignment of this shape
+        import mlxlearn_amd
+        centres = mlxlearn_amd.DeviceNeighborIndex(Cd)
+        tuned = tune(centres, Xd, 1, precision="bfloat16", max_trials=12)
+        centres.close()
+    t, (labels, resid, counts, inertia, st), spread = timed(step, args.steps, args.warmup)
+    spread["first_call_ms"] = first
     log(f"c5: {t * 1e3:.3f} ms/step, stats {st}")
     sample = min(4096, n)
     rows = np.random.default_rng(1).choice(n, sample, replace=False)
@@ -255,6 +309,8 @@ def run_c5(args):
                      {"workload": f"c5: KMeans one exact assignment (bf16 MFMA screen, k=1) + one float64 update, "
                                   f"{n}x96 float32 rows, {kc} centroids", "n": n, "d": X.shape[1], "clusters": kc})
     line["roofline"] = roofline(flops, st["screen_ms"], "bfloat16", "k_screen<bf16,96>", st["rounds"])
+    line["step_ms"] = spread
+    line["autotune"] = tuned
     line["check"] = {"labels_vs_oracle": float((lab[rows] == wl).mean()), "oracle_sample_rows": sample,
                      "max_rel_residual_err": float(np.max(np.abs(res[rows] - wr) / np.maximum(wr, 1e-300))),
                      "inertia": float(inertia.item()), "inertia_fsum_rel_err":

@@ -87,6 +87,39 @@ int mlk_set_tuning(int32_t splits, int32_t capacity, int32_t query_tiles, int32_
                    int32_t n_rounds, const double* round_fractions);
 
 /*
+ * Autotuner. Replaces the tile planner resolve_blocks
+ * (src/mlxlearn_device/_neighbors_plan.py:74-126) and the per-GPU-family
+ * selection tables (src/mlxlearn_device/_neighbors_general.py:80-170).
+ * mlk_autotune times screen plans (shape, database splits, round schedule,
+ * list capacity) on the caller's queries Q_dev (m x d) with the index and
+ * keeps the fastest in a process-wide cache keyed by (k-NN or radius,
+ * screen precision, padded width, k, n and m in half-octave buckets); later
+ * calls of that shape use it, including mlk_kmeans_assign (k = 1 over a
+ * centres index of the same shape). With MLK_TUNE_RADIUS it tunes
+ * mlk_radius_query at R2 (k ignored). max_trials bounds the plans timed
+ * (each twice). Results never depend on the plan. mlk_get_plan reports the
+ * plan a call of that shape would use; mlk_clear_plans empties the cache.
+ */
+#define MLK_TUNE_RADIUS 0x1000u
+#define MLK_MAX_ROUNDS 8
+typedef struct {
+    int32_t splits;       /* database splits per screen round                       */
+    int32_t capacity;     /* candidate slots per (query, split, lane half), k-NN    */
+    int32_t query_tiles;  /* 32-query tiles per screen wave                         */
+    int32_t stages;       /* stages of the screen's LDS tile ring                   */
+    int32_t n_rounds;     /* screen rounds (k-NN; 0 for radius)                     */
+    double rounds[MLK_MAX_ROUNDS]; /* cumulative fractions of each split, last = 1   */
+    int32_t tuned;        /* 1 if the plan came from the autotuner                  */
+    int32_t trials;       /* plans timed by the autotuner                           */
+    double tuned_ms;      /* device time of one call with this plan                 */
+    double default_ms;    /* device time of one call with the untuned plan          */
+} mlk_plan;
+int mlk_autotune(mlk_index* index, const float* Q_dev, int64_t m, int32_t k, double R2,
+                 uint32_t flags, int32_t max_trials, mlk_plan* out, void* stream);
+int mlk_get_plan(const mlk_index* index, int64_t m, int32_t k, uint32_t flags, mlk_plan* out);
+int mlk_clear_plans(void);
+
+/*
  * Exact brute-force index over X (n x d float32, row-major, device memory).
  * The index owns a copy of X plus its centre, float64 row norms and the
  * screen mirrors. Replaces DeviceNeighborIndex / fit_neighbors

@@ -19,6 +19,10 @@ static std::mutex g_config_lock;
 
 void set_error(const std::string& msg) { g_error = msg; }
 Tuning& tuning() { return g_tuning; }
+Tuning tuning_snapshot() {
+    std::lock_guard<std::mutex> g(g_config_lock);
+    return g_tuning;
+}
 TestHooks& hooks() { return g_hooks; }
 mlk_stats& stats() { return g_stats; }
 
@@ -176,6 +180,40 @@ int mlk_radius_query(mlk_index* index, const float* Q_dev, int64_t m, double R2,
         return invalid("exclude_self requires query rows to be training rows");
     std::lock_guard<std::mutex> g(index->lock);
     return radius_query_impl(index, Q_dev, m, R2, q_offset, flags, indptr_dev, nnz, out, (hipStream_t)stream);
+}
+
+int mlk_autotune(mlk_index* index, const float* Q_dev, int64_t m, int32_t k, double R2, uint32_t flags,
+                 int32_t max_trials, mlk_plan* out, void* stream) {
+    MLK_TRY(check_fail_hook());
+    if (!out) return invalid("plan output is NULL");
+    if (!index) return invalid("index is NULL");
+    if (m < 1 || !Q_dev) return invalid("autotuning needs at least one query row on the device");
+    if (max_trials < 0) return invalid("max_trials must be non-negative");
+    if (flags & ~(MLK_PREC_MASK | MLK_TUNE_RADIUS | MLK_SORT_RESULTS | MLK_NEED_DISTANCE))
+        return invalid("unsupported autotune flags");
+    if ((flags & MLK_PREC_MASK) == MLK_PREC_MASK) return invalid("unknown screen precision");
+    if (flags & MLK_TUNE_RADIUS) {
+        if (!(R2 >= 0)) return invalid("R2 must be a non-negative number");
+    } else {
+        if (k < 1) return invalid("Expected n_neighbors > 0. Got " + std::to_string(k));
+        if (k > index->n)
+            return invalid("n_neighbors must be <= " + std::to_string(index->n) + ", got " + std::to_string(k));
+    }
+    std::lock_guard<std::mutex> g(index->lock);
+    return autotune_impl(index, Q_dev, m, k, R2, flags, max_trials ? max_trials : 24, out, (hipStream_t)stream);
+}
+
+int mlk_get_plan(const mlk_index* index, int64_t m, int32_t k, uint32_t flags, mlk_plan* out) {
+    if (!index || !out) return invalid("index and plan output are required");
+    if (m < 0) return invalid("query row count must be non-negative");
+    if (!(flags & MLK_TUNE_RADIUS) && k < 1) return invalid("Expected n_neighbors > 0. Got " + std::to_string(k));
+    if ((flags & MLK_PREC_MASK) == MLK_PREC_MASK) return invalid("unknown screen precision");
+    return plan_info_impl(index, m, k, flags, out);
+}
+
+int mlk_clear_plans(void) {
+    clear_plans_impl();
+    return MLK_OK;
 }
 
 int mlk_radius_fill(mlk_radius_result* r, int64_t* indices_dev, double* dist_dev, void* stream) {

@@ -74,6 +74,7 @@ struct Tuning {
     std::vector<double> rounds;
 };
 Tuning& tuning();
+Tuning tuning_snapshot();  // a copy taken under the configuration lock
 
 struct TestHooks {
     uint32_t hooks = 0;
@@ -113,6 +114,11 @@ int prepare_bias(mlk_index* index, int prec, double H, hipStream_t stream);
 int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q_offset, uint32_t flags,
                    double* dist, int64_t* idx, hipStream_t stream, const int64_t* batch_offsets,
                    int n_batches);
+// implemented in mlk_tune.hip
+int autotune_impl(mlk_index* index, const float* Q, int64_t m, int k, double R2, uint32_t flags, int max_trials,
+                  mlk_plan* out, hipStream_t stream);
+int plan_info_impl(const mlk_index* index, int64_t m, int k, uint32_t flags, mlk_plan* out);
+void clear_plans_impl();
 // implemented in mlk_radius.hip
 int radius_query_impl(mlk_index* index, const float* Q, int64_t m, double R2, int64_t q_offset,
                       uint32_t flags, int64_t* indptr, int64_t* nnz, mlk_radius_result** out,

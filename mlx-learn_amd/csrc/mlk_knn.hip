@@ -744,52 +744,6 @@ Consts make_consts(int prec, int dp, int d, bool exact_inputs, double scale) {
     return c;
 }
 
-struct Plan {
-    ScreenShape shape;
-    int splits, cap;
-    std::vector<double> rounds;  // cumulative fractions of each split, ending at 1
-};
-
-static Plan make_plan(int prec, int dp, int64_t n, int64_t m, int k) {
-    Plan p;
-    const Tuning& t = tuning();
-    p.shape = default_shape(prec, dp);
-    if (dp <= 128) {  // wide rows have one shape: its query block size must match
-        if (t.query_tiles) p.shape.qt = t.query_tiles;
-        if (t.stages) p.shape.ns = t.stages;
-    }
-    const int64_t BQ = 4 * p.shape.qt * 32;
-    const int64_t qb = (m + BQ - 1) / BQ;
-    if (t.splits) {
-        p.splits = t.splits;
-    } else {
-        // about 16 blocks per CU, at least 128 rows per split, multiple of 8
-        int64_t want = (4096 + qb - 1) / qb;
-        int64_t most = std::max<int64_t>(1, n / 128);
-        want = std::min(want, most);
-        want = std::max<int64_t>(want, 1);
-        if (want >= 8) want = (want + 7) / 8 * 8;
-        p.splits = (int)std::min<int64_t>(want, 4096);
-    }
-    // slots per (query, split, lane half) sub-list
-    // (at least 6k slots per query over its 2 * splits sub-lists)
-    const int lists = 2 * p.splits;
-    // few lists per query (small databases, k-means centroids): more slots each
-    const int base = (prec == PREC_FP32 ? 24 : 48) * (lists <= 16 ? 2 : 1);
-    const int need = (6 * k + lists - 1) / lists + 4;
-    p.cap = t.capacity ? t.capacity : std::max(base, need);
-    if (!t.rounds.empty()) {
-        p.rounds = t.rounds;
-    } else {
-        // geometric rounds: each threshold comes from four to eight times fewer
-        // rows than the round it filters. Measured on the 1M x 128 mixture:
-        // five rounds beat seven (each launch and merge has a fixed cost that
-        // outweighs the extra candidates of the coarser steps).
-        p.rounds = {1.0 / 256, 1.0 / 32, 1.0 / 8, 1.0 / 2, 1.0};
-    }
-    return p;
-}
-
 int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q_offset, uint32_t flags,
                    double* dist, int64_t* idx, hipStream_t stream, const int64_t* /*batch_offsets*/,
                    int /*n_batches*/) {
@@ -817,7 +771,7 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
     Mirror& mr = index->mirror[prec];
     const int dp = mr.dp;
     ScreenFn screen = nullptr;
-    Plan plan = make_plan(prec, dp, n, m, k);
+    Plan plan = resolve_plan(PLAN_KNN, prec, dp, n, m, k);
     screen = pick_screen(SCREEN_LISTS, prec, dp, plan.shape);
     const bool all_fallback = !screen || k > 64 || mr.n_unsafe > 64 || (hooks().hooks & MLK_HOOK_FORCE_FALLBACK);
     if (hooks().hooks & MLK_HOOK_CAPACITY) plan.cap = (int)std::max<int64_t>(1, hooks().param);

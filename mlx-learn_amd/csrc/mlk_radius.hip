@@ -257,20 +257,13 @@ int radius_query_impl(mlk_index* index, const float* Q, int64_t m, double R2, in
     MLK_TRY(prepare_mirror(index, prec, stream));
     Mirror& mr = index->mirror[prec];
     const int dp = mr.dp;
-    const ScreenShape shape = default_shape(prec, dp);
-    ScreenFn screen = pick_screen(SCREEN_PAIRS, prec, dp, shape);
-    const int qt = shape.qt;
+    const Plan plan = resolve_plan(PLAN_RADIUS, prec, dp, n, m, 0);
+    ScreenFn screen = pick_screen(SCREEN_PAIRS, prec, dp, plan.shape);
+    const int qt = plan.shape.qt;
     const int BQ = 4 * qt * 32;
     const int64_t QB = (m + BQ - 1) / BQ;
     const int64_t m_pad = QB * BQ;
-    const Tuning& tn = tuning();
-    int S = tn.splits;
-    if (!S) {
-        int64_t want = std::max<int64_t>(1, (1024 + QB - 1) / QB);
-        want = std::min<int64_t>(want, std::max<int64_t>(1, n / 128));
-        if (want >= 8) want = (want + 7) / 8 * 8;
-        S = (int)std::min<int64_t>(want, 4096);
-    }
+    const int S = plan.splits;
     st.splits = S;
 
     DevBuf* ws = index->ws;

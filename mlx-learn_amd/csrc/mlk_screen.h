@@ -126,4 +126,35 @@ inline int effective_prec(int prec, int d) { return (prec == PREC_FP32 && d > 12
 // have 256 threads and cover 4 * qt * 32 queries.
 ScreenFn pick_screen(int mode, int prec, int dp, ScreenShape shape);
 
+// Every shape built for a mode, precision and padded width.
+std::vector<ScreenShape> screen_shapes(int mode, int prec, int dp);
+
+// ---------------------------------------------------------------- plans
+// How a k-NN or radius call tiles its screen: the shape, the database splits
+// of each round, the candidate slots per (query, split, lane half) sub-list
+// and the cumulative round fractions (k-NN only). Results never depend on
+// the plan; only the time does.
+struct Plan {
+    ScreenShape shape{1, 2};
+    int splits = 1;
+    int cap = 0;
+    std::vector<double> rounds;
+};
+enum PlanKind { PLAN_KNN = 0, PLAN_RADIUS = 1 };
+
+// The measured default plan for a call's shape. `shape` (qt > 0) replaces the
+// default screen shape; the splits follow from the query block it implies.
+Plan heuristic_plan(int kind, int prec, int dp, int64_t n, int64_t m, int k, ScreenShape shape = {0, 0});
+
+// Candidate slots per sub-list: at least 6k per query over its 2 * splits
+// sub-lists, with a per-precision floor.
+int knn_capacity(int prec, int splits, int k);
+
+// The plan a call uses: the autotuner's trial plan of the calling thread if
+// one is set, else the tuned plan cached for the call's shape (kind,
+// precision, padded width, k, n and m in half-octave buckets), else the
+// heuristic; mlk_set_tuning's non-zero knobs override fields of the last two.
+// *tuned reports whether a cached plan was used.
+Plan resolve_plan(int kind, int prec, int dp, int64_t n, int64_t m, int k, bool* tuned = nullptr);
+
 }  // namespace mlk

@@ -486,4 +486,16 @@ ScreenFn pick_screen(int mode, int prec, int dp, ScreenShape shape) {
     return pick_shape<PREC_FP16, SCREEN_PAIRS>(dp, shape.qt, shape.ns);
 }
 
+std::vector<ScreenShape> screen_shapes(int mode, int prec, int dp) {
+    std::vector<ScreenShape> out;
+    if (dp > 128) {  // wide rows: one shape, whatever is asked for
+        if (pick_screen(mode, prec, dp, default_shape(prec, dp))) out.push_back(default_shape(prec, dp));
+        return out;
+    }
+    for (int qt = 1; qt <= 2; ++qt)
+        for (int ns = 2; ns <= 4; ++ns)
+            if (pick_screen(mode, prec, dp, ScreenShape{qt, ns})) out.push_back(ScreenShape{qt, ns});
+    return out;
+}
+
 }  // namespace mlk

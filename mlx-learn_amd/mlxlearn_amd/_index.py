@@ -159,6 +159,34 @@ class DeviceNeighborIndex:
             self._lib.mlk_radius_free(res)
         return indptr, indices, (dist if need_distance else None)
 
+    def autotune(self, Q, k=None, *, radius_sq=None, precision="float32", sort_results=False, max_trials=24):
+        """Time screen plans on these queries and cache the fastest for calls of this shape.
+
+        k-NN with ``k``; radius search with ``radius_sq`` (R2) instead. Returns the
+        chosen plan with its time and the untuned plan's time (milliseconds).
+        """
+        Qd = to_device_f32(Q)
+        if Qd.ndim != 2 or Qd.shape[1] != self.n_features:
+            raise ValueError(f"queries must have {self.n_features} features")
+        if (k is None) == (radius_sq is None):
+            raise ValueError("give exactly one of k and radius_sq")
+        flags = precision_flag(precision)
+        if radius_sq is not None:
+            flags |= _lib.TUNE_RADIUS | _lib.NEED_DISTANCE | (_lib.SORT_RESULTS if sort_results else 0)
+        plan = _lib.Plan()
+        _lib.check(self._lib.mlk_autotune(self._handle, _ptr(Qd), int(Qd.shape[0]), int(k or 0),
+                                          float(radius_sq or 0.0), flags, int(max_trials), ctypes.byref(plan),
+                                          _stream()), "mlk_autotune")
+        return plan.to_dict()
+
+    def plan(self, m, k=None, *, radius=False, precision="float32"):
+        """The plan a call with m queries (k neighbours, or a radius call) would use."""
+        flags = precision_flag(precision) | (_lib.TUNE_RADIUS if radius else 0)
+        plan = _lib.Plan()
+        _lib.check(self._lib.mlk_get_plan(self._handle, int(m), int(k or 0), flags, ctypes.byref(plan)),
+                   "mlk_get_plan")
+        return plan.to_dict()
+
 
 @dataclass
 class RadiusCSR:

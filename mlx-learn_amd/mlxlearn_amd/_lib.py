@@ -27,6 +27,8 @@ HOOK_FORCE_FALLBACK = 0x1
 HOOK_CAPACITY = 0x2
 HOOK_FAIL = 0x4
 HOOK_FORCE_UNCERTIFIED = 0x8
+TUNE_RADIUS = 0x1000
+MAX_ROUNDS = 8
 
 PRECISIONS = {"float32": PREC_FP32, "bfloat16": PREC_BF16, "float16": PREC_FP16}
 
@@ -39,6 +41,18 @@ class Stats(ctypes.Structure):
     _fields_ = [("queries", _i64), ("candidates", _i64), ("reranked", _i64), ("fallback_queries", _i64),
                 ("rounds", _i64), ("splits", _i64), ("capacity", _i64), ("screen_precision", _i64),
                 ("screen_ms", _dbl), ("total_ms", _dbl)]
+
+
+class Plan(ctypes.Structure):
+    _fields_ = [("splits", _i32), ("capacity", _i32), ("query_tiles", _i32), ("stages", _i32), ("n_rounds", _i32),
+                ("rounds", _dbl * MAX_ROUNDS), ("tuned", _i32), ("trials", _i32), ("tuned_ms", _dbl),
+                ("default_ms", _dbl)]
+
+    def to_dict(self) -> dict:
+        return {"splits": self.splits, "capacity": self.capacity, "query_tiles": self.query_tiles,
+                "stages": self.stages, "rounds": [self.rounds[i] for i in range(self.n_rounds)],
+                "tuned": bool(self.tuned), "trials": self.trials, "tuned_ms": self.tuned_ms,
+                "default_ms": self.default_ms}
 
 
 SIGNATURES = {
@@ -59,6 +73,9 @@ SIGNATURES = {
     "mlk_kmeans_assign": ([_vp, _i64, _i32, _vp, _i64, _u32, _vp, _vp, _vp], ctypes.c_int),
     "mlk_kmeans_update": ([_vp, _i64, _i32, _vp, _i64, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
     "mlk_knn_votes": ([_vp, _vp, _i64, _i32, _vp, _i32, _i32, _vp, _vp, _vp], ctypes.c_int),
+    "mlk_autotune": ([_vp, _vp, _i64, _i32, _dbl, _u32, _i32, ctypes.POINTER(Plan), _vp], ctypes.c_int),
+    "mlk_get_plan": ([_vp, _i64, _i32, _u32, ctypes.POINTER(Plan)], ctypes.c_int),
+    "mlk_clear_plans": ([], ctypes.c_int),
 }
 
 _LIB = None
@@ -115,3 +132,8 @@ def set_tuning(splits: int = 0, capacity: int = 0, query_tiles: int = 0, stages:
     arr = (ctypes.c_double * max(len(rounds), 1))(*rounds)
     check(load().mlk_set_tuning(int(splits), int(capacity), int(query_tiles), int(stages), len(rounds), arr),
           "mlk_set_tuning")
+
+
+def clear_plans() -> None:
+    """Forget every autotuned plan of this process."""
+    check(load().mlk_clear_plans(), "mlk_clear_plans")

@@ -38,6 +38,12 @@ def test_library_validation_without_device():
     assert rc == 1 and b"index is NULL" in lib.mlk_last_error()
     assert lib.mlk_set_tuning(-1, 0, 0, 0, 0, None) == 1
     assert lib.mlk_set_tuning(0, 0, 0, 1, 0, None) == 1  # a one-stage ring does not exist
+    plan = _lib.Plan()
+    assert lib.mlk_autotune(None, None, 10, 5, 0.0, 0, 0, ctypes.byref(plan), None) == 1
+    assert b"index is NULL" in lib.mlk_last_error()
+    assert lib.mlk_autotune(None, None, 10, 5, 0.0, 0, 0, None, None) == 1
+    assert lib.mlk_get_plan(None, 10, 5, 0, ctypes.byref(plan)) == 1
+    assert lib.mlk_clear_plans() == 0
     with pytest.raises(ValueError):
         _lib.check(1)
     with pytest.raises(MemoryError):

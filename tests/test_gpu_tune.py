@@ -1,0 +1,135 @@
+"""Screen plans and the autotuner on the GPU.
+
+Tile planning is performance-only in the reference (resolve_blocks,
+_neighbors_plan.py:74-126; blocking invariance, tests/parity/test_neighbors.py:57-75):
+every plan -- each built screen shape, any split count, round schedule or
+tuned plan -- must return the oracle's indices bit for bit.
+"""
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+PRECISIONS = ["float32", "bfloat16", "float16"]
+SHAPES = {"float32": [(1, 2), (1, 3)], "bfloat16": [(1, 2), (2, 2), (2, 3), (2, 4)],
+          "float16": [(1, 2), (2, 2), (2, 3), (2, 4)]}
+
+
+@pytest.fixture(scope="module")
+def mlk():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import mlxlearn_amd
+    from mlxlearn_amd import _lib
+    _lib.load()
+    yield mlxlearn_amd
+    _lib.set_tuning()
+    _lib.clear_plans()
+
+
+def knn(index, Q, k, precision):
+    d, i = index.query(Q, k, precision=precision)
+    return d.cpu().numpy(), i.cpu().numpy()
+
+
+def assert_parity(got, want):
+    np.testing.assert_array_equal(got[1], want[1])
+    np.testing.assert_allclose(got[0], want[0], rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_every_screen_shape_matches_oracle(mlk, precision):
+    """Each built (query tiles, stages) shape, narrow widths 32..128, ragged query counts."""
+    from mlxlearn_amd import _lib
+    r = np.random.default_rng(4400)
+    for d in (20, 64, 128):
+        X, _ = oracle.gen_c2(seed=d, n=20_000, m=1)
+        X = np.ascontiguousarray(X[:, :d])
+        Q = np.concatenate([X[:3], r.normal(size=(297, d)).astype(np.float32) * 2])
+        want = oracle.knn(X, Q, 7)
+        index = mlk.DeviceNeighborIndex(X)
+        try:
+            for qt, ns in SHAPES[precision]:
+                _lib.set_tuning(query_tiles=qt, stages=ns)
+                assert_parity(knn(index, Q, 7, precision), want)
+                st = _lib.stats()
+                assert st["fallback_queries"] == 0, (d, qt, ns)
+        finally:
+            _lib.set_tuning()
+            index.close()
+
+
+@pytest.mark.parametrize("precision", ["float32", "bfloat16"])
+def test_autotuned_knn_plan_is_used_and_exact(mlk, precision):
+    from mlxlearn_amd import _lib
+    _lib.clear_plans()
+    X, Q = oracle.gen_c2(seed=31, n=50_000, m=1500)
+    want = oracle.knn(X, Q, 10)
+    index = mlk.DeviceNeighborIndex(X)
+    try:
+        before = index.plan(1500, 10, precision=precision)
+        assert not before["tuned"] and before["rounds"][-1] == 1.0
+        assert_parity(knn(index, Q, 10, precision), want)
+        plan = index.autotune(Q, 10, precision=precision, max_trials=12)
+        assert plan["tuned"] and 2 <= plan["trials"] <= 12
+        assert 0 < plan["tuned_ms"] <= plan["default_ms"]
+        after = index.plan(1500, 10, precision=precision)
+        assert after["tuned"] and after["splits"] == plan["splits"] and after["capacity"] == plan["capacity"]
+        assert (after["query_tiles"], after["stages"]) == (plan["query_tiles"], plan["stages"])
+        # the same query-count bucket uses the plan; another k does not
+        assert index.plan(1400, 10, precision=precision)["tuned"]
+        assert not index.plan(1500, 11, precision=precision)["tuned"]
+        got = knn(index, Q, 10, precision)
+        assert _lib.stats()["splits"] == plan["splits"]
+        assert_parity(got, want)
+        _lib.clear_plans()
+        assert not index.plan(1500, 10, precision=precision)["tuned"]
+    finally:
+        index.close()
+
+
+def test_autotuned_radius_plan_is_exact(mlk):
+    from mlxlearn_amd import _lib
+    _lib.clear_plans()
+    X, Q, sizes, radius = oracle.gen_c4(seed=3, n=100_000, n_batches=12, max_batch=400)
+    m = Q.shape[0]
+    R2 = radius * radius
+    want = oracle.radius(X, Q, R2, sort_results=True)
+    index = mlk.DeviceNeighborIndex(X)
+    try:
+        plan = index.autotune(Q, radius_sq=R2, sort_results=True, max_trials=8)
+        assert plan["tuned"] and plan["rounds"] == [] and plan["tuned_ms"] <= plan["default_ms"]
+        assert index.plan(m, radius=True)["splits"] == plan["splits"]
+        indptr, indices, dist = index.radius(Q, R2, sort_results=True)
+        assert _lib.stats()["splits"] == plan["splits"]
+        np.testing.assert_array_equal(indptr.cpu().numpy(), want[0])
+        np.testing.assert_array_equal(indices.cpu().numpy(), want[1])
+        np.testing.assert_allclose(dist.cpu().numpy(), want[2], rtol=1e-12, atol=0)
+    finally:
+        _lib.clear_plans()
+        index.close()
+
+
+def test_kmeans_assignment_uses_tuned_plan(mlk):
+    """A plan tuned on a centres index applies to mlk_kmeans_assign of the same shape."""
+    from mlxlearn_amd import _lib
+    from mlxlearn_amd.cluster._k_means import assign_device
+    _lib.clear_plans()
+    r = np.random.default_rng(77)
+    X = r.normal(size=(40_000, 24)).astype(np.float32)
+    C = X[r.choice(40_000, 300, replace=False)] + np.float32(0.01)
+    index = mlk.DeviceNeighborIndex(C)
+    try:
+        plan = index.autotune(X, 1, precision="bfloat16", max_trials=10)
+    finally:
+        index.close()
+    labels, resid = assign_device(X, C, precision="bfloat16")
+    st = _lib.stats()
+    assert st["splits"] == plan["splits"] and st["capacity"] == plan["capacity"]
+    want_l, want_r = oracle.kmeans_assign(X, C)
+    np.testing.assert_array_equal(labels.cpu().numpy(), want_l)
+    np.testing.assert_allclose(resid.cpu().numpy(), want_r, rtol=1e-12, atol=0)
+    _lib.clear_plans()
