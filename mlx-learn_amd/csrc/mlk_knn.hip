@@ -81,11 +81,21 @@ __device__ __forceinline__ double fast_sq(const float* __restrict__ q, const flo
 __global__ void k_tau0(const float* __restrict__ X, int64_t n, int d, const float* __restrict__ Q, int64_t m,
                        int64_t m_pad, int k, int max_samples, int64_t self_off, int exclude, const double* __restrict__ nq,
                        const uint8_t* __restrict__ qflags, Consts c, float* __restrict__ tau,
-                       float* __restrict__ thr, int* __restrict__ status) {
+                       float* __restrict__ thr, int* __restrict__ status, int S2, int* __restrict__ cnt,
+                       int* __restrict__ seen, float* __restrict__ kbest) {
     __shared__ uint32_t sb[4][TAU0_SAMPLES];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int64_t q = (int64_t)blockIdx.x * 4 + w;
     if (q >= m_pad) return;
+    // the query's candidate sub-list counters start empty and its k best
+    // upper bounds at +inf (screened calls only)
+    if (cnt) {
+        for (int j = lane; j < S2; j += 64) {
+            cnt[q * S2 + j] = 0;
+            seen[q * S2 + j] = 0;
+        }
+        for (int j = lane; j < k; j += 64) kbest[q * k + j] = INFINITY;
+    }
     if (q >= m) {
         if (lane == 0) { tau[q] = -1.0f; thr[q] = INFINITY; status[q] = 0; }
         return;
@@ -385,7 +395,10 @@ __global__ __launch_bounds__(256) void k_rank(const RankArgs a, int64_t n_items)
     int64_t q;
     if (a.mode == 0) {
         q = item;
-        if (a.status[q]) return;  // handled by the fallback
+        if (a.status[q]) {  // handled by the fallback
+            if (lane == 0 && a.qstats) a.qstats[3 * q] = a.qstats[3 * q + 1] = a.qstats[3 * q + 2] = 0;
+            return;
+        }
     } else {
         q = a.fq[item];
     }
@@ -445,7 +458,10 @@ __global__ __launch_bounds__(256) void k_rank(const RankArgs a, int64_t n_items)
         int extra = 0;
         for (int u = 0; u < a.n_unsafe; ++u) extra += a.unsafe[u] != self;
         if (over || total + extra > LCAP) {
-            if (lane == 0) a.status[q] = ST_OVERFLOW;
+            if (lane == 0) {
+                a.status[q] = ST_OVERFLOW;
+                if (a.qstats) a.qstats[3 * q] = a.qstats[3 * q + 1] = a.qstats[3 * q + 2] = 0;
+            }
             return;
         }
         if (lane == 0) {
@@ -626,9 +642,18 @@ __global__ __launch_bounds__(256) void k_rank(const RankArgs a, int64_t n_items)
 
 // ---------------------------------------------------------------- fallback
 __global__ void k_fb_dist(const float* __restrict__ X, int64_t n, int d, const float* __restrict__ Q,
-                          const int* __restrict__ fq, int64_t self_off, int exclude, double* __restrict__ s_out) {
+                          const int* __restrict__ fq, int64_t self_off, int exclude, double* __restrict__ s_out,
+                          int k, unsigned* __restrict__ hist, unsigned long long* __restrict__ prefix,
+                          unsigned* __restrict__ rank) {
     // 16 lanes per row, 16 rows per block pass; float64 partial sums in any order.
     const int f = blockIdx.y;
+    if (blockIdx.x == 0) {  // radix-select state of this query
+        hist[f * 256 + threadIdx.x] = 0;
+        if (threadIdx.x == 0) {
+            prefix[f] = 0;
+            rank[f] = (unsigned)k;
+        }
+    }
     const int64_t q = fq[f];
     const float* qv = Q + q * d;
     const int g = threadIdx.x & 15;
@@ -645,44 +670,53 @@ __global__ void k_fb_dist(const float* __restrict__ X, int64_t n, int d, const f
     if (row < n && g == 0) s_out[(size_t)f * n + row] = (exclude && row == self_off + q) ? INFINITY : s;
 }
 
-// k-th smallest float64 of each fallback row by 8-bit radix select over the bit pattern.
-__global__ void k_fb_select(const double* __restrict__ s_in, int64_t n, int k, int d, double* __restrict__ cut) {
-    __shared__ unsigned hist[256];
-    __shared__ unsigned long long s_prefix;
-    __shared__ unsigned s_rank;
-    const int f = blockIdx.x;
+// k-th smallest float64 of each fallback query by 8-bit radix select over the
+// bit pattern (non-negative doubles order like their bits): eight passes, each
+// a histogram of the values that share the digits chosen so far, spread over
+// many blocks per query (k_fb_hist), then one block per query that picks the
+// digit holding the k-th value (k_fb_pick). k_fb_dist sets prefix 0, rank k and
+// clears the histograms.
+__global__ void k_fb_hist(const double* __restrict__ s_in, int64_t n, int shift,
+                          const unsigned long long* __restrict__ prefix, unsigned* __restrict__ hist) {
+    __shared__ unsigned h[256];
+    const int f = blockIdx.y;
+    h[threadIdx.x] = 0;
+    __syncthreads();
     const double* s = s_in + (size_t)f * n;
-    unsigned long long prefix = 0, mask = 0;
-    unsigned rank = (unsigned)k;  // 1-based rank within the current prefix class
-    for (int shift = 56; shift >= 0; shift -= 8) {
-        for (int b = threadIdx.x; b < 256; b += blockDim.x) hist[b] = 0;
-        __syncthreads();
-        for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
-            const unsigned long long v = (unsigned long long)__double_as_longlong(s[i]);
-            if ((v & mask) == prefix) atomicAdd(&hist[(v >> shift) & 255], 1u);
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            unsigned acc = 0;
-            int b = 0;
-            for (; b < 256; ++b) {
-                if (acc + hist[b] >= rank) break;
-                acc += hist[b];
-            }
-            if (b == 256) b = 255;
-            s_prefix = prefix | ((unsigned long long)b << shift);
-            s_rank = rank - acc;
-        }
-        __syncthreads();
-        prefix = s_prefix;
-        rank = s_rank;
-        mask |= 255ull << shift;
-        __syncthreads();
+    const unsigned long long pre = prefix[f];
+    const unsigned long long mask = shift >= 56 ? 0ull : ~0ull << (shift + 8);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const unsigned long long v = (unsigned long long)__double_as_longlong(s[i]);
+        if ((v & mask) == pre) atomicAdd(&h[(v >> shift) & 255], 1u);
     }
+    __syncthreads();
+    if (h[threadIdx.x]) atomicAdd(&hist[f * 256 + threadIdx.x], h[threadIdx.x]);
+}
+
+__global__ void k_fb_pick(int shift, int d, unsigned* __restrict__ hist, unsigned long long* __restrict__ prefix,
+                          unsigned* __restrict__ rank, double* __restrict__ cut) {
+    __shared__ unsigned h[256];
+    const int f = blockIdx.x;
+    h[threadIdx.x] = hist[f * 256 + threadIdx.x];
+    hist[f * 256 + threadIdx.x] = 0;  // ready for the next pass
+    __syncthreads();
     if (threadIdx.x == 0) {
-        const double tk = __longlong_as_double((long long)prefix);
-        const double g = f64_interval(d);
-        cut[f] = tk < INFINITY ? tk * (1.0 + g) / (1.0 - g) * (1.0 + 0x1p-50) : INFINITY;
+        const unsigned want = rank[f];  // 1-based rank within the current prefix class
+        unsigned acc = 0;
+        int b = 0;
+        for (; b < 256; ++b) {
+            if (acc + h[b] >= want) break;
+            acc += h[b];
+        }
+        if (b == 256) b = 255;
+        const unsigned long long p = prefix[f] | ((unsigned long long)b << shift);
+        prefix[f] = p;
+        rank[f] = want - acc;
+        if (shift == 0) {
+            const double tk = __longlong_as_double((long long)p);
+            const double g = f64_interval(d);
+            cut[f] = tk < INFINITY ? tk * (1.0 + g) / (1.0 - g) * (1.0 + 0x1p-50) : INFINITY;
+        }
     }
 }
 
@@ -707,11 +741,6 @@ __global__ void k_qprep_status(int64_t m, const int* __restrict__ status, int* _
         const unsigned at = atomicAdd(nfq, 1u);
         fq[at] = (int)q;
     }
-}
-
-__global__ void k_fill_f32(float* __restrict__ p, int64_t n, float v) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
 }
 
 __global__ void k_force(int64_t m, const int64_t* __restrict__ rows, int64_t count, int* __restrict__ status,
@@ -796,12 +825,18 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
     unsigned* dflag;
     MLK_TRY(ws[6].reserve(64));
     dflag = ws[6].as<unsigned>();
-    MLK_CHECK_HIP(hipMemsetAsync(dflag, 0, 16, stream));
+    // flag words: [0] query rounding inexact, [2] fallback count, [4..10) three
+    // 64-bit statistics sums
+    MLK_CHECK_HIP(hipMemsetAsync(dflag, 0, 40, stream));
     MLK_TRY(mirror_rows(prec, Q, m, m_pad, d, dp, index->centre.as<float>(), mr.scale, ws[0].ptr,
                         ws[1].as<double>(), ws[2].as<uint8_t>(), dflag, stream));
+    // the queries' rounding only matters when every database value is exact
+    // in the screen type (integer data); otherwise no wait here
     unsigned q_inexact = 1;
-    MLK_CHECK_HIP(hipMemcpyAsync(&q_inexact, dflag, 4, hipMemcpyDeviceToHost, stream));
-    MLK_CHECK_HIP(hipStreamSynchronize(stream));
+    if (mr.exact) {
+        MLK_CHECK_HIP(hipMemcpyAsync(&q_inexact, dflag, 4, hipMemcpyDeviceToHost, stream));
+        MLK_CHECK_HIP(hipStreamSynchronize(stream));
+    }
     const Consts c = make_consts(prec, dp, d, mr.exact && q_inexact == 0, mr.scale);
     MLK_TRY(prepare_bias(index, prec, c.G, stream));
 
@@ -811,9 +846,17 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
     int* status = ws[5].as<int>();
     // sampled start threshold: 32 rows per neighbour asked for, 64 to 256 rows
     const int tau_samples = std::min(TAU0_SAMPLES, std::max(64, 32 * k));
+    const int S2 = 2 * S;  // candidate sub-lists per query (split x lane half)
+    if (!all_fallback) {
+        MLK_TRY(ws[8].reserve((size_t)m_pad * S2 * sizeof(int)));          // counts
+        MLK_TRY(ws[9].reserve((size_t)m_pad * S2 * cap * sizeof(int2)));   // (row, dot-product estimate)
+        MLK_TRY(ws[17].reserve((size_t)m_pad * S2 * sizeof(int)));         // merged counts
+        MLK_TRY(ws[18].reserve((size_t)m_pad * k * sizeof(float)));        // k smallest upper bounds
+    }
     hipLaunchKernelGGL(k_tau0, dim3((unsigned)((m_pad + 3) / 4)), dim3(256), 0, stream, X, n, d, Q, m, m_pad, k,
                        tau_samples, q_offset, (int)exclude, ws[1].as<double>(), ws[2].as<uint8_t>(), c, tau, thr,
-                       status);
+                       status, S2, all_fallback ? nullptr : ws[8].as<int>(), ws[17].as<int>(),
+                       ws[18].as<float>());
     MLK_CHECK_HIP(hipGetLastError());
 
     if ((hooks().hooks & MLK_HOOK_FORCE_UNCERTIFIED) && !hooks().forced_rows.empty()) {
@@ -829,15 +872,6 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
     MLK_CHECK_HIP(hipEventRecord(ev1, stream));
     std::vector<hipEvent_t> round_events;
     if (!all_fallback) {
-        const int S2 = 2 * S;  // candidate sub-lists per query (split x lane half)
-        MLK_TRY(ws[8].reserve((size_t)m_pad * S2 * sizeof(int)));          // counts
-        MLK_TRY(ws[9].reserve((size_t)m_pad * S2 * cap * sizeof(int2)));   // (row, dot-product estimate)
-        MLK_TRY(ws[17].reserve((size_t)m_pad * S2 * sizeof(int)));         // merged counts
-        MLK_TRY(ws[18].reserve((size_t)m_pad * k * sizeof(float)));        // k smallest upper bounds
-        MLK_CHECK_HIP(hipMemsetAsync(ws[8].ptr, 0, (size_t)m_pad * S2 * sizeof(int), stream));
-        MLK_CHECK_HIP(hipMemsetAsync(ws[17].ptr, 0, (size_t)m_pad * S2 * sizeof(int), stream));
-        hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)((m_pad * k + 255) / 256)), dim3(256), 0, stream,
-                           ws[18].as<float>(), (int64_t)(m_pad * k), INFINITY);
         ScreenArgs sa{};
         sa.xs = mr.xs.ptr;
         sa.bias = mr.bias.as<float>();
@@ -904,11 +938,9 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
         ra.mode = 0; ra.self_off = q_offset; ra.exclude = exclude; ra.status = status;
         ra.out_idx = idx; ra.out_dist = dist; ra.squared = (flags & MLK_SQUARED_DIST) != 0;
         MLK_TRY(ws[19].reserve((size_t)m * 3 * sizeof(int)));
-        ra.qstats = ws[19].as<int>();
-        MLK_CHECK_HIP(hipMemsetAsync(ra.qstats, 0, (size_t)m * 3 * sizeof(int), stream));
+        ra.qstats = ws[19].as<int>();  // written for every query by k_rank
         hipLaunchKernelGGL(k_rank, dim3((unsigned)((m + 3) / 4)), dim3(256), 0, stream, ra, m);
         MLK_CHECK_HIP(hipGetLastError());
-        MLK_CHECK_HIP(hipMemsetAsync(dflag + 4, 0, 3 * sizeof(unsigned long long), stream));
         hipLaunchKernelGGL(k_sum_qstats, dim3((unsigned)std::min<int64_t>(1024, (m + 255) / 256)), dim3(256), 0,
                            stream, ra.qstats, m, (unsigned long long*)(dflag + 4));
         MLK_CHECK_HIP(hipGetLastError());
@@ -922,20 +954,21 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
     // queries without a certificate -> exact fallback
     MLK_TRY(ws[11].reserve((size_t)m * sizeof(int)));
     int* fq = ws[11].as<int>();
-    MLK_CHECK_HIP(hipMemsetAsync(dflag + 2, 0, 4, stream));
     hipLaunchKernelGGL(k_qprep_status, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, stream, m, status, fq,
                        dflag + 2);
     MLK_CHECK_HIP(hipGetLastError());
-    unsigned nf = 0;
-    MLK_CHECK_HIP(hipMemcpyAsync(&nf, dflag + 2, 4, hipMemcpyDeviceToHost, stream));
+    // one read-back: the fallback count and the statistics sums
+    struct {
+        unsigned nf, pad;
+        unsigned long long cnts[3];
+    } words{};
+    static_assert(sizeof(words) == 32, "flag words 2..10");
+    MLK_CHECK_HIP(hipMemcpyAsync(&words, dflag + 2, sizeof(words), hipMemcpyDeviceToHost, stream));
     MLK_CHECK_HIP(hipStreamSynchronize(stream));
+    const unsigned nf = words.nf;
     st.fallback_queries = nf;
-    if (!all_fallback) {
-        unsigned long long cnts[3] = {0, 0, 0};
-        MLK_CHECK_HIP(hipMemcpy(cnts, dflag + 4, sizeof(cnts), hipMemcpyDeviceToHost));
-        st.candidates = (int64_t)cnts[0];
-        st.reranked = (int64_t)cnts[2];
-    }
+    st.candidates = (int64_t)words.cnts[0];
+    st.reranked = (int64_t)words.cnts[2];
     if (nf) {
         std::vector<int> fq_host(nf);
         MLK_CHECK_HIP(hipMemcpy(fq_host.data(), fq, nf * sizeof(int), hipMemcpyDeviceToHost));
@@ -947,17 +980,33 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
         MLK_TRY(ws[12].reserve((size_t)fb * n * sizeof(double)));  // distances
         MLK_TRY(ws[13].reserve((size_t)fb * n * sizeof(int)));     // plist
         MLK_TRY(ws[14].reserve((size_t)fb * n * sizeof(double)));  // ps
-        MLK_TRY(ws[15].reserve((size_t)fb * (sizeof(int) + sizeof(double)) + 64));
-        int* pcount = ws[15].as<int>();
-        double* cut = (double*)((char*)ws[15].ptr + ((fb * sizeof(int) + 15) / 16) * 16);
+        // per fallback query: count, cut, radix prefix and rank, 256-bin histogram
+        const size_t per = sizeof(int) + sizeof(double) + sizeof(unsigned long long) + sizeof(unsigned) + 1024;
+        MLK_TRY(ws[15].reserve((size_t)fb * per + 5 * 64));  // five pieces, each 64-byte aligned
+        char* wp = (char*)ws[15].ptr;
+        auto carve = [&](size_t bytes) {
+            char* at = wp;
+            wp += (bytes + 63) / 64 * 64;
+            return at;
+        };
+        int* pcount = (int*)carve(fb * sizeof(int));
+        double* cut = (double*)carve(fb * sizeof(double));
+        unsigned long long* rprefix = (unsigned long long*)carve(fb * sizeof(unsigned long long));
+        unsigned* rrank = (unsigned*)carve(fb * sizeof(unsigned));
+        unsigned* hist = (unsigned*)carve(fb * 1024);
         for (int64_t f0 = 0; f0 < (int64_t)nf; f0 += fb) {
             const int64_t nb = std::min<int64_t>(fb, nf - f0);
             dim3 g2((unsigned)((n + 255) / 256), (unsigned)nb);
             dim3 g16((unsigned)((n + 15) / 16), (unsigned)nb);
             hipLaunchKernelGGL(k_fb_dist, g16, dim3(256), 0, stream, X, n, d, Q, fq + f0, q_offset, (int)exclude,
-                               ws[12].as<double>());
-            hipLaunchKernelGGL(k_fb_select, dim3((unsigned)nb), dim3(256), 0, stream, ws[12].as<double>(), n, k, d,
-                               cut);
+                               ws[12].as<double>(), k, hist, rprefix, rrank);
+            // about 4096 values per histogram block
+            const dim3 gh((unsigned)std::max<int64_t>(1, std::min<int64_t>(256, (n + 4095) / 4096)), (unsigned)nb);
+            for (int shift = 56; shift >= 0; shift -= 8) {
+                hipLaunchKernelGGL(k_fb_hist, gh, dim3(256), 0, stream, ws[12].as<double>(), n, shift, rprefix, hist);
+                hipLaunchKernelGGL(k_fb_pick, dim3((unsigned)nb), dim3(256), 0, stream, shift, d, hist, rprefix,
+                                   rrank, cut);
+            }
             MLK_CHECK_HIP(hipMemsetAsync(pcount, 0, nb * sizeof(int), stream));
             hipLaunchKernelGGL(k_fb_collect, g2, dim3(256), 0, stream, ws[12].as<double>(), n, cut, pcount,
                                ws[13].as<int>(), ws[14].as<double>(), n);
