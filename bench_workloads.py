@@ -315,6 +315,20 @@ def run_c5(args):
                      "max_rel_residual_err": float(np.max(np.abs(res[rows] - wr) / np.maximum(wr, 1e-300))),
                      "inertia": float(inertia.item()), "inertia_fsum_rel_err":
                          abs(float(inertia.item()) - math.fsum(res.tolist())) / max(math.fsum(res.tolist()), 1e-300)}
+    # a Lloyd iteration after the first: the updated centres, assigned with
+    # certified reuse of the labels above (lloyd_bounds.reusable_labels) and
+    # without it; both give the same labels
+    sums, counts, _ = update_device(Xd, labels, kc, resid)
+    C1 = torch.where((counts > 0)[:, None], sums / counts.clamp(min=1).to(torch.float64)[:, None],
+                     Cd.to(torch.float64)).to(torch.float32).contiguous()
+    reused = []
+    t_reuse, (l_reuse, _), _ = timed(lambda: assign_device(Xd, C1, precision="bfloat16", previous=labels,
+                                                           out_reused=reused), 3, 1)
+    t_full, (l_full, _), _ = timed(lambda: assign_device(Xd, C1, precision="bfloat16"), 3, 1)
+    line["lloyd_iteration_2"] = {"assign_reuse_ms": round(t_reuse * 1e3, 3), "assign_full_ms": round(t_full * 1e3, 3),
+                                 "reused_rows": reused[-1], "reused_fraction": round(reused[-1] / n, 4),
+                                 "labels_equal": bool(torch.equal(l_reuse, l_full))}
+    log(f"c5 iteration 2: {line['lloyd_iteration_2']}")
     if not args.no_cpu:
         nr = 256
         t0 = time.perf_counter()

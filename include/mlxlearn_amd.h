@@ -176,6 +176,22 @@ int mlk_kmeans_assign(const float* X_dev, int64_t n, int32_t d, const float* C_d
                       uint32_t flags, int64_t* labels_dev, double* residual_dev, void* stream);
 
 /*
+ * k-means assignment that reuses the previous labels prev_dev (n int64) where
+ * a certificate proves them exact: a row keeps centre j when the distance from
+ * j to its nearest other centre exceeds twice the row's distance to j (float64
+ * bounds, so j is then the unique nearest centre); every other row gets the
+ * full exact assignment. Outputs are bit-identical to mlk_kmeans_assign's;
+ * labels_dev may be prev_dev. *reused (host, optional) receives the number of
+ * rows that kept their label without a search. Replaces reusable_labels /
+ * separation_lower and the pruned assignment tiles of the Lloyd loop
+ * (src/mlxlearn/_kernels/lloyd_bounds.py:24-76;
+ * src/mlxlearn/cluster/_k_means.py:159-175, 309-360).
+ */
+int mlk_kmeans_assign_reuse(const float* X_dev, int64_t n, int32_t d, const float* C_dev, int64_t kc,
+                            const int64_t* prev_dev, uint32_t flags, int64_t* labels_dev,
+                            double* residual_dev, int64_t* reused, void* stream);
+
+/*
  * k-means update: float64 per-cluster coordinate sums (kc x d), counts (kc)
  * and inertia (sum of residuals), deterministic (fixed summation order).
  * Replaces KMeans._statistics / e_means.batch_stats

@@ -78,6 +78,106 @@ __global__ void k_sum_final(const double* __restrict__ partial, int np, double* 
     }
 }
 
+// ------------------------------------------------------- label reuse (Lloyd)
+// Certified reuse of the previous labels (reference lloyd_bounds.py:24-76,
+// which certifies against float32 scores with rounding allowances; here the
+// certificate is exact). With sep_j the distance from centre j to its nearest
+// other centre and R the distance from row x to centre j: if sep_j > 2R, then
+// for every other centre i, |x - c_i| >= |c_i - c_j| - |x - c_j| >= sep_j - R > R,
+// so j is the unique nearest centre and the row keeps it. sep_j is bounded
+// below and R above from their canonical float64 squares (relative error at
+// most f64_interval(d)), with a 2^-50 margin for the square roots.
+
+// Lower bound of each centre's separation from its exact nearest-other-centre
+// squared distance (float64, +inf when there is no other centre).
+__global__ void k_sep_lower(const double* __restrict__ sep2, int64_t kc, int d, double* __restrict__ sep_lo) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= kc) return;
+    const double s = sep2 ? sep2[j] : INFINITY;
+    sep_lo[j] = s < INFINITY ? sqrt(fmax(s, 0.0) * (1.0 - f64_interval(d))) * (1.0 - 0x1p-50) : INFINITY;
+}
+
+// One thread per row: canonical residual to the previous centre and the
+// certificate; uncertain rows (and labels outside [0, kc), counted in
+// words[1]) are appended to `uncertain`, one atomic per wave.
+__global__ void k_reuse(const float* __restrict__ X, int64_t n, int d, const float* __restrict__ C, int64_t kc,
+                        const int64_t* __restrict__ prev, const double* __restrict__ sep_lo,
+                        int64_t* __restrict__ labels, double* __restrict__ residual, int* __restrict__ uncertain,
+                        unsigned* __restrict__ words) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    bool keep = false, bad = false;
+    if (i < n) {
+        const int64_t j = prev[i];
+        bad = j < 0 || j >= kc;
+        if (!bad) {
+            const double r2 = canonical_sq(X + i * d, C + j * d, d);
+            const double r_hi = sqrt(r2 * (1.0 + f64_interval(d))) * (1.0 + 0x1p-50);
+            keep = sep_lo[j] > 2.0 * r_hi;
+            if (keep) {
+                labels[i] = j;
+                residual[i] = r2;
+            }
+        }
+    }
+    const bool send = i < n && !keep;
+    const unsigned long long mask = ballot(send);
+    const unsigned long long badmask = ballot(bad);
+    unsigned base = 0;
+    if (lane == 0 && mask) base = atomicAdd(&words[0], (unsigned)__popcll(mask));
+    if (lane == 0 && badmask) atomicAdd(&words[1], (unsigned)__popcll(badmask));
+    base = __shfl(base, 0);
+    if (send) {
+        const unsigned below = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0));
+        uncertain[base + below] = (int)i;
+    }
+}
+
+// Labels outside [0, kc) counted in words[1].
+__global__ void k_prev_check(const int64_t* __restrict__ prev, int64_t n, int64_t kc, unsigned* __restrict__ words) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool bad = i < n && (prev[i] < 0 || prev[i] >= kc);
+    const unsigned long long mask = ballot(bad);
+    if ((threadIdx.x & 63) == 0 && mask) atomicAdd(&words[1], (unsigned)__popcll(mask));
+}
+
+// Certified rows among `samples` evenly spaced rows (words[2]); the gate that
+// decides whether a reuse pass can pay for itself.
+__global__ void k_reuse_sample(const float* __restrict__ X, int64_t n, int d, const float* __restrict__ C,
+                               int64_t kc, const int64_t* __restrict__ prev, const double* __restrict__ sep_lo,
+                               int samples, unsigned* __restrict__ words) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    bool keep = false;
+    if (t < samples) {
+        const int64_t i = (int64_t)t * (n / samples);
+        const int64_t j = prev[i];
+        if (j >= 0 && j < kc) {
+            const double r2 = canonical_sq(X + i * d, C + j * d, d);
+            keep = sep_lo[j] > 2.0 * sqrt(r2 * (1.0 + f64_interval(d))) * (1.0 + 0x1p-50);
+        }
+    }
+    const unsigned long long mask = ballot(keep);
+    if ((threadIdx.x & 63) == 0 && mask) atomicAdd(&words[2], (unsigned)__popcll(mask));
+}
+
+// Gather the uncertain rows into a compact matrix (one block per row).
+__global__ void k_gather_rows(const float* __restrict__ X, int d, const int* __restrict__ rows, int64_t count,
+                              float* __restrict__ out) {
+    const int64_t r = blockIdx.x;
+    if (r >= count) return;
+    const float* src = X + (int64_t)rows[r] * d;
+    for (int j = threadIdx.x; j < d; j += blockDim.x) out[r * d + j] = src[j];
+}
+
+__global__ void k_scatter_labels(const int* __restrict__ rows, int64_t count, const int64_t* __restrict__ lab,
+                                 const double* __restrict__ res, int64_t* __restrict__ labels,
+                                 double* __restrict__ residual) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= count) return;
+    labels[rows[r]] = lab[r];
+    residual[rows[r]] = res[r];
+}
+
 // One thread per query row: votes of its k neighbours in neighbour order.
 __global__ void k_votes(const int64_t* __restrict__ idx, const double* __restrict__ dist, int64_t m, int k,
                         const int32_t* __restrict__ codes, int n_classes, int weights, double* __restrict__ votes,
@@ -132,6 +232,104 @@ extern "C" int mlk_kmeans_assign(const float* X_dev, int64_t n, int32_t d, const
     const uint32_t qflags = (flags & MLK_PREC_MASK) | MLK_SQUARED_DIST;
     std::lock_guard<std::mutex> lk(centres->lock);
     return knn_query_impl(centres, X_dev, n, 1, 0, qflags, residual_dev, labels_dev, (hipStream_t)stream, nullptr, 0);
+}
+
+extern "C" int mlk_kmeans_assign_reuse(const float* X_dev, int64_t n, int32_t d, const float* C_dev, int64_t kc,
+                                       const int64_t* prev_dev, uint32_t flags, int64_t* labels_dev,
+                                       double* residual_dev, int64_t* reused, void* stream_) {
+    hipStream_t stream = (hipStream_t)stream_;
+    if (reused) *reused = 0;
+    if (n < 0 || d < 1 || kc < 1 || (n && (!X_dev || !labels_dev || !residual_dev || !prev_dev)) || !C_dev) {
+        set_error("kmeans_assign_reuse: invalid shapes or pointers");
+        return MLK_EINVAL;
+    }
+    if (kc >= (int64_t(1) << 31) || n >= (int64_t(1) << 31)) {
+        set_error("kmeans_assign_reuse: rows and centres must fit 32-bit indices");
+        return MLK_EINVAL;
+    }
+    if (flags & ~MLK_PREC_MASK) {
+        set_error("kmeans_assign_reuse: only precision flags are accepted");
+        return MLK_EINVAL;
+    }
+    if (n == 0) return MLK_OK;
+    mlk_index* centres = nullptr;
+    MLK_TRY(create_index(C_dev, kc, d, stream, &centres));
+    struct G {
+        mlk_index* p;
+        ~G() { (void)hipDeviceSynchronize(); delete p; }
+    } g{centres};
+    std::lock_guard<std::mutex> lk(centres->lock);
+    const uint32_t qflags = (flags & MLK_PREC_MASK) | MLK_SQUARED_DIST;
+    DevBuf sep2, sep_lo, near, unc, words, Xu, lab_u, res_u;
+    MLK_TRY(sep_lo.reserve((size_t)kc * sizeof(double)));
+    if (kc > 1) {
+        // exact nearest other centre of every centre: the same exact search
+        MLK_TRY(sep2.reserve((size_t)kc * sizeof(double)));
+        MLK_TRY(near.reserve((size_t)kc * sizeof(int64_t)));
+        MLK_TRY(knn_query_impl(centres, C_dev, kc, 1, 0, qflags | MLK_EXCLUDE_SELF, sep2.as<double>(),
+                               near.as<int64_t>(), stream, nullptr, 0));
+    }
+    hipLaunchKernelGGL(k_sep_lower, dim3(nblk(kc)), dim3(256), 0, stream, kc > 1 ? sep2.as<double>() : nullptr, kc, d,
+                       sep_lo.as<double>());
+    MLK_TRY(words.reserve(64));
+    MLK_CHECK_HIP(hipMemsetAsync(words.ptr, 0, 16, stream));
+    // Gate (as the reference's sampled test, _k_means.py:309-324): with dense
+    // centres the separation test rarely holds; then the plain assignment is
+    // cheaper than a reuse pass that certifies few rows.
+    constexpr int SAMPLES = 1024;
+    if (n >= 4 * SAMPLES) {
+        hipLaunchKernelGGL(k_reuse_sample, dim3(SAMPLES / 256), dim3(256), 0, stream, X_dev, n, d, C_dev, kc, prev_dev,
+                           sep_lo.as<double>(), SAMPLES, words.as<unsigned>());
+        MLK_CHECK_HIP(hipGetLastError());
+        unsigned w3[3] = {0, 0, 0};
+        MLK_CHECK_HIP(hipMemcpyAsync(w3, words.ptr, sizeof(w3), hipMemcpyDeviceToHost, stream));
+        MLK_CHECK_HIP(hipStreamSynchronize(stream));
+        if (w3[2] < SAMPLES * 3 / 4) {
+            // the labels are still validated: the caller's contract does not
+            // depend on the gate
+            hipLaunchKernelGGL(k_prev_check, dim3(nblk(n)), dim3(256), 0, stream, prev_dev, n, kc, words.as<unsigned>());
+            MLK_CHECK_HIP(hipGetLastError());
+            unsigned w[2] = {0, 0};
+            MLK_CHECK_HIP(hipMemcpyAsync(w, words.ptr, sizeof(w), hipMemcpyDeviceToHost, stream));
+            MLK_CHECK_HIP(hipStreamSynchronize(stream));
+            if (w[1]) {
+                set_error("kmeans_assign_reuse: " + std::to_string(w[1]) + " previous labels outside [0, n_clusters)");
+                return MLK_EINVAL;
+            }
+            MLK_TRY(knn_query_impl(centres, X_dev, n, 1, 0, qflags, residual_dev, labels_dev, stream, nullptr, 0));
+            MLK_CHECK_HIP(hipStreamSynchronize(stream));
+            return MLK_OK;
+        }
+    }
+    MLK_TRY(unc.reserve((size_t)n * sizeof(int)));
+    hipLaunchKernelGGL(k_reuse, dim3(nblk(n)), dim3(256), 0, stream, X_dev, n, d, C_dev, kc, prev_dev,
+                       sep_lo.as<double>(), labels_dev, residual_dev, unc.as<int>(), words.as<unsigned>());
+    MLK_CHECK_HIP(hipGetLastError());
+    unsigned w[2] = {0, 0};
+    MLK_CHECK_HIP(hipMemcpyAsync(w, words.ptr, sizeof(w), hipMemcpyDeviceToHost, stream));
+    MLK_CHECK_HIP(hipStreamSynchronize(stream));
+    if (w[1]) {
+        set_error("kmeans_assign_reuse: " + std::to_string(w[1]) + " previous labels outside [0, n_clusters)");
+        return MLK_EINVAL;
+    }
+    const int64_t nu = w[0];
+    if (reused) *reused = n - nu;
+    if (nu) {
+        // the rows without a certificate: the full exact assignment
+        MLK_TRY(Xu.reserve((size_t)nu * d * sizeof(float)));
+        MLK_TRY(lab_u.reserve((size_t)nu * sizeof(int64_t)));
+        MLK_TRY(res_u.reserve((size_t)nu * sizeof(double)));
+        hipLaunchKernelGGL(k_gather_rows, dim3((unsigned)nu), dim3(d < 256 ? 64 : 256), 0, stream, X_dev, d,
+                           unc.as<int>(), nu, Xu.as<float>());
+        MLK_CHECK_HIP(hipGetLastError());
+        MLK_TRY(knn_query_impl(centres, Xu.as<float>(), nu, 1, 0, qflags, res_u.as<double>(), lab_u.as<int64_t>(),
+                               stream, nullptr, 0));
+        hipLaunchKernelGGL(k_scatter_labels, dim3(nblk(nu)), dim3(256), 0, stream, unc.as<int>(), nu,
+                           lab_u.as<int64_t>(), res_u.as<double>(), labels_dev, residual_dev);
+        MLK_CHECK_HIP(hipGetLastError());
+    }
+    MLK_CHECK_HIP(hipStreamSynchronize(stream));
+    return MLK_OK;
 }
 
 extern "C" int mlk_kmeans_update(const float* X_dev, int64_t n, int32_t d, const int64_t* labels_dev, int64_t kc,

@@ -40,16 +40,33 @@ def _ptr(t):
     return ctypes.c_void_p(t.data_ptr()) if t is not None and t.numel() else ctypes.c_void_p(0)
 
 
-def assign_device(X, C, *, precision="float32"):
-    """Exact nearest centre of every row: (int64 labels, float64 squared residuals) CUDA tensors."""
+def assign_device(X, C, *, precision="float32", previous=None, out_reused=None):
+    """Exact nearest centre of every row: (int64 labels, float64 squared residuals) CUDA tensors.
+
+    With ``previous`` labels (a Lloyd iteration after the first), rows whose
+    label a certificate proves unchanged keep it without a search
+    (``reusable_labels``, reference lloyd_bounds.py:24-76); the result is the
+    same. ``out_reused`` (a list) receives the number of such rows.
+    """
     lib = require_gpu()
     torch = _torch()
     Xd, Cd = to_device_f32(X), to_device_f32(C)
     n, d = Xd.shape
     labels = torch.empty(n, dtype=torch.int64, device=Xd.device)
     residual = torch.empty(n, dtype=torch.float64, device=Xd.device)
-    _lib.check(lib.mlk_kmeans_assign(_ptr(Xd), n, d, _ptr(Cd), Cd.shape[0], precision_flag(precision), _ptr(labels),
-                                     _ptr(residual), _stream()), "mlk_kmeans_assign")
+    if previous is None:
+        _lib.check(lib.mlk_kmeans_assign(_ptr(Xd), n, d, _ptr(Cd), Cd.shape[0], precision_flag(precision),
+                                         _ptr(labels), _ptr(residual), _stream()), "mlk_kmeans_assign")
+        return labels, residual
+    prev = torch.as_tensor(previous, device=Xd.device).to(torch.int64).contiguous()
+    if prev.shape != (n,):
+        raise ValueError(f"previous labels must have shape ({n},)")
+    reused = ctypes.c_int64()
+    _lib.check(lib.mlk_kmeans_assign_reuse(_ptr(Xd), n, d, _ptr(Cd), Cd.shape[0], _ptr(prev), precision_flag(precision),
+                                           _ptr(labels), _ptr(residual), ctypes.byref(reused), _stream()),
+               "mlk_kmeans_assign_reuse")
+    if out_reused is not None:
+        out_reused.append(int(reused.value))
     return labels, residual
 
 
@@ -162,13 +179,15 @@ class KMeans(ClusterMixin, TransformerMixin, BaseEstimator):
             var = Xd.to(torch.float64).var(dim=0, unbiased=False).mean().item()
             tolerance = var * float(self.tol)
         best = None
+        reused = []  # rows per assignment whose label was certified without a search
         for restart, C0 in enumerate(initials):
             centres = to_device_f32(C0)
             previous = None
             reason = "max_iter"
             it = 0
             for it in range(int(self.max_iter)):
-                labels, residual = assign_device(Xd, centres, precision=self.precision)
+                labels, residual = assign_device(Xd, centres, precision=self.precision, previous=previous,
+                                                 out_reused=reused)
                 sums, counts, _ = update_device(Xd, labels, k, residual)
                 sums, counts, empty = self._relocate_empty(Xd, labels, residual, sums, counts)
                 new = (sums / counts.clamp(min=1).to(torch.float64)[:, None])
@@ -189,7 +208,8 @@ class KMeans(ClusterMixin, TransformerMixin, BaseEstimator):
                     reason = "tolerance"
                     break
                 previous = labels
-            labels, residual = assign_device(Xd, centres, precision=self.precision)
+            labels, residual = assign_device(Xd, centres, precision=self.precision, previous=labels,
+                                             out_reused=reused)
             _, _, inertia = update_device(Xd, labels, k, residual)
             value = float(inertia.item())
             if best is None or value < best[0]:
@@ -200,6 +220,7 @@ class KMeans(ClusterMixin, TransformerMixin, BaseEstimator):
         self.inertia_ = value
         self.n_iter_ = n_iter
         self.convergence_reason_ = reason
+        self.reused_labels_ = int(sum(reused))
         self.n_features_in_ = d
         self._n_features_out = k
         distinct = np.unique(self.labels_).size

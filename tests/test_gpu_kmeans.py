@@ -132,3 +132,96 @@ def test_kmeans_converges_on_separated_blobs(mlk):
     np.testing.assert_array_equal(km.labels_, l)
     assert math.isclose(km.inertia_, math.fsum(res), rel_tol=1e-12)
     assert km.convergence_reason_ in ("labels", "tolerance")
+
+
+def _blobs(seed, n, d, k, scale=10.0):
+    r = np.random.default_rng(seed)
+    centres = r.normal(scale=scale, size=(k, d))
+    X = (centres[r.integers(0, k, n)] + r.normal(size=(n, d))).astype(np.float32)
+    return X, (centres + r.normal(scale=0.5, size=centres.shape)).astype(np.float32)
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_label_reuse_equals_exact_assignment(mlk, precision):
+    """Certified reuse of the previous labels (reference lloyd_bounds.py:24-76): the
+    labels and residuals are those of a full exact assignment, bit for bit."""
+    torch = pytest.importorskip("torch")
+    from mlxlearn_amd.cluster._k_means import assign_device
+    X, C0 = _blobs(21, 40_000, 32, 64)
+    l0, _ = oracle.kmeans_assign(X, C0)
+    sums, counts = oracle.kmeans_update(X, l0, 64)
+    C1 = np.where(counts[:, None] > 0, sums / np.maximum(counts, 1)[:, None], C0).astype(np.float32)
+    reused = []
+    labels, resid = assign_device(X, C1, precision=precision, previous=torch.as_tensor(l0).cuda(),
+                                  out_reused=reused)
+    want_l, want_r = oracle.kmeans_assign(X, C1)
+    np.testing.assert_array_equal(labels.cpu().numpy(), want_l)
+    np.testing.assert_allclose(resid.cpu().numpy(), want_r, rtol=1e-12, atol=0)
+    full_l, full_r = assign_device(X, C1, precision=precision)
+    assert torch.equal(labels, full_l) and torch.equal(resid, full_r)
+    assert reused[0] > len(X) // 2, reused  # separated blobs: most rows keep their label
+    # previous labels that are all wrong: nothing is certified wrongly
+    bad_prev = (torch.as_tensor(want_l).cuda() + 1) % 64
+    labels, resid = assign_device(X, C1, precision=precision, previous=bad_prev)
+    np.testing.assert_array_equal(labels.cpu().numpy(), want_l)
+    assert torch.equal(resid, full_r)
+    # dense centres (the separation test fails on the sample): the plain assignment
+    Xd, _ = oracle.gen_c5(seed=21, n=20_000, d=32, n_components=16, n_clusters=16)
+    Cd = Xd[np.random.default_rng(2).choice(len(Xd), 512, replace=False)].copy()
+    want_l, want_r = oracle.kmeans_assign(Xd, Cd)
+    reused = []
+    labels, resid = assign_device(Xd, Cd, precision=precision, previous=torch.as_tensor(want_l).cuda(),
+                                  out_reused=reused)
+    np.testing.assert_array_equal(labels.cpu().numpy(), want_l)
+    np.testing.assert_allclose(resid.cpu().numpy(), want_r, rtol=1e-12, atol=0)
+
+
+def test_label_reuse_edge_cases(mlk):
+    import ctypes
+    torch = pytest.importorskip("torch")
+    from mlxlearn_amd import _lib
+    from mlxlearn_amd.cluster._k_means import assign_device
+    X, C = _blobs(5, 6000, 12, 20)
+    # duplicate centres: separation 0, never certified; the lowest index wins
+    C[7] = C[3]
+    C[15] = C[3]
+    want_l, want_r = oracle.kmeans_assign(X, C)
+    prev = torch.as_tensor(want_l).cuda()
+    prev[torch.as_tensor(want_l == 3).cuda()] = 15
+    reused = []
+    labels, resid = assign_device(X, C, previous=prev, out_reused=reused)
+    np.testing.assert_array_equal(labels.cpu().numpy(), want_l)
+    np.testing.assert_allclose(resid.cpu().numpy(), want_r, rtol=1e-12, atol=0)
+    assert 0 < reused[0] < 6000
+    # one centre: every row keeps label 0
+    reused = []
+    labels, _ = assign_device(X, C[:1], previous=torch.zeros(6000, dtype=torch.int64).cuda(), out_reused=reused)
+    assert reused == [6000] and int(labels.abs().sum().item()) == 0
+    # labels outside [0, n_clusters) are an input error, whether or not the gate passes
+    for Cx in (C, np.zeros_like(C)):
+        bad = prev.clone()
+        bad[11] = 20
+        with pytest.raises(ValueError, match="outside"):
+            assign_device(X, Cx, previous=bad)
+    # in place: labels written over the previous labels
+    Xd, Cd = torch.as_tensor(X).cuda(), torch.as_tensor(C).cuda()
+    inplace = torch.as_tensor(want_l).cuda()
+    resid = torch.empty(6000, dtype=torch.float64).cuda()
+    n_reused = ctypes.c_int64()
+    lib = _lib.load()
+    s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    _lib.check(lib.mlk_kmeans_assign_reuse(ctypes.c_void_p(Xd.data_ptr()), 6000, 12, ctypes.c_void_p(Cd.data_ptr()), 20,
+                                           ctypes.c_void_p(inplace.data_ptr()), 0, ctypes.c_void_p(inplace.data_ptr()),
+                                           ctypes.c_void_p(resid.data_ptr()), ctypes.byref(n_reused), s))
+    np.testing.assert_array_equal(inplace.cpu().numpy(), want_l)
+    assert 0 < n_reused.value < 6000
+
+
+def test_kmeans_fit_reuses_labels(mlk):
+    """The Lloyd loop reuses certified labels after its first iteration; results stay exact."""
+    X, _ = _blobs(4, 20_000, 16, 24)
+    km = mlk.KMeans(n_clusters=24, init="k-means++", random_state=3, max_iter=20, tol=0).fit(X)
+    l, res = oracle.kmeans_assign(X, km.cluster_centers_)
+    np.testing.assert_array_equal(km.labels_, l)
+    assert math.isclose(km.inertia_, math.fsum(res), rel_tol=1e-12)
+    assert km.n_iter_ > 1 and km.reused_labels_ > 0
