@@ -153,7 +153,7 @@ def main():
         first = bench_workloads.cold(lambda: run(precision))
         if args.autotune:
             # plan search on these queries: untimed preparation, like fit
-            tuned[precision] = bench_workloads.tune(index, Qd, k, precision=precision, max_trials=16)
+            tuned[precision] = bench_workloads.tune(index, Qd, k, precision=precision, max_trials=24)
         screen_ms = []
 
         def step():

@@ -164,8 +164,9 @@ void clear_plans_impl() {
 }
 
 // Coordinate descent over the plan: screen shape, then database splits (a
-// line search by factors of two each way), then the round schedule, then (if
-// queries fell back) list capacity. A plan replaces the best one only when it
+// line search by factors of two each way, then counts that fill whole waves
+// of resident blocks), then the round schedule, then (if queries fell back)
+// list capacity. A plan replaces the best one only when it
 // is at least 2% faster; each plan is timed as the best of two calls, after
 // one untimed call that builds the mirrors and workspace.
 int autotune_impl(mlk_index* index, const float* Q, int64_t m, int k, double R2, uint32_t flags, int max_trials,
@@ -274,6 +275,33 @@ int autotune_impl(mlk_index* index, const float* Q, int64_t m, int k, double R2,
             const int r = consider(p);
             MLK_TRY(check(r));
             if (r != 1) break;
+        }
+    }
+    // 2b. splits that fill whole waves of resident blocks: the last wave of a
+    // launch otherwise runs part-empty (its tail)
+    {
+        int dev = 0, cus = 0, per_cu = 0;
+        MLK_CHECK_HIP(hipGetDevice(&dev));
+        MLK_CHECK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        ScreenFn fn = pick_screen(mode, prec, dp, best.shape);
+        if (fn && hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(fn), 256, 0) ==
+                      hipSuccess && per_cu > 0 && cus > 0) {
+            const int64_t resident = (int64_t)cus * per_cu;
+            const int64_t BQ = 4 * best.shape.qt * 32;
+            const int64_t qb = std::max<int64_t>(1, (m + BQ - 1) / BQ);
+            const double waves = (double)qb * best.splits / (double)resident;
+            const Plan around = best;
+            for (int64_t j : {(int64_t)std::floor(waves), (int64_t)std::ceil(waves), (int64_t)std::ceil(waves) + 1}) {
+                if (j < 1) continue;
+                const int64_t fit = j * resident / qb;  // blocks just within j waves
+                for (int64_t sp : {fit, fit / 8 * 8}) {
+                    if (sp < 1 || sp > max_splits || sp == around.splits) continue;
+                    Plan p = around;
+                    p.splits = (int)sp;
+                    if (kind == PLAN_KNN) p.cap = knn_capacity(prec, p.splits, k);
+                    MLK_TRY(check(consider(p)));
+                }
+            }
         }
     }
     if (kind == PLAN_KNN) {
