@@ -93,6 +93,21 @@ def copy_calibration(nbytes):
     print(json.dumps({"copy_bytes": src.numel() * 4, "launches": 3}), flush=True)
 
 
+def mfma_calibration(blocks=1024, iterations=20000):
+    """The library's register-only bf16 MFMA loop (4 waves per SIMD): the
+    attainable MFMA rate at the current clock, and the reference point of the
+    MFMA busy-cycle counters of a rocprofv3 --pmc run around it."""
+    import ctypes
+    import torch
+    from mlxlearn_amd import _lib
+    lib = _lib.load()
+    out = ctypes.c_double()
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    _lib.check(lib.mlk_mfma_probe(blocks, iterations, ctypes.byref(out), stream), "mlk_mfma_probe")
+    print(json.dumps({"mfma_probe_tflops": round(out.value, 1), "blocks": blocks, "iterations": iterations,
+                      "frac_of_2500": round(out.value / PEAK_BF16_TFLOPS, 4)}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,11 +124,18 @@ def main():
                     help="c2 (default) is the headline; c1, c3, c4, c5 are the other configurations")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="time the heuristic plans instead of autotuned ones")
+    ap.add_argument("--mfma-calibration", action="store_true",
+                    help="only run the library's register-only MFMA probe (MFMA counter calibration) and exit")
+    ap.add_argument("--no-iter2", dest="iter2", action="store_false",
+                    help="c5: skip the second Lloyd iteration measured after the timed steps")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (HBM counter calibration) and exit")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.mfma_calibration:
+        mfma_calibration()
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:

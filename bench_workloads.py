@@ -318,6 +318,8 @@ def run_c5(args):
     # a Lloyd iteration after the first: the updated centres, assigned with
     # certified reuse of the labels above (lloyd_bounds.reusable_labels) and
     # without it; both give the same labels
+    if not args.iter2:
+        return finish_c5(args, line, X, C, kc)
     sums, counts, _ = update_device(Xd, labels, kc, resid)
     C1 = torch.where((counts > 0)[:, None], sums / counts.clamp(min=1).to(torch.float64)[:, None],
                      Cd.to(torch.float64)).to(torch.float32).contiguous()
@@ -329,6 +331,11 @@ def run_c5(args):
                                  "reused_rows": reused[-1], "reused_fraction": round(reused[-1] / n, 4),
                                  "labels_equal": bool(torch.equal(l_reuse, l_full))}
     log(f"c5 iteration 2: {line['lloyd_iteration_2']}")
+    return finish_c5(args, line, X, C, kc)
+
+
+def finish_c5(args, line, X, C, kc):
+    import oracle
     if not args.no_cpu:
         nr = 256
         t0 = time.perf_counter()

@@ -213,6 +213,15 @@ int mlk_knn_votes(const int64_t* idx_dev, const double* dist_dev, int64_t m, int
                   const int32_t* codes_dev, int32_t n_classes, int32_t weights, int64_t* pred_dev,
                   double* proba_dev, void* stream);
 
+/*
+ * Diagnostics, no counterpart in the reference: runs `blocks` blocks of a
+ * register-only bf16 MFMA loop (4 waves x 4 independent 32x32x16 MFMA chains,
+ * `iterations` steps) twice and writes the second run's rate to *tflops. It
+ * measures the attainable dense MFMA rate at the device's current clock and
+ * is the reference point for MFMA busy-cycle counters.
+ */
+int mlk_mfma_probe(int32_t blocks, int32_t iterations, double* tflops, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -77,6 +77,7 @@ SIGNATURES = {
     "mlk_autotune": ([_vp, _vp, _i64, _i32, _dbl, _u32, _i32, ctypes.POINTER(Plan), _vp], ctypes.c_int),
     "mlk_get_plan": ([_vp, _i64, _i32, _u32, ctypes.POINTER(Plan)], ctypes.c_int),
     "mlk_clear_plans": ([], ctypes.c_int),
+    "mlk_mfma_probe": ([_i32, _i32, ctypes.POINTER(_dbl), _vp], ctypes.c_int),
 }
 
 _LIB = None

@@ -133,3 +133,17 @@ def test_kmeans_assignment_uses_tuned_plan(mlk):
     np.testing.assert_array_equal(labels.cpu().numpy(), want_l)
     np.testing.assert_allclose(resid.cpu().numpy(), want_r, rtol=1e-12, atol=0)
     _lib.clear_plans()
+
+
+def test_mfma_probe_reports_a_plausible_rate(mlk):
+    """Diagnostics: the register-only MFMA loop runs and reports a rate below the dense bf16 peak."""
+    import ctypes
+    torch = pytest.importorskip("torch")
+    from mlxlearn_amd import _lib
+    lib = _lib.load()
+    out = ctypes.c_double()
+    s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    _lib.check(lib.mlk_mfma_probe(256, 2000, ctypes.byref(out), s))
+    assert 50.0 < out.value < 2600.0
+    with pytest.raises(ValueError):
+        _lib.check(lib.mlk_mfma_probe(0, 10, ctypes.byref(out), s))
