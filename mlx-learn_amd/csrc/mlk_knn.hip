@@ -74,91 +74,124 @@ __device__ __forceinline__ double fast_sq(const float* __restrict__ q, const flo
     return (s0 + s1) + (s2 + s3);
 }
 
-// One wave per query: the k-th smallest certified upper bound over up to 256
-// evenly spaced rows is an upper bound of the query's k-th smallest S. Sixteen
-// lanes share a sample row (coalesced 64-byte segments), four rows per step;
-// the float64 partial sums may be added in any order (bound g).
-__global__ void k_tau0(const float* __restrict__ X, int64_t n, int d, const float* __restrict__ Q, int64_t m,
-                       int64_t m_pad, int k, int max_samples, int64_t self_off, int exclude, const double* __restrict__ nq,
-                       const uint8_t* __restrict__ qflags, Consts c, float* __restrict__ tau,
-                       float* __restrict__ thr, int* __restrict__ status, int S2, int* __restrict__ cnt,
-                       int* __restrict__ seen, float* __restrict__ kbest) {
-    __shared__ uint32_t sb[4][TAU0_SAMPLES];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int64_t q = (int64_t)blockIdx.x * 4 + w;
-    if (q >= m_pad) return;
-    // the query's candidate sub-list counters start empty and its k best
-    // upper bounds at +inf (screened calls only)
+// Start threshold: for each query, the k-th smallest certified upper bound of
+// S over up to 256 evenly spaced rows is an upper bound of its k-th smallest
+// S. The block stages the sample rows in LDS, 32 features at a time, with
+// coalesced loads (32 lanes per 128-byte row segment); lane j of each wave
+// owns samples j + 64u (u < U) and takes their features into registers eight
+// at a time while the wave walks its QW queries (wave-uniform query loads), so
+// each staged value serves 4 * QW queries. float64 differences and fused
+// squares-and-adds: any evaluation order is covered by the relative bound
+// g = f64_interval(d). Each wave also initialises its queries' candidate
+// sub-list counters and k-best bounds (screened calls only).
+template <int U, int QW>
+__global__ __launch_bounds__(256) void k_tau0(const float* __restrict__ X, int64_t n, int d,
+                                              const float* __restrict__ Q, int64_t m, int64_t m_pad, int k,
+                                              int max_samples, int64_t self_off, int exclude,
+                                              const double* __restrict__ nq, const uint8_t* __restrict__ qflags,
+                                              Consts c, float* __restrict__ tau, float* __restrict__ thr,
+                                              int* __restrict__ status, int S2, int* __restrict__ cnt,
+                                              int* __restrict__ seen, float* __restrict__ kbest) {
+    constexpr int FL = 32;      // features staged per pass
+    constexpr int FC = 8;       // features per register chunk
+    constexpr int NS = 64 * U;  // sample slots of the block
+    __shared__ float sx[NS][FL + 1];
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int64_t q0 = ((int64_t)blockIdx.x * 4 + w) * QW;
+    // queries of this wave: padding included (nqw), real ones (live)
+    const int nqw = q0 >= m_pad ? 0 : (m_pad - q0 < QW ? (int)(m_pad - q0) : QW);
+    const int live = q0 >= m ? 0 : (m - q0 < nqw ? (int)(m - q0) : nqw);
     if (cnt) {
-        for (int j = lane; j < S2; j += 64) {
-            cnt[q * S2 + j] = 0;
-            seen[q * S2 + j] = 0;
+        for (int qi = 0; qi < nqw; ++qi) {
+            const int64_t q = q0 + qi;
+            for (int j = lane; j < S2; j += 64) {
+                cnt[q * S2 + j] = 0;
+                seen[q * S2 + j] = 0;
+            }
+            for (int j = lane; j < k; j += 64) kbest[q * k + j] = INFINITY;
         }
-        for (int j = lane; j < k; j += 64) kbest[q * k + j] = INFINITY;
     }
-    if (q >= m) {
-        if (lane == 0) { tau[q] = -1.0f; thr[q] = INFINITY; status[q] = 0; }
-        return;
-    }
-    const int samples = n < max_samples ? (int)n : max_samples;
-    const int span = (samples + 15) / 16 * 16;  // entries past `samples` hold +inf
-    const double g = f64_interval(d);
-    const float* qv = Q + q * d;
-    const int gl = lane & 15;
+    const int samples = n < max_samples ? (int)n : max_samples;  // <= NS
     const int64_t stride = n / samples;  // sample j is row j * stride
-    // four sample rows per 16-lane group and pass; every load of a
-    // 128-feature slice is issued before the arithmetic
-    for (int j0 = 0; j0 < span; j0 += 16) {
-        int64_t row[4];
-        const float* xp[4];
+    double acc[QW][U];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int j = j0 + (lane >> 4) + 4 * u;
-            row[u] = j < samples ? (int64_t)j * stride : -1;
-            xp[u] = X + (row[u] < 0 ? 0 : row[u]) * d;
+    for (int qi = 0; qi < QW; ++qi)
+#pragma unroll
+        for (int u = 0; u < U; ++u) acc[qi][u] = 0.0;
+    for (int f0 = 0; f0 < d; f0 += FL) {
+        __syncthreads();
+        // stage features [f0, f0 + FL) of every sample row (zeros past d / samples)
+        for (int idx = threadIdx.x; idx < NS * FL; idx += 256) {
+            const int j = idx / FL, e = idx % FL;
+            float v = 0.0f;
+            if (j < samples && f0 + e < d) v = X[(int64_t)j * stride * d + f0 + e];
+            sx[j][e] = v;
         }
-        double part[4] = {0.0, 0.0, 0.0, 0.0};
-        for (int e0 = 0; e0 < d; e0 += 128) {
-            // unconditional loads from clamped addresses (issued together)
-            float qq[8], xx[4][8];
+        __syncthreads();
+        const int fl = d - f0 < FL ? d - f0 : FL;
+        for (int c0 = 0; c0 < fl; c0 += FC) {
+            double sv[U][FC];
 #pragma unroll
-            for (int t = 0; t < 8; ++t) {
-                const int e = e0 + gl + 16 * t;
-                const int ec = e < d ? e : d - 1;
-                qq[t] = qv[ec];
+            for (int u = 0; u < U; ++u)
 #pragma unroll
-                for (int u = 0; u < 4; ++u) xx[u][t] = xp[u][ec];
-            }
+                for (int e = 0; e < FC; ++e) sv[u][e] = (double)sx[lane + 64 * u][c0 + e];
+            const int fe = fl - c0 < FC ? fl - c0 : FC;
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
+            for (int qi = 0; qi < QW; ++qi) {
+                if (qi < live) {
+                    const float* qv = Q + (q0 + qi) * d + f0 + c0;
 #pragma unroll
-                for (int t = 0; t < 8; ++t) {
-                    const double dt = e0 + gl + 16 * t < d ? (double)qq[t] - (double)xx[u][t] : 0.0;
-                    part[u] += dt * dt;
+                    for (int e = 0; e < FC; ++e) {
+                        if (e < fe) {
+                            const double dq = (double)qv[e];
+#pragma unroll
+                            for (int u = 0; u < U; ++u) {
+                                const double t = dq - sv[u][e];
+                                acc[qi][u] = fma(t, t, acc[qi][u]);
+                            }
+                        }
+                    }
                 }
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            for (int o = 8; o > 0; o >>= 1) part[u] += __shfl_xor(part[u], o, 16);
-            const int j = j0 + (lane >> 4) + 4 * u;
-            if (gl == 0) {
-                uint32_t bits = 0x7f800000u;
-                if (row[u] >= 0 && !(exclude && row[u] == self_off + q))
-                    bits = __float_as_uint(__double2float_ru(part[u] * (1.0 + g) * c.scale2));
-                sb[w][j] = bits;
             }
         }
     }
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    const uint32_t* mine = sb[w];
-    const uint32_t kb = wave_kth_bits(span, k, [&](int j) { return mine[j]; });
-    const float t0 = __uint_as_float(kb);
-    if (lane == 0) {
-        const int st = qflags[q] ? 0 : ST_UNSAFE;
-        status[q] = st;
-        tau[q] = t0;
-        thr[q] = st ? INFINITY : thr_from_tau((double)t0, nq[q], c);
+    const double g = f64_interval(d);
+#pragma unroll
+    for (int qi = 0; qi < QW; ++qi) {
+        const int64_t q = q0 + qi;
+        if (qi >= live) {  // padding query (or past m_pad)
+            if (lane == 0 && qi < nqw) { tau[q] = -1.0f; thr[q] = INFINITY; status[q] = 0; }
+            continue;
+        }
+        uint32_t bits[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int j = lane + 64 * u;
+            const int64_t row = (int64_t)j * stride;
+            bits[u] = 0x7f800000u;
+            if (j < samples && !(exclude && row == self_off + q))
+                bits[u] = __float_as_uint(__double2float_ru(acc[qi][u] * (1.0 + g) * c.scale2));
+        }
+        uint32_t kb;
+        if (k == 1) {  // the minimum (non-negative float bit patterns order as integers)
+            kb = bits[0];
+#pragma unroll
+            for (int u = 1; u < U; ++u) kb = kb < bits[u] ? kb : bits[u];
+            for (int o = 32; o > 0; o >>= 1) {
+                const uint32_t other = (uint32_t)__shfl_xor((int)kb, o);
+                kb = kb < other ? kb : other;
+            }
+        } else {
+            kb = wave_kth_bits(64 * U, k, [&](int j) { return bits[j >> 6]; });
+        }
+        if (lane == 0) {
+            const float t0 = __uint_as_float(kb);
+            const int st = qflags[q] ? 0 : ST_UNSAFE;
+            status[q] = st;
+            tau[q] = t0;
+            thr[q] = st ? INFINITY : thr_from_tau((double)t0, nq[q], c);
+        }
     }
 }
 
@@ -640,6 +673,67 @@ __global__ __launch_bounds__(256) void k_rank(const RankArgs a, int64_t n_items)
     }
 }
 
+// Rank for k = 1, one lane per query (many queries with short candidate
+// lists, as k-means assignment): the lane streams its query's sub-lists, keeps
+// candidates whose certified lower bound does not exceed tau, computes their
+// float64 distances (order-free bound g) and keeps the minimum under the exact
+// (S, index) order; the result's distance is the canonical sum. Statuses and
+// statistics are those of k_rank (no database row is unsafe on this route).
+__global__ __launch_bounds__(256) void k_rank1_lane(const RankArgs a, int64_t m) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= m) return;
+    if (a.status[q]) {  // handled by the fallback
+        if (a.qstats) a.qstats[3 * q] = a.qstats[3 * q + 1] = a.qstats[3 * q + 2] = 0;
+        return;
+    }
+    const int d = a.d;
+    const double g = f64_interval(d);
+    const float* qv = a.Q + q * d;
+    const float tau = a.tau[q];
+    const double nqq = a.nq[q];
+    const double cut = (double)tau / a.c.scale2;
+    int listed = 0, count = 0, kept = 0;
+    bool over = false;
+    double best_s = INFINITY;
+    long long best_i = -1;
+    for (int sl = 0; sl < a.S; ++sl) {
+        int cs = a.cnt[q * a.S + sl];
+        over |= cs > a.cap;
+        cs = cs < a.cap ? cs : a.cap;
+        listed += cs;
+        const int2* lst = a.cand + ((size_t)q * a.S + sl) * a.cap;
+        for (int j = 0; j < cs; ++j) {
+            const int2 e = lst[j];
+            double L, U;
+            hit_bounds(__int_as_float(e.y), a.nx32[e.x], nqq, a.c, L, U);
+            if (!(L <= (double)tau)) continue;
+            ++count;
+            const double sj = fast_sq(qv, a.X + (size_t)e.x * d, d);
+            if (!(sj * (1.0 - g) <= cut)) continue;
+            ++kept;
+            if (best_i < 0 || exact_precedes(sj, e.x, best_s, best_i, g, qv, a.X, d)) {
+                best_s = sj;
+                best_i = e.x;
+            }
+        }
+    }
+    if (over || kept < 1) {
+        a.status[q] = ST_OVERFLOW;
+        if (a.qstats) a.qstats[3 * q] = a.qstats[3 * q + 1] = a.qstats[3 * q + 2] = 0;
+        return;
+    }
+    if (a.qstats) {
+        a.qstats[3 * q] = listed;
+        a.qstats[3 * q + 1] = count;
+        a.qstats[3 * q + 2] = kept;
+    }
+    a.out_idx[q] = best_i;
+    if (a.out_dist) {
+        const double sc = canonical_sq(qv, a.X + (size_t)best_i * d, d);
+        a.out_dist[q] = a.squared ? sc : sqrt(sc);
+    }
+}
+
 // ---------------------------------------------------------------- fallback
 __global__ void k_fb_dist(const float* __restrict__ X, int64_t n, int d, const float* __restrict__ Q,
                           const int* __restrict__ fq, int64_t self_off, int exclude, double* __restrict__ s_out,
@@ -853,10 +947,44 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
         MLK_TRY(ws[17].reserve((size_t)m_pad * S2 * sizeof(int)));         // merged counts
         MLK_TRY(ws[18].reserve((size_t)m_pad * k * sizeof(float)));        // k smallest upper bounds
     }
-    hipLaunchKernelGGL(k_tau0, dim3((unsigned)((m_pad + 3) / 4)), dim3(256), 0, stream, X, n, d, Q, m, m_pad, k,
-                       tau_samples, q_offset, (int)exclude, ws[1].as<double>(), ws[2].as<uint8_t>(), c, tau, thr,
-                       status, S2, all_fallback ? nullptr : ws[8].as<int>(), ws[17].as<int>(),
-                       ws[18].as<float>());
+    {
+        // U sample rows per lane (64 U samples), QW queries per wave: U * QW = 32
+        // float64 accumulators per lane
+        const int samples = (int)std::min<int64_t>(n, tau_samples);
+        auto launch = [&](auto kern, int qw) {
+            const int64_t waves = (m_pad + qw - 1) / qw;
+            hipLaunchKernelGGL(kern, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, stream, X, n, d, Q, m, m_pad,
+                               k, tau_samples, q_offset, (int)exclude, ws[1].as<double>(), ws[2].as<uint8_t>(), c,
+                               tau, thr, status, S2, all_fallback ? nullptr : ws[8].as<int>(), ws[17].as<int>(),
+                               ws[18].as<float>());
+        };
+        // the most queries per wave (register reuse of the samples) that still
+        // leaves about 4096 waves (4 per SIMD) to fill the device
+        auto qw_for = [&](int most) {
+            int qw = most;
+            while (qw > 1 && (m_pad + qw - 1) / qw < 4096) qw >>= 1;
+            return qw;
+        };
+        if (samples <= 64) {
+            const int qw = qw_for(32);
+            if (qw == 32) launch(k_tau0<1, 32>, 32);
+            else if (qw >= 8) launch(k_tau0<1, 8>, 8);
+            else if (qw >= 2) launch(k_tau0<1, 2>, 2);
+            else launch(k_tau0<1, 1>, 1);
+        } else if (samples <= 128) {
+            const int qw = qw_for(16);
+            if (qw == 16) launch(k_tau0<2, 16>, 16);
+            else if (qw >= 4) launch(k_tau0<2, 4>, 4);
+            else if (qw >= 2) launch(k_tau0<2, 2>, 2);
+            else launch(k_tau0<2, 1>, 1);
+        } else {
+            const int qw = qw_for(8);
+            if (qw == 8) launch(k_tau0<4, 8>, 8);
+            else if (qw >= 4) launch(k_tau0<4, 4>, 4);
+            else if (qw >= 2) launch(k_tau0<4, 2>, 2);
+            else launch(k_tau0<4, 1>, 1);
+        }
+    }
     MLK_CHECK_HIP(hipGetLastError());
 
     if ((hooks().hooks & MLK_HOOK_FORCE_UNCERTIFIED) && !hooks().forced_rows.empty()) {
@@ -939,7 +1067,10 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
         ra.out_idx = idx; ra.out_dist = dist; ra.squared = (flags & MLK_SQUARED_DIST) != 0;
         MLK_TRY(ws[19].reserve((size_t)m * 3 * sizeof(int)));
         ra.qstats = ws[19].as<int>();  // written for every query by k_rank
-        hipLaunchKernelGGL(k_rank, dim3((unsigned)((m + 3) / 4)), dim3(256), 0, stream, ra, m);
+        if (k == 1 && mr.n_unsafe == 0)
+            hipLaunchKernelGGL(k_rank1_lane, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, stream, ra, m);
+        else
+            hipLaunchKernelGGL(k_rank, dim3((unsigned)((m + 3) / 4)), dim3(256), 0, stream, ra, m);
         MLK_CHECK_HIP(hipGetLastError());
         hipLaunchKernelGGL(k_sum_qstats, dim3((unsigned)std::min<int64_t>(1024, (m + 255) / 256)), dim3(256), 0,
                            stream, ra.qstats, m, (unsigned long long*)(dflag + 4));
