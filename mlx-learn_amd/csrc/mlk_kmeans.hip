@@ -100,6 +100,7 @@ __global__ void k_sep_lower(const double* __restrict__ sep2, int64_t kc, int d, 
 // One thread per row: canonical residual to the previous centre and the
 // certificate; uncertain rows (and labels outside [0, kc), counted in
 // words[1]) are appended to `uncertain`, one atomic per wave.
+template <bool VEC>
 __global__ void k_reuse(const float* __restrict__ X, int64_t n, int d, const float* __restrict__ C, int64_t kc,
                         const int64_t* __restrict__ prev, const double* __restrict__ sep_lo,
                         int64_t* __restrict__ labels, double* __restrict__ residual, int* __restrict__ uncertain,
@@ -111,7 +112,7 @@ __global__ void k_reuse(const float* __restrict__ X, int64_t n, int d, const flo
         const int64_t j = prev[i];
         bad = j < 0 || j >= kc;
         if (!bad) {
-            const double r2 = canonical_sq(X + i * d, C + j * d, d);
+            const double r2 = VEC ? canonical_sq4(X + i * d, C + j * d, d) : canonical_sq(X + i * d, C + j * d, d);
             const double r_hi = sqrt(r2 * (1.0 + f64_interval(d))) * (1.0 + 0x1p-50);
             keep = sep_lo[j] > 2.0 * r_hi;
             if (keep) {
@@ -302,7 +303,9 @@ extern "C" int mlk_kmeans_assign_reuse(const float* X_dev, int64_t n, int32_t d,
         }
     }
     MLK_TRY(unc.reserve((size_t)n * sizeof(int)));
-    hipLaunchKernelGGL(k_reuse, dim3(nblk(n)), dim3(256), 0, stream, X_dev, n, d, C_dev, kc, prev_dev,
+    const bool vec = d % 4 == 0 && ((uintptr_t)X_dev % 16) == 0 && ((uintptr_t)C_dev % 16) == 0;
+    hipLaunchKernelGGL(vec ? k_reuse<true> : k_reuse<false>, dim3(nblk(n)), dim3(256), 0, stream, X_dev, n, d, C_dev,
+                       kc, prev_dev,
                        sep_lo.as<double>(), labels_dev, residual_dev, unc.as<int>(), words.as<unsigned>());
     MLK_CHECK_HIP(hipGetLastError());
     unsigned w[2] = {0, 0};

@@ -101,6 +101,8 @@ __global__ void k_pair_keys(const int2* __restrict__ pairs, int64_t np, unsigned
     if (i < np) keys[i] = ((unsigned long long)(unsigned)pairs[i].x << 32) | (unsigned)pairs[i].y;
 }
 
+// VEC: d % 4 == 0 and 16-byte aligned rows (four features per load).
+template <bool VEC>
 __global__ void k_classify(const unsigned long long* __restrict__ keys, int64_t np, const float* __restrict__ X,
                            const float* __restrict__ Q, int d, RadiusConsts rc, uint8_t* __restrict__ member,
                            double* __restrict__ dist) {
@@ -111,7 +113,7 @@ __global__ void k_classify(const unsigned long long* __restrict__ keys, int64_t 
     const int64_t row = (int64_t)(key & 0xffffffffull);
     const float* qv = Q + q * d;
     const float* xv = X + row * d;
-    const double s = canonical_sq(qv, xv, d);
+    const double s = VEC ? canonical_sq4(qv, xv, d) : canonical_sq(qv, xv, d);
     bool in;
     if (s <= rc.f_in) in = true;
     else if (!(s <= rc.f_out)) in = false;
@@ -364,8 +366,9 @@ int radius_query_impl(mlk_index* index, const float* Q, int64_t m, double R2, in
         MLK_CHECK_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp_bytes, keys, skeys, (int)np, 0, 32 + qbits, stream));
         MLK_TRY(ws[14].reserve(tmp_bytes + 16));
         MLK_CHECK_HIP(hipcub::DeviceRadixSort::SortKeys(ws[14].ptr, tmp_bytes, keys, skeys, (int)np, 0, 32 + qbits, stream));
-        hipLaunchKernelGGL(k_classify, dim3(blocks_for(np)), dim3(256), 0, stream, skeys, np, X, Q, d, rc,
-                           ws[13].as<uint8_t>(), ws[12].as<double>());
+        const bool vec = d % 4 == 0 && ((uintptr_t)Q % 16) == 0 && ((uintptr_t)X % 16) == 0;
+        hipLaunchKernelGGL(vec ? k_classify<true> : k_classify<false>, dim3(blocks_for(np)), dim3(256), 0, stream,
+                           skeys, np, X, Q, d, rc, ws[13].as<uint8_t>(), ws[12].as<double>());
         MLK_CHECK_HIP(hipGetLastError());
     }
     // compact members (order preserved)

@@ -225,3 +225,20 @@ def test_kmeans_fit_reuses_labels(mlk):
     np.testing.assert_array_equal(km.labels_, l)
     assert math.isclose(km.inertia_, math.fsum(res), rel_tol=1e-12)
     assert km.n_iter_ > 1 and km.reused_labels_ > 0
+
+
+def test_misaligned_rows_assignment(mlk, rng):
+    """Rows one float into a buffer (not 16-byte aligned): same labels and residuals."""
+    torch = pytest.importorskip("torch")
+    from mlxlearn_amd.cluster._k_means import assign_device
+    X = rng.normal(size=(9000, 96)).astype(np.float32)
+    C = X[:300] + np.float32(0.05)
+    want_l, want_r = oracle.kmeans_assign(X, C)
+    buf = torch.zeros(X.size + 1, dtype=torch.float32, device="cuda")
+    Xm = buf[1:].view(X.shape)
+    Xm.copy_(torch.as_tensor(X))
+    assert Xm.data_ptr() % 16 != 0
+    for prec in PRECISIONS:
+        labels, resid = assign_device(Xm, C, precision=prec)
+        np.testing.assert_array_equal(labels.cpu().numpy(), want_l)
+        np.testing.assert_allclose(resid.cpu().numpy(), want_r, rtol=1e-12, atol=0)

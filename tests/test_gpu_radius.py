@@ -145,3 +145,24 @@ def test_wide_rows(mlk, d):
     R2 = float(radius_cases.typical_radius(X, Q, 12)) ** 2
     for p in PRECISIONS:
         assert_csr(gpu_radius(mlk, X, Q, R2, p, sort_results=True), oracle.radius(X, Q, R2, sort_results=True))
+
+
+def test_misaligned_queries_take_the_scalar_path(mlk, rng):
+    """Query rows that are not 16-byte aligned (a view one float into a buffer) give the
+    same CSR as aligned ones: the four-wide reads are used only on aligned rows."""
+    torch = pytest.importorskip("torch")
+    X = rng.normal(size=(6000, 32)).astype(np.float32)
+    Q = rng.normal(size=(150, 32)).astype(np.float32)
+    R2 = float(radius_cases.typical_radius(X, Q, 20)) ** 2
+    want = oracle.radius(X, Q, R2, sort_results=True)
+    buf = torch.zeros(Q.size + 1, dtype=torch.float32, device="cuda")
+    Qm = buf[1:].view(Q.shape)
+    Qm.copy_(torch.as_tensor(Q))
+    assert Qm.data_ptr() % 16 != 0
+    for Qx in (torch.as_tensor(Q).cuda(), Qm):
+        index = mlk.DeviceNeighborIndex(X)
+        try:
+            indptr, indices, dist = index.radius(Qx, R2, sort_results=True)
+            assert_csr((indptr.cpu().numpy(), indices.cpu().numpy(), dist.cpu().numpy()), want)
+        finally:
+            index.close()
