@@ -167,6 +167,26 @@ int mlk_radius_fill(mlk_radius_result* result, int64_t* indices_dev, double* dis
 int mlk_radius_free(mlk_radius_result* result);
 
 /*
+ * Reductions of a radius result (before or instead of mlk_radius_fill), for
+ * RadiusNeighborsClassifier / RadiusNeighborsRegressor. Each query's entries
+ * are summed in CSR order (ascending training index), in float64.
+ * mlk_radius_votes: votes_dev (m x n_classes) = per-class sums of weights, the
+ * class of training row i being codes_dev[i]; counts_dev (m, optional) =
+ * neighbours per query. mlk_radius_mean: pred_dev (m) = weighted mean of
+ * targets_dev over the neighbours, NaN for an empty row. Weights: uniform, or
+ * distance (1/d; a row holding a zero distance uses the indicator of zero
+ * distances), which needs a result made with MLK_NEED_DISTANCE. Replaces
+ * reduce_votes_csr / reduce_mean_csr / distance_weights
+ * (src/mlxlearn/_kernels/radius_reduce.py:19-73) behind
+ * RadiusNeighborsClassifier / RadiusNeighborsRegressor
+ * (src/mlxlearn/neighbors/_radius.py:162-338).
+ */
+int mlk_radius_votes(const mlk_radius_result* result, const int64_t* indptr_dev, const int32_t* codes_dev,
+                     int32_t n_classes, int32_t weights, double* votes_dev, int64_t* counts_dev, void* stream);
+int mlk_radius_mean(const mlk_radius_result* result, const int64_t* indptr_dev, const double* targets_dev,
+                    int32_t weights, double* pred_dev, int64_t* counts_dev, void* stream);
+
+/*
  * k-means assignment: exact nearest centre of every row of X (n x d) among
  * C (kc x d), lowest index on ties; residual_dev gets the canonical float64
  * squared distance. Replaces KMeans._assignment / e_means.assign

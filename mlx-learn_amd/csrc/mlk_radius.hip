@@ -176,6 +176,60 @@ __global__ void k_everything_indptr(int64_t m, int64_t per, int64_t* __restrict_
 
 static inline unsigned blocks_for(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
 
+// ------------------------------------------------------ CSR reductions
+// Radius votes and means (reference radius_reduce.py:19-73): one thread per
+// query row sums its entries in CSR order (ascending index, as the
+// reference's np.bincount accumulates them), so the float64 results are
+// bitwise those of the reference's reduction. Distance weights are 1/d,
+// except on rows holding a zero distance, which use the indicator of zero
+// distances (sklearn's _get_weights).
+__device__ __forceinline__ bool row_coincident(const double* __restrict__ dist, int64_t b, int64_t e) {
+    for (int64_t j = b; j < e; ++j)
+        if (dist[j] == 0.0) return true;
+    return false;
+}
+
+__device__ __forceinline__ double entry_weight(const double* __restrict__ dist, int64_t j, bool coincident) {
+    return coincident ? (dist[j] == 0.0 ? 1.0 : 0.0) : 1.0 / dist[j];
+}
+
+__global__ void k_radius_votes(const int64_t* __restrict__ indptr, int64_t m, const int64_t* __restrict__ idx,
+                               const double* __restrict__ dist, const int32_t* __restrict__ codes, int n_classes,
+                               int weighted, double* __restrict__ votes, int64_t* __restrict__ counts) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= m) return;
+    const int64_t b = indptr[q], e = indptr[q + 1];
+    if (counts) counts[q] = e - b;
+    double* v = votes + q * n_classes;
+    for (int c = 0; c < n_classes; ++c) v[c] = 0.0;
+    const bool coincident = weighted && row_coincident(dist, b, e);
+    for (int64_t j = b; j < e; ++j) v[codes[idx[j]]] += weighted ? entry_weight(dist, j, coincident) : 1.0;
+}
+
+__global__ void k_radius_mean(const int64_t* __restrict__ indptr, int64_t m, const int64_t* __restrict__ idx,
+                              const double* __restrict__ dist, const double* __restrict__ targets, int weighted,
+                              double* __restrict__ pred, int64_t* __restrict__ counts) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= m) return;
+    const int64_t b = indptr[q], e = indptr[q + 1];
+    if (counts) counts[q] = e - b;
+    const bool coincident = weighted && row_coincident(dist, b, e);
+    double num = 0.0, den = 0.0;
+    for (int64_t j = b; j < e; ++j) {
+        if (weighted) {
+            // product rounded before the sum (no contraction), as the reference
+            // forms values * weights before adding
+            const double w = entry_weight(dist, j, coincident);
+            num = __dadd_rn(num, __dmul_rn(targets[idx[j]], w));
+            den = __dadd_rn(den, w);
+        } else {
+            num = __dadd_rn(num, targets[idx[j]]);
+        }
+    }
+    if (!weighted) den = (double)(e - b);
+    pred[q] = e == b ? NAN : num / den;
+}
+
 // Stable segmented sort of every CSR row by its distance bits (non-negative
 // doubles order like their bit patterns); rows arrive in ascending index
 // order, so equal distances stay in index order.
@@ -420,3 +474,56 @@ int radius_query_impl(mlk_index* index, const float* Q, int64_t m, double R2, in
 }
 
 }  // namespace mlk
+
+using namespace mlk;
+
+extern "C" int mlk_radius_votes(const mlk_radius_result* r, const int64_t* indptr_dev, const int32_t* codes_dev,
+                                int32_t n_classes, int32_t weights, double* votes_dev, int64_t* counts_dev,
+                                void* stream_) {
+    if (!r || (r->m && (!indptr_dev || !votes_dev)) || (r->nnz && !codes_dev) || n_classes < 1) {
+        set_error("radius_votes: result, row pointers, codes and votes are required");
+        return MLK_EINVAL;
+    }
+    if (weights != MLK_WEIGHTS_UNIFORM && weights != MLK_WEIGHTS_DISTANCE) {
+        set_error("weights must be uniform or distance");
+        return MLK_EINVAL;
+    }
+    if (weights == MLK_WEIGHTS_DISTANCE && !r->has_dist) {
+        set_error("distance weights require a radius result with distances");
+        return MLK_EINVAL;
+    }
+    hipStream_t stream = (hipStream_t)stream_;
+    if (r->m) {
+        hipLaunchKernelGGL(k_radius_votes, dim3(blocks_for(r->m)), dim3(256), 0, stream, indptr_dev, r->m,
+                           r->indices.as<int64_t>(), r->has_dist ? r->dist.as<double>() : nullptr, codes_dev,
+                           n_classes, (int)(weights == MLK_WEIGHTS_DISTANCE), votes_dev, counts_dev);
+        MLK_CHECK_HIP(hipGetLastError());
+    }
+    MLK_CHECK_HIP(hipStreamSynchronize(stream));
+    return MLK_OK;
+}
+
+extern "C" int mlk_radius_mean(const mlk_radius_result* r, const int64_t* indptr_dev, const double* targets_dev,
+                               int32_t weights, double* pred_dev, int64_t* counts_dev, void* stream_) {
+    if (!r || (r->m && (!indptr_dev || !pred_dev)) || (r->nnz && !targets_dev)) {
+        set_error("radius_mean: result, row pointers, targets and output are required");
+        return MLK_EINVAL;
+    }
+    if (weights != MLK_WEIGHTS_UNIFORM && weights != MLK_WEIGHTS_DISTANCE) {
+        set_error("weights must be uniform or distance");
+        return MLK_EINVAL;
+    }
+    if (weights == MLK_WEIGHTS_DISTANCE && !r->has_dist) {
+        set_error("distance weights require a radius result with distances");
+        return MLK_EINVAL;
+    }
+    hipStream_t stream = (hipStream_t)stream_;
+    if (r->m) {
+        hipLaunchKernelGGL(k_radius_mean, dim3(blocks_for(r->m)), dim3(256), 0, stream, indptr_dev, r->m,
+                           r->indices.as<int64_t>(), r->has_dist ? r->dist.as<double>() : nullptr, targets_dev,
+                           (int)(weights == MLK_WEIGHTS_DISTANCE), pred_dev, counts_dev);
+        MLK_CHECK_HIP(hipGetLastError());
+    }
+    MLK_CHECK_HIP(hipStreamSynchronize(stream));
+    return MLK_OK;
+}

@@ -159,6 +159,41 @@ class DeviceNeighborIndex:
             self._lib.mlk_radius_free(res)
         return indptr, indices, (dist if need_distance else None)
 
+    def radius_reduce(self, Q, R2, *, codes=None, n_classes=None, targets=None, weighted=False,
+                      exclude_self=False, q_offset=0, precision="float32"):
+        """Radius neighbourhoods reduced on the device without materialising the CSR.
+
+        With ``codes`` (int32 class code per training row, device): float64 votes
+        (m x n_classes); with ``targets`` (float64 per training row, device): the
+        (weighted) mean per query, NaN for an empty row. Returns (result, counts).
+        """
+        torch = _torch()
+        if (codes is None) == (targets is None):
+            raise ValueError("give exactly one of codes and targets")
+        Qd = to_device_f32(Q)
+        m = int(Qd.shape[0])
+        indptr = torch.empty(m + 1, dtype=torch.int64, device=self._device)
+        flags = precision_flag(precision) | (_lib.EXCLUDE_SELF if exclude_self else 0)
+        flags |= _lib.NEED_DISTANCE if weighted else 0
+        weights = _lib.WEIGHTS_DISTANCE if weighted else _lib.WEIGHTS_UNIFORM
+        nnz = ctypes.c_int64()
+        res = ctypes.c_void_p()
+        _lib.check(self._lib.mlk_radius_query(self._handle, _ptr(Qd), m, float(R2), int(q_offset), flags, _ptr(indptr),
+                                              ctypes.byref(nnz), ctypes.byref(res), _stream()), "mlk_radius_query")
+        counts = torch.empty(m, dtype=torch.int64, device=self._device)
+        try:
+            if codes is not None:
+                out = torch.empty((m, int(n_classes)), dtype=torch.float64, device=self._device)
+                _lib.check(self._lib.mlk_radius_votes(res, _ptr(indptr), _ptr(codes), int(n_classes), weights,
+                                                      _ptr(out), _ptr(counts), _stream()), "mlk_radius_votes")
+            else:
+                out = torch.empty(m, dtype=torch.float64, device=self._device)
+                _lib.check(self._lib.mlk_radius_mean(res, _ptr(indptr), _ptr(targets), weights, _ptr(out),
+                                                     _ptr(counts), _stream()), "mlk_radius_mean")
+        finally:
+            self._lib.mlk_radius_free(res)
+        return out, counts
+
     def autotune(self, Q, k=None, *, radius_sq=None, precision="float32", sort_results=False, max_trials=24):
         """Time screen plans on these queries and cache the fastest for calls of this shape.
 

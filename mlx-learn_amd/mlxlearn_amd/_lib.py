@@ -70,6 +70,8 @@ SIGNATURES = {
     "mlk_radius_query": ([_vp, _vp, _i64, _dbl, _i64, _u32, _vp, _i64p, ctypes.POINTER(_vp), _vp], ctypes.c_int),
     "mlk_radius_fill": ([_vp, _vp, _vp, _vp], ctypes.c_int),
     "mlk_radius_free": ([_vp], ctypes.c_int),
+    "mlk_radius_votes": ([_vp, _vp, _vp, _i32, _i32, _vp, _vp, _vp], ctypes.c_int),
+    "mlk_radius_mean": ([_vp, _vp, _vp, _i32, _vp, _vp, _vp], ctypes.c_int),
     "mlk_kmeans_assign": ([_vp, _i64, _i32, _vp, _i64, _u32, _vp, _vp, _vp], ctypes.c_int),
     "mlk_kmeans_assign_reuse": ([_vp, _i64, _i32, _vp, _i64, _vp, _u32, _vp, _vp, _i64p, _vp], ctypes.c_int),
     "mlk_kmeans_update": ([_vp, _i64, _i32, _vp, _i64, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),

@@ -1,4 +1,4 @@
-"""NearestNeighbors, KNeighborsClassifier and KNeighborsRegressor.
+"""NearestNeighbors, KNeighbors{Classifier,Regressor} and RadiusNeighbors{Classifier,Regressor}.
 
 Constructor signatures and defaults follow the reference
 (``src/mlxlearn/neighbors/_unsupervised.py:84-93``,
@@ -9,14 +9,18 @@ certified candidates are re-ranked in float64, with identical results).
 """
 from __future__ import annotations
 
+import warnings
+
 import numpy as np
 from sklearn.base import BaseEstimator, ClassifierMixin, RegressorMixin
 from sklearn.utils.validation import check_is_fitted
 
 from .. import _lib
+from .._index import require_gpu, validate_radius
 from ._base import NeighborsMixin, RadiusMixin, compute_weights
 
-__all__ = ["NearestNeighbors", "KNeighborsClassifier", "KNeighborsRegressor"]
+__all__ = ["NearestNeighbors", "KNeighborsClassifier", "KNeighborsRegressor", "RadiusNeighborsClassifier",
+           "RadiusNeighborsRegressor"]
 
 
 class NearestNeighbors(RadiusMixin, NeighborsMixin, BaseEstimator):
@@ -146,3 +150,177 @@ class KNeighborsRegressor(NeighborsMixin, RegressorMixin, BaseEstimator):
         if w is None:
             return vals.mean(axis=1)
         return (vals * w).sum(axis=1) / w.sum(axis=1)
+
+
+def _check_weights(weights):
+    if weights not in (None, "uniform", "distance"):
+        raise ValueError(f"weights not recognized: should be 'uniform' or 'distance', got {weights!r}")
+    return weights == "distance"
+
+
+def _radius_query(est, X):
+    """(query rows, exclude_self, R2) of a radius estimator's prediction."""
+    check_is_fitted(est, "_index_")
+    require_gpu()
+    query_is_train = X is None
+    Q = est._index_.X if query_is_train else est._validate_query(X, "predict")
+    r = validate_radius(est.radius)
+    return Q, query_is_train, r * r
+
+
+class RadiusNeighborsClassifier(NeighborsMixin, RadiusMixin, ClassifierMixin, BaseEstimator):
+    """Every training row within ``radius`` votes (reference neighbors/_radius.py:16-216).
+
+    Votes are reduced on the GPU from the exact radius neighbourhoods, in
+    ascending training index per query; a tied vote goes to the smallest class
+    code. Queries with no neighbour take ``outlier_label`` (``None`` raises).
+    """
+
+    n_neighbors = None
+
+    def __init__(self, radius=1.0, *, weights="uniform", algorithm="auto", leaf_size=30, p=2,
+                 metric="minkowski", outlier_label=None, metric_params=None, n_jobs=None, precision="float32"):
+        self.radius = radius
+        self.weights = weights
+        self.algorithm = algorithm
+        self.leaf_size = leaf_size
+        self.p = p
+        self.metric = metric
+        self.outlier_label = outlier_label
+        self.metric_params = metric_params
+        self.n_jobs = n_jobs
+        self.precision = precision
+
+    def fit(self, X, y):
+        self._fit_neighbors(X, y, target="classification")
+        y = self._y_checked_
+        del self._y_checked_
+        self.classes_, encoded = np.unique(y, return_inverse=True)
+        self._y = encoded.astype(np.intp, copy=False)
+        self.outputs_2d_ = False
+        self.outlier_label_ = _resolve_outlier_label(self.outlier_label, self.classes_, self._y)
+        self._codes_device = None
+        return self
+
+    _codes = KNeighborsClassifier._codes
+    __getstate__ = KNeighborsClassifier.__getstate__
+
+    def _proba(self, X):
+        weighted = _check_weights(self.weights)
+        Q, query_is_train, R2 = _radius_query(self, X)
+        if Q.shape[0] == 0:
+            return np.empty((0, len(self.classes_)))
+        votes, counts = self._index_.device_index.radius_reduce(
+            Q, R2, codes=self._codes(), n_classes=len(self.classes_), weighted=weighted,
+            exclude_self=query_is_train, precision=self._index_.precision)
+        self.last_query_backend_ = "gpu"
+        votes = votes.cpu().numpy()
+        counts = counts.cpu().numpy()
+        outliers = np.flatnonzero(counts == 0)
+        if outliers.size > 0 and self.outlier_label_ is None:
+            raise ValueError(f"No neighbors found for test samples {outliers!r}, you can try using larger radius, "
+                             "giving a label for outliers, or considering removing them from your dataset.")
+        if outliers.size > 0:
+            label = self.outlier_label_[0]
+            match = np.flatnonzero(self.classes_ == label)
+            if match.size == 1:
+                votes[outliers, match[0]] = 1.0
+            else:
+                warnings.warn(f"Outlier label {label} is not in training classes. All class probabilities of "
+                              "outliers will be assigned with 0.", stacklevel=3)
+        normalizer = votes.sum(axis=1)[:, None]
+        normalizer[normalizer == 0.0] = 1.0
+        return votes / normalizer
+
+    def predict(self, X):
+        """Labels from ``classes_``; an empty neighbourhood takes the outlier label."""
+        proba = self._proba(X)
+        labels = self.classes_.take(proba.argmax(axis=1)) if proba.size else self.classes_[:0]
+        empty = np.all(proba == 0.0, axis=1)
+        if np.any(empty):
+            labels = np.array(labels, copy=True)
+            labels[empty] = self.outlier_label_[0]
+        return labels
+
+    def predict_proba(self, X):
+        """Vote shares per class (float64 rows summing to one; outliers one-hot or zero)."""
+        return self._proba(X)
+
+
+class RadiusNeighborsRegressor(NeighborsMixin, RadiusMixin, RegressorMixin, BaseEstimator):
+    """(Weighted) mean target of the training rows within ``radius``
+    (reference neighbors/_radius.py:219-338); NaN, with a warning, for a query
+    without neighbours."""
+
+    n_neighbors = None
+
+    def __init__(self, radius=1.0, *, weights="uniform", algorithm="auto", leaf_size=30, p=2,
+                 metric="minkowski", metric_params=None, n_jobs=None, precision="float32"):
+        self.radius = radius
+        self.weights = weights
+        self.algorithm = algorithm
+        self.leaf_size = leaf_size
+        self.p = p
+        self.metric = metric
+        self.metric_params = metric_params
+        self.n_jobs = n_jobs
+        self.precision = precision
+
+    def fit(self, X, y):
+        self._fit_neighbors(X, y, target="regression")
+        checked = self._y_checked_
+        del self._y_checked_
+        self._y = np.array(checked, dtype=np.float64, copy=True)
+        self._y_out_dtype_ = np.float32 if np.asarray(checked).dtype == np.float32 else np.float64
+        self._targets_device = None
+        return self
+
+    def __getstate__(self):
+        state = dict(self.__dict__)
+        state["_targets_device"] = None
+        return state
+
+    def predict(self, X):
+        """float64 (float32 for float32 targets) means; NaN where no neighbour lies inside ``radius``."""
+        import torch
+        weighted = _check_weights(self.weights)
+        Q, query_is_train, R2 = _radius_query(self, X)
+        if Q.shape[0] == 0:
+            return np.empty(0, dtype=self._y_out_dtype_)
+        if getattr(self, "_targets_device", None) is None:
+            self._targets_device = torch.as_tensor(self._y).cuda()
+        pred, counts = self._index_.device_index.radius_reduce(
+            Q, R2, targets=self._targets_device, weighted=weighted, exclude_self=query_is_train,
+            precision=self._index_.precision)
+        self.last_query_backend_ = "gpu"
+        pred = pred.cpu().numpy()
+        if np.any(counts.cpu().numpy() == 0):
+            warnings.warn("One or more samples have no neighbors within specified radius; predicting NaN.",
+                          stacklevel=2)
+        return pred.astype(self._y_out_dtype_, copy=False)
+
+
+def _resolve_outlier_label(outlier_label, classes, encoded_y):
+    """``outlier_label_`` for a single output, as scikit-learn fills it (reference _radius.py:341-380)."""
+    if outlier_label is None:
+        return None
+    if isinstance(outlier_label, str) and outlier_label == "most_frequent":
+        counts = np.bincount(encoded_y, minlength=len(classes))
+        return [classes[int(counts.argmax())]]
+
+    def arraylike(value):
+        return (hasattr(value, "__len__") or hasattr(value, "shape") or hasattr(value, "__array__")) \
+            and not isinstance(value, (str, bytes))
+
+    if arraylike(outlier_label):
+        if len(outlier_label) != 1:
+            raise ValueError(f"The length of outlier_label: {outlier_label} is inconsistent with the output "
+                             "length: 1")
+        label = list(outlier_label)[0]
+    else:
+        label = outlier_label
+    if arraylike(label):
+        raise TypeError(f"The outlier_label of classes {classes} is supposed to be a scalar, got {label}.")
+    if np.append(classes, label).dtype != classes.dtype:
+        raise TypeError(f"The dtype of outlier_label {label} is inconsistent with classes {classes} in y.")
+    return [label]

@@ -155,3 +155,38 @@ def test_pickle_drops_device_state():
     nn2 = pickle.loads(pickle.dumps(nn))
     np.testing.assert_array_equal(nn2._fit_X, np.eye(4, dtype=np.float32))
     assert nn2._index_._device_index is None
+
+
+def test_radius_estimators_host_logic():
+    """RadiusNeighborsClassifier outlier-label rules (reference neighbors/_radius.py:341-380) and loud failure
+    without a device."""
+    from mlxlearn_amd import RadiusNeighborsClassifier, RadiusNeighborsRegressor
+    from mlxlearn_amd.exceptions import DeviceError
+    from mlxlearn_amd.neighbors._estimators import _resolve_outlier_label
+    classes = np.array([1, 2, 5])
+    assert _resolve_outlier_label(None, classes, np.array([0])) is None
+    assert _resolve_outlier_label("most_frequent", classes, np.array([0, 2, 2, 1])) == [5]
+    assert _resolve_outlier_label(7, classes, np.array([0])) == [7]
+    assert _resolve_outlier_label([7], classes, np.array([0])) == [7]
+    with pytest.raises(ValueError, match="inconsistent with the output length"):
+        _resolve_outlier_label([7, 8], classes, np.array([0]))
+    with pytest.raises(TypeError, match="supposed to be a scalar"):
+        _resolve_outlier_label([[7]], classes, np.array([0]))
+    with pytest.raises(TypeError, match="inconsistent with classes"):
+        _resolve_outlier_label("x", classes, np.array([0]))
+    X, y = np.array([[0.0], [1.0], [2.0], [3.0]]), np.array([0, 0, 1, 1])
+    clf = RadiusNeighborsClassifier(radius=1.0, outlier_label="most_frequent").fit(X, y)
+    assert clf.outlier_label_ == [0] and list(clf.classes_) == [0, 1]
+    reg = RadiusNeighborsRegressor(radius=1.0).fit(X, y.astype(np.float32))
+    for est in (clf, reg):
+        if not _gpu():
+            with pytest.raises(DeviceError):
+                est.predict([[1.5]])
+
+
+def _gpu():
+    try:
+        import torch
+        return torch.cuda.is_available()
+    except Exception:
+        return False
