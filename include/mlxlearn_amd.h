@@ -118,6 +118,13 @@ int mlk_autotune(mlk_index* index, const float* Q_dev, int64_t m, int32_t k, dou
                  uint32_t flags, int32_t max_trials, mlk_plan* out, void* stream);
 int mlk_get_plan(const mlk_index* index, int64_t m, int32_t k, uint32_t flags, mlk_plan* out);
 int mlk_clear_plans(void);
+/* The plan cache as text (one line per tuned plan), so a plan tuned once can
+ * be reused by later processes: mlk_export_plans writes *len bytes into buf
+ * (buf NULL and cap 0 only report *len); mlk_import_plans adds the plans of a
+ * text made by mlk_export_plans, rejecting the whole text (MLK_EINVAL) if a
+ * line is not a valid plan for a built screen shape. */
+int mlk_export_plans(char* buf, int64_t cap, int64_t* len);
+int mlk_import_plans(const char* text, int64_t len);
 
 /*
  * Exact brute-force index over X (n x d float32, row-major, device memory).

@@ -216,6 +216,23 @@ int mlk_clear_plans(void) {
     return MLK_OK;
 }
 
+int mlk_export_plans(char* buf, int64_t cap, int64_t* len) {
+    if (!len || cap < 0 || (cap && !buf)) return invalid("export_plans: length output and buffer are required");
+    const std::string text = export_plans_impl();
+    *len = (int64_t)text.size();
+    if ((int64_t)text.size() > cap) {
+        if (buf) return invalid("export_plans: buffer too small (" + std::to_string(text.size()) + " bytes needed)");
+        return MLK_OK;  // size query
+    }
+    std::memcpy(buf, text.data(), text.size());
+    return MLK_OK;
+}
+
+int mlk_import_plans(const char* text, int64_t len) {
+    if (len < 0 || (len && !text)) return invalid("import_plans: text is required");
+    return import_plans_impl(text, (size_t)len);
+}
+
 int mlk_radius_fill(mlk_radius_result* r, int64_t* indices_dev, double* dist_dev, void* stream) {
     if (!r) return invalid("result is NULL");
     if (r->nnz && !indices_dev) return invalid("indices output is NULL");

@@ -119,6 +119,8 @@ int autotune_impl(mlk_index* index, const float* Q, int64_t m, int k, double R2,
                   mlk_plan* out, hipStream_t stream);
 int plan_info_impl(const mlk_index* index, int64_t m, int k, uint32_t flags, mlk_plan* out);
 void clear_plans_impl();
+std::string export_plans_impl();
+int import_plans_impl(const char* text, size_t len);
 // implemented in mlk_radius.hip
 int radius_query_impl(mlk_index* index, const float* Q, int64_t m, double R2, int64_t q_offset,
                       uint32_t flags, int64_t* indptr, int64_t* nnz, mlk_radius_result** out,

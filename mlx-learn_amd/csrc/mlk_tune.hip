@@ -13,8 +13,12 @@
 #include <array>
 #include <cmath>
 #include <limits>
+#include <cstdio>
+#include <cstdlib>
 #include <map>
 #include <mutex>
+#include <sstream>
+#include <string>
 
 #include "mlk_internal.h"
 #include "mlk_screen.h"
@@ -161,6 +165,89 @@ int plan_info_impl(const mlk_index* index, int64_t m, int k, uint32_t flags, mlk
 void clear_plans_impl() {
     std::lock_guard<std::mutex> g(g_plan_lock);
     g_plans.clear();
+}
+
+// Text form of the plan cache, one plan per line:
+//   plan <kind> <prec> <dp> <k> <n bucket> <m bucket> <qt> <ns> <splits> <cap>
+//        <rounds> <fraction>... <tuned ms> <default ms> <trials>
+// (fractions and times as hexadecimal floating point, so they round-trip).
+std::string export_plans_impl() {
+    std::lock_guard<std::mutex> g(g_plan_lock);
+    std::string out;
+    char buf[128];
+    for (const auto& kv : g_plans) {
+        const PlanKey& key = kv.first;
+        const TunedEntry& e = kv.second;
+        out += "plan";
+        for (int64_t v : key) {
+            snprintf(buf, sizeof buf, " %lld", (long long)v);
+            out += buf;
+        }
+        snprintf(buf, sizeof buf, " %d %d %d %d %zu", e.plan.shape.qt, e.plan.shape.ns, e.plan.splits, e.plan.cap,
+                 e.plan.rounds.size());
+        out += buf;
+        for (double f : e.plan.rounds) {
+            snprintf(buf, sizeof buf, " %a", f);
+            out += buf;
+        }
+        snprintf(buf, sizeof buf, " %a %a %d\n", e.ms, e.default_ms, e.trials);
+        out += buf;
+    }
+    return out;
+}
+
+int import_plans_impl(const char* text, size_t len) {
+    std::map<PlanKey, TunedEntry> add;
+    std::string all(text, len);
+    size_t pos = 0;
+    int line_no = 0;
+    while (pos < all.size()) {
+        size_t end = all.find('\n', pos);
+        if (end == std::string::npos) end = all.size();
+        const std::string line = all.substr(pos, end - pos);
+        pos = end + 1;
+        ++line_no;
+        if (line.empty()) continue;
+        std::istringstream in(line);
+        std::string tag;
+        PlanKey key{};
+        TunedEntry e;
+        size_t nr = 0;
+        in >> tag;
+        for (auto& v : key) in >> v;
+        in >> e.plan.shape.qt >> e.plan.shape.ns >> e.plan.splits >> e.plan.cap >> nr;
+        bool ok = tag == "plan" && !in.fail() && nr <= MLK_MAX_ROUNDS;
+        std::string tok;
+        for (size_t i = 0; ok && i < nr; ++i) {
+            ok = static_cast<bool>(in >> tok);
+            if (ok) e.plan.rounds.push_back(std::strtod(tok.c_str(), nullptr));
+        }
+        std::string t1, t2;
+        ok = ok && static_cast<bool>(in >> t1 >> t2 >> e.trials);
+        if (ok) {
+            e.ms = std::strtod(t1.c_str(), nullptr);
+            e.default_ms = std::strtod(t2.c_str(), nullptr);
+        }
+        // the plan must name a built screen shape and a well-formed schedule
+        const int kind = (int)key[0], prec = (int)key[1], dp = (int)key[2];
+        ok = ok && (kind == PLAN_KNN || kind == PLAN_RADIUS) && prec >= PREC_FP32 && prec <= PREC_FP16 &&
+             e.plan.splits >= 1 && e.plan.splits <= 4096 && (kind == PLAN_RADIUS || e.plan.cap >= 1) &&
+             pick_screen(kind == PLAN_KNN ? SCREEN_LISTS : SCREEN_PAIRS, prec, dp, e.plan.shape) != nullptr;
+        double prev = 0.0;
+        for (double f : e.plan.rounds) {
+            ok = ok && f > prev && f <= 1.0;
+            prev = f;
+        }
+        ok = ok && (kind == PLAN_RADIUS ? e.plan.rounds.empty() : (!e.plan.rounds.empty() && prev == 1.0));
+        if (!ok) {
+            set_error("plan text, line " + std::to_string(line_no) + ": not a valid plan");
+            return MLK_EINVAL;
+        }
+        add[key] = e;
+    }
+    std::lock_guard<std::mutex> g(g_plan_lock);
+    for (const auto& kv : add) g_plans[kv.first] = kv.second;
+    return MLK_OK;
 }
 
 // Coordinate descent over the plan: screen shape, then database splits (a

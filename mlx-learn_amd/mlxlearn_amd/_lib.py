@@ -79,6 +79,8 @@ SIGNATURES = {
     "mlk_autotune": ([_vp, _vp, _i64, _i32, _dbl, _u32, _i32, ctypes.POINTER(Plan), _vp], ctypes.c_int),
     "mlk_get_plan": ([_vp, _i64, _i32, _u32, ctypes.POINTER(Plan)], ctypes.c_int),
     "mlk_clear_plans": ([], ctypes.c_int),
+    "mlk_export_plans": ([ctypes.c_char_p, _i64, _i64p], ctypes.c_int),
+    "mlk_import_plans": ([ctypes.c_char_p, _i64], ctypes.c_int),
     "mlk_mfma_probe": ([_i32, _i32, ctypes.POINTER(_dbl), _vp], ctypes.c_int),
 }
 
@@ -141,3 +143,19 @@ def set_tuning(splits: int = 0, capacity: int = 0, query_tiles: int = 0, stages:
 def clear_plans() -> None:
     """Forget every autotuned plan of this process."""
     check(load().mlk_clear_plans(), "mlk_clear_plans")
+
+
+def export_plans() -> str:
+    """The autotuned plans of this process as text (see import_plans)."""
+    lib = load()
+    n = ctypes.c_int64()
+    check(lib.mlk_export_plans(None, 0, ctypes.byref(n)), "mlk_export_plans")
+    buf = ctypes.create_string_buffer(max(int(n.value), 1))
+    check(lib.mlk_export_plans(buf, int(n.value), ctypes.byref(n)), "mlk_export_plans")
+    return buf.raw[:n.value].decode()
+
+
+def import_plans(text: str) -> None:
+    """Add plans exported by export_plans (in this or another process)."""
+    data = text.encode()
+    check(load().mlk_import_plans(data, len(data)), "mlk_import_plans")

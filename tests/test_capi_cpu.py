@@ -190,3 +190,23 @@ def _gpu():
         return torch.cuda.is_available()
     except Exception:
         return False
+
+
+def test_plan_cache_text_round_trip():
+    """Exported plans import back unchanged; a plan naming a shape that is not built is rejected whole."""
+    from mlxlearn_amd import _lib
+    _lib.clear_plans()
+    assert _lib.export_plans() == ""
+    line = "plan 0 1 128 10 40 27 1 2 24 48 5 0x1p-8 0x1p-5 0x1p-3 0x1p-1 0x1p+0 0x1.dp+1 0x1.fp+1 16\n"
+    radius = "plan 1 0 32 0 42 34 1 2 8 0 0 0x1.cp+7 0x1.1p+8 5\n"
+    _lib.import_plans(line + radius)
+    text = _lib.export_plans()
+    assert sorted(text.splitlines()) == sorted([line.strip(), radius.strip()])
+    for bad in ("plan 0 1 128 10 40 27 3 2 24 48 5 0x1p-8 0x1p-5 0x1p-3 0x1p-1 0x1p+0 1 2 3\n",  # qt 3
+                "plan 0 1 128 10 40 27 1 2 24 48 2 0x1p-1 0x1p-2 1 2 3\n",                          # decreasing
+                "plan 1 0 32 0 42 34 1 2 8 0 1 0x1p+0 1 2 3\n",                                   # radius rounds
+                "nonsense\n"):
+        with pytest.raises(ValueError, match="not a valid plan"):
+            _lib.import_plans(line + bad)
+    assert sorted(_lib.export_plans().splitlines()) == sorted([line.strip(), radius.strip()])
+    _lib.clear_plans()

@@ -85,8 +85,14 @@ def test_autotuned_knn_plan_is_used_and_exact(mlk, precision):
         got = knn(index, Q, 10, precision)
         assert _lib.stats()["splits"] == plan["splits"]
         assert_parity(got, want)
+        text = _lib.export_plans()
         _lib.clear_plans()
         assert not index.plan(1500, 10, precision=precision)["tuned"]
+        _lib.import_plans(text)  # as a later process would
+        again = index.plan(1500, 10, precision=precision)
+        assert again["tuned"] and again["splits"] == plan["splits"] and again["tuned_ms"] == plan["tuned_ms"]
+        assert_parity(knn(index, Q, 10, precision), want)
+        _lib.clear_plans()
     finally:
         index.close()
 
