@@ -96,21 +96,23 @@ __global__ void k_unsafe_pairs(int64_t m, const int* __restrict__ unsafe, int n_
     if (at < cap) pairs[at] = int2{(int)q, row};
 }
 
-__global__ void k_pair_keys(const int2* __restrict__ pairs, int64_t np, unsigned long long* __restrict__ keys) {
+// Pair keys (query << rb) | row, rb = bits of the row count: sorting them
+// orders pairs by (query, row) with as few key bits as the shape needs.
+__global__ void k_pair_keys(const int2* __restrict__ pairs, int64_t np, int rb, unsigned long long* __restrict__ keys) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < np) keys[i] = ((unsigned long long)(unsigned)pairs[i].x << 32) | (unsigned)pairs[i].y;
+    if (i < np) keys[i] = ((unsigned long long)(unsigned)pairs[i].x << rb) | (unsigned)pairs[i].y;
 }
 
 // VEC: d % 4 == 0 and 16-byte aligned rows (four features per load).
 template <bool VEC>
-__global__ void k_classify(const unsigned long long* __restrict__ keys, int64_t np, const float* __restrict__ X,
+__global__ void k_classify(const unsigned long long* __restrict__ keys, int64_t np, int rb, const float* __restrict__ X,
                            const float* __restrict__ Q, int d, RadiusConsts rc, uint8_t* __restrict__ member,
                            double* __restrict__ dist) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
     const unsigned long long key = keys[i];
-    const int64_t q = (int64_t)(key >> 32);
-    const int64_t row = (int64_t)(key & 0xffffffffull);
+    const int64_t q = (int64_t)(key >> rb);
+    const int64_t row = (int64_t)(key & ((1ull << rb) - 1));
     const float* qv = Q + q * d;
     const float* xv = X + row * d;
     const double s = VEC ? canonical_sq4(qv, xv, d) : canonical_sq(qv, xv, d);
@@ -124,11 +126,11 @@ __global__ void k_classify(const unsigned long long* __restrict__ keys, int64_t 
 
 // CSR row pointers from the sorted member keys: indptr[q] is the first key
 // whose query part is >= q (binary search; indptr[m] = nnz).
-__global__ void k_indptr_search(const unsigned long long* __restrict__ keys, int64_t nnz, int64_t m,
+__global__ void k_indptr_search(const unsigned long long* __restrict__ keys, int64_t nnz, int64_t m, int rb,
                                 int64_t* __restrict__ indptr) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q > m) return;
-    const unsigned long long target = (unsigned long long)q << 32;
+    const unsigned long long target = (unsigned long long)q << rb;
     int64_t lo = 0, hi = nnz;
     while (lo < hi) {
         const int64_t mid = lo + ((hi - lo) >> 1);
@@ -137,9 +139,10 @@ __global__ void k_indptr_search(const unsigned long long* __restrict__ keys, int
     indptr[q] = lo;
 }
 
-__global__ void k_key_to_index(const unsigned long long* __restrict__ keys, int64_t nnz, int64_t* __restrict__ idx) {
+__global__ void k_key_to_index(const unsigned long long* __restrict__ keys, int64_t nnz, int rb,
+                               int64_t* __restrict__ idx) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nnz) idx[i] = (int64_t)(keys[i] & 0xffffffffull);
+    if (i < nnz) idx[i] = (int64_t)(keys[i] & ((1ull << rb) - 1));
 }
 
 __global__ void k_dist_bits(const double* __restrict__ dist, int64_t nnz, unsigned long long* __restrict__ bits) {
@@ -405,7 +408,9 @@ int radius_query_impl(mlk_index* index, const float* Q, int64_t m, double R2, in
     const int64_t np = (int64_t)count;
     st.candidates = np;
 
-    // sort pairs by (query, row)
+    // sort pairs by (query, row): keys (query << rb) | row, rb = bits of n
+    int rb = 1;
+    while ((1ll << rb) < n) ++rb;
     MLK_TRY(ws[10].reserve((size_t)std::max<int64_t>(np, 1) * 8));   // keys in
     MLK_TRY(ws[11].reserve((size_t)std::max<int64_t>(np, 1) * 8));   // keys sorted
     MLK_TRY(ws[12].reserve((size_t)std::max<int64_t>(np, 1) * 8));   // dist
@@ -413,16 +418,16 @@ int radius_query_impl(mlk_index* index, const float* Q, int64_t m, double R2, in
     unsigned long long* keys = ws[10].as<unsigned long long>();
     unsigned long long* skeys = ws[11].as<unsigned long long>();
     if (np) {
-        hipLaunchKernelGGL(k_pair_keys, dim3(blocks_for(np)), dim3(256), 0, stream, ws[9].as<int2>(), np, keys);
+        hipLaunchKernelGGL(k_pair_keys, dim3(blocks_for(np)), dim3(256), 0, stream, ws[9].as<int2>(), np, rb, keys);
         int qbits = 1;
         while ((1ll << qbits) < m) ++qbits;
         size_t tmp_bytes = 0;
-        MLK_CHECK_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp_bytes, keys, skeys, (int)np, 0, 32 + qbits, stream));
+        MLK_CHECK_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp_bytes, keys, skeys, (int)np, 0, rb + qbits, stream));
         MLK_TRY(ws[14].reserve(tmp_bytes + 16));
-        MLK_CHECK_HIP(hipcub::DeviceRadixSort::SortKeys(ws[14].ptr, tmp_bytes, keys, skeys, (int)np, 0, 32 + qbits, stream));
+        MLK_CHECK_HIP(hipcub::DeviceRadixSort::SortKeys(ws[14].ptr, tmp_bytes, keys, skeys, (int)np, 0, rb + qbits, stream));
         const bool vec = d % 4 == 0 && ((uintptr_t)Q % 16) == 0 && ((uintptr_t)X % 16) == 0;
         hipLaunchKernelGGL(vec ? k_classify<true> : k_classify<false>, dim3(blocks_for(np)), dim3(256), 0, stream,
-                           skeys, np, X, Q, d, rc, ws[13].as<uint8_t>(), ws[12].as<double>());
+                           skeys, np, rb, X, Q, d, rc, ws[13].as<uint8_t>(), ws[12].as<double>());
         MLK_CHECK_HIP(hipGetLastError());
     }
     // compact members (order preserved)
@@ -451,12 +456,12 @@ int radius_query_impl(mlk_index* index, const float* Q, int64_t m, double R2, in
 
     // CSR row pointers
     hipLaunchKernelGGL(k_indptr_search, dim3(blocks_for(m + 1)), dim3(256), 0, stream, mkeys.as<unsigned long long>(),
-                       nnz, m, indptr);
+                       nnz, m, rb, indptr);
     MLK_CHECK_HIP(hipGetLastError());
     MLK_TRY(res->indices.reserve((size_t)std::max<int64_t>(nnz, 1) * sizeof(int64_t)));
     if (nnz)
         hipLaunchKernelGGL(k_key_to_index, dim3(blocks_for(nnz)), dim3(256), 0, stream, mkeys.as<unsigned long long>(), nnz,
-                           res->indices.as<int64_t>());
+                           rb, res->indices.as<int64_t>());
     if (res->has_dist) {
         MLK_TRY(res->dist.reserve((size_t)std::max<int64_t>(nnz, 1) * sizeof(double)));
         if (nnz)
