@@ -42,13 +42,10 @@ extern "C" int mlk_mfma_probe(int32_t blocks, int32_t iterations, double* tflops
     hipStream_t stream = (hipStream_t)stream_;
     DevBuf out;
     MLK_TRY(out.reserve((size_t)blocks * 256 * sizeof(float)));
+    Events events;
     hipEvent_t e0, e1;
-    MLK_CHECK_HIP(hipEventCreate(&e0));
-    MLK_CHECK_HIP(hipEventCreate(&e1));
-    struct EvGuard {
-        hipEvent_t a, b;
-        ~EvGuard() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); }
-    } evg{e0, e1};
+    MLK_CHECK_HIP(events.make(&e0));
+    MLK_CHECK_HIP(events.make(&e1));
     hipLaunchKernelGGL(k_mfma_probe, dim3((unsigned)blocks), dim3(256), 0, stream, out.as<float>(), iterations);
     MLK_CHECK_HIP(hipGetLastError());
     MLK_CHECK_HIP(hipEventRecord(e0, stream));

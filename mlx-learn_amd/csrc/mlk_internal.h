@@ -34,6 +34,23 @@ struct Status {
         if (_rc != MLK_OK) return _rc;                                                        \
     } while (0)
 
+// HIP events owned by one call: every event made here is destroyed when the
+// owner goes out of scope, on every return path.
+struct Events {
+    std::vector<hipEvent_t> ev;
+    Events() = default;
+    Events(const Events&) = delete;
+    Events& operator=(const Events&) = delete;
+    ~Events() {
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    }
+    hipError_t make(hipEvent_t* out) {
+        const hipError_t r = hipEventCreate(out);
+        if (r == hipSuccess) ev.push_back(*out);
+        return r;
+    }
+};
+
 // Grow-only device buffer.
 struct DevBuf {
     void* ptr = nullptr;

@@ -895,14 +895,11 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
     const bool exclude = (flags & MLK_EXCLUDE_SELF) != 0;
     st.screen_precision = (uint32_t)prec << 4;  // the precision actually screened
 
+    Events events;  // every event of this call
     hipEvent_t ev0, ev1, ev2;
-    MLK_CHECK_HIP(hipEventCreate(&ev0));
-    MLK_CHECK_HIP(hipEventCreate(&ev1));
-    MLK_CHECK_HIP(hipEventCreate(&ev2));
-    struct EvGuard {
-        hipEvent_t a, b, c;
-        ~EvGuard() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); (void)hipEventDestroy(c); }
-    } evg{ev0, ev1, ev2};
+    MLK_CHECK_HIP(events.make(&ev0));
+    MLK_CHECK_HIP(events.make(&ev1));
+    MLK_CHECK_HIP(events.make(&ev2));
     MLK_CHECK_HIP(hipEventRecord(ev0, stream));
 
     MLK_TRY(prepare_mirror(index, prec, stream));
@@ -1056,8 +1053,8 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
             prev = hi;
             if (sa.off_hi <= sa.off_lo) continue;
             hipEvent_t ea, eb;
-            MLK_CHECK_HIP(hipEventCreate(&ea));
-            MLK_CHECK_HIP(hipEventCreate(&eb));
+            MLK_CHECK_HIP(events.make(&ea));
+            MLK_CHECK_HIP(events.make(&eb));
             round_ev.push_back(ea);
             round_ev.push_back(eb);
             MLK_CHECK_HIP(hipEventRecord(ea, stream));
@@ -1168,11 +1165,7 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
         }
     }
     hipEvent_t ev3;
-    MLK_CHECK_HIP(hipEventCreate(&ev3));
-    struct RoundEvents {
-        std::vector<hipEvent_t>& v;
-        ~RoundEvents() { for (auto e : v) (void)hipEventDestroy(e); }
-    } rev{round_events};
+    MLK_CHECK_HIP(events.make(&ev3));
     MLK_CHECK_HIP(hipEventRecord(ev3, stream));
     MLK_CHECK_HIP(hipEventSynchronize(ev3));
     float ms = 0.0f;
@@ -1181,7 +1174,6 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
     for (size_t i = 0; i + 1 < round_events.size(); i += 2)
         if (hipEventElapsedTime(&ms, round_events[i], round_events[i + 1]) == hipSuccess) st.screen_ms += ms;
     if (hipEventElapsedTime(&ms, ev0, ev3) == hipSuccess) st.total_ms = ms;
-    (void)hipEventDestroy(ev3);
     return MLK_OK;
 }
 

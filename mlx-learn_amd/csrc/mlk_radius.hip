@@ -236,10 +236,15 @@ __global__ void k_radius_mean(const int64_t* __restrict__ indptr, int64_t m, con
 // Stable segmented sort of every CSR row by its distance bits (non-negative
 // doubles order like their bit patterns); rows arrive in ascending index
 // order, so equal distances stay in index order.
-static int sort_rows_by_distance(mlk_radius_result* res, const int64_t* indptr, int64_t m, int64_t nnz,
-                                 DevBuf& tmp, hipStream_t stream) {
+// Temporaries live in the index workspace (ws[14] sort scratch, ws[18..20]
+// keys and indices); the sorted indices are swapped into the result.
+static int sort_rows_by_distance(mlk_radius_result* res, const int64_t* indptr, int64_t m, int64_t nnz, DevBuf* ws,
+                                 hipStream_t stream) {
     if (!nnz) return MLK_OK;
-    DevBuf bits_in, bits_out, idx_out;
+    DevBuf& tmp = ws[14];
+    DevBuf& bits_in = ws[18];
+    DevBuf& bits_out = ws[19];
+    DevBuf& idx_out = ws[20];
     MLK_TRY(bits_in.reserve((size_t)nnz * 8));
     MLK_TRY(bits_out.reserve((size_t)nnz * 8));
     MLK_TRY(idx_out.reserve((size_t)nnz * 8));
@@ -253,12 +258,12 @@ static int sort_rows_by_distance(mlk_radius_result* res, const int64_t* indptr, 
     MLK_CHECK_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(
         tmp.ptr, tmp_bytes, bits_in.as<unsigned long long>(), bits_out.as<unsigned long long>(),
         res->indices.as<int64_t>(), idx_out.as<int64_t>(), (int)nnz, (int)m, indptr, indptr + 1, 0, 64, stream));
-    MLK_CHECK_HIP(hipMemcpyAsync(res->indices.ptr, idx_out.ptr, nnz * 8, hipMemcpyDeviceToDevice, stream));
     hipLaunchKernelGGL(k_bits_dist, dim3(blocks_for(nnz)), dim3(256), 0, stream, bits_out.as<unsigned long long>(),
                        nnz, res->dist.as<double>());
     MLK_CHECK_HIP(hipGetLastError());
-    // the temporaries are freed on return: wait for the kernels that use them
-    MLK_CHECK_HIP(hipStreamSynchronize(stream));
+    // the sorted indices become the result's; its unsorted buffer becomes workspace
+    std::swap(res->indices.ptr, idx_out.ptr);
+    std::swap(res->indices.bytes, idx_out.bytes);
     return MLK_OK;
 }
 
@@ -302,7 +307,7 @@ int radius_query_impl(mlk_index* index, const float* Q, int64_t m, double R2, in
                            (int)exclude, R2, res->indices.as<int64_t>(), res->has_dist ? res->dist.as<double>() : nullptr);
         hipLaunchKernelGGL(k_everything_indptr, dim3(blocks_for(m + 1)), dim3(256), 0, stream, m, per, indptr);
         MLK_CHECK_HIP(hipGetLastError());
-        if (sort_rows) MLK_TRY(sort_rows_by_distance(res, indptr, m, total, index->ws[14], stream));
+        if (sort_rows) MLK_TRY(sort_rows_by_distance(res, indptr, m, total, index->ws, stream));
         MLK_CHECK_HIP(hipStreamSynchronize(stream));
         res->nnz = total;
         *nnz_out = total;
@@ -352,9 +357,10 @@ int radius_query_impl(mlk_index* index, const float* Q, int64_t m, double R2, in
     if (ws[9].bytes / sizeof(int2) > cap) cap = ws[9].bytes / sizeof(int2);
     unsigned long long count = 0;
     unsigned long long* dcount = (unsigned long long*)(dflag + 2);
+    Events events;
     hipEvent_t e0, e1;
-    MLK_CHECK_HIP(hipEventCreate(&e0));
-    MLK_CHECK_HIP(hipEventCreate(&e1));
+    MLK_CHECK_HIP(events.make(&e0));
+    MLK_CHECK_HIP(events.make(&e1));
     for (int attempt = 0; attempt < 3; ++attempt) {
         MLK_TRY(ws[9].reserve((size_t)cap * sizeof(int2)));
         cap = ws[9].bytes / sizeof(int2);
@@ -403,8 +409,6 @@ int radius_query_impl(mlk_index* index, const float* Q, int64_t m, double R2, in
     }
     float ms = 0.0f;
     if (hipEventElapsedTime(&ms, e0, e1) == hipSuccess) st.screen_ms = ms;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     const int64_t np = (int64_t)count;
     st.candidates = np;
 
@@ -434,7 +438,8 @@ int radius_query_impl(mlk_index* index, const float* Q, int64_t m, double R2, in
     MLK_TRY(ws[15].reserve(64));
     int64_t* dnnz = ws[15].as<int64_t>();
     MLK_CHECK_HIP(hipMemsetAsync(dnnz, 0, sizeof(int64_t), stream));
-    DevBuf mkeys, mdist;
+    DevBuf& mkeys = ws[16];  // member keys and distances (workspace, grown on demand)
+    DevBuf& mdist = ws[17];
     MLK_TRY(mkeys.reserve((size_t)std::max<int64_t>(np, 1) * 8));
     MLK_TRY(mdist.reserve((size_t)std::max<int64_t>(np, 1) * 8));
     if (np) {
@@ -467,7 +472,7 @@ int radius_query_impl(mlk_index* index, const float* Q, int64_t m, double R2, in
         if (nnz)
             MLK_CHECK_HIP(hipMemcpyAsync(res->dist.ptr, mdist.ptr, nnz * sizeof(double), hipMemcpyDeviceToDevice, stream));
     }
-    if (sort_rows) MLK_TRY(sort_rows_by_distance(res, indptr, m, nnz, ws[14], stream));
+    if (sort_rows) MLK_TRY(sort_rows_by_distance(res, indptr, m, nnz, ws, stream));
     MLK_CHECK_HIP(hipGetLastError());
     MLK_CHECK_HIP(hipStreamSynchronize(stream));
     res->nnz = nnz;

@@ -280,13 +280,10 @@ int autotune_impl(mlk_index* index, const float* Q, int64_t m, int k, double R2,
     } else {
         MLK_TRY(indptr.reserve((size_t)(m + 1) * sizeof(int64_t)));
     }
+    Events events;
     hipEvent_t e0, e1;
-    MLK_CHECK_HIP(hipEventCreate(&e0));
-    MLK_CHECK_HIP(hipEventCreate(&e1));
-    struct EvGuard {
-        hipEvent_t a, b;
-        ~EvGuard() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); }
-    } evg{e0, e1};
+    MLK_CHECK_HIP(events.make(&e0));
+    MLK_CHECK_HIP(events.make(&e1));
     struct TrialGuard {
         explicit TrialGuard(const Plan* p) { t_trial = p; }
         ~TrialGuard() { t_trial = nullptr; }
