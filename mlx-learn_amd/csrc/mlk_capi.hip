@@ -24,11 +24,16 @@ Tuning tuning_snapshot() {
     return g_tuning;
 }
 TestHooks& hooks() { return g_hooks; }
+TestHooks hooks_snapshot() {
+    std::lock_guard<std::mutex> g(g_config_lock);
+    return g_hooks;
+}
 mlk_stats& stats() { return g_stats; }
 
 int create_index(const float* X, int64_t n, int d, hipStream_t stream, mlk_index** out);
 
 static int check_fail_hook() {
+    std::lock_guard<std::mutex> g(g_config_lock);
     if (g_hooks.hooks & MLK_HOOK_FAIL) {
         g_hooks.hooks &= ~MLK_HOOK_FAIL;
         set_error("injected failure (test hook)");

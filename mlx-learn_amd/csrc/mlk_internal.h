@@ -99,6 +99,7 @@ struct TestHooks {
     std::vector<int64_t> forced_rows;
 };
 TestHooks& hooks();
+TestHooks hooks_snapshot();  // a copy taken under the configuration lock
 mlk_stats& stats();
 
 }  // namespace mlk

@@ -908,8 +908,9 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
     ScreenFn screen = nullptr;
     Plan plan = resolve_plan(PLAN_KNN, prec, dp, n, m, k);
     screen = pick_screen(SCREEN_LISTS, prec, dp, plan.shape);
-    const bool all_fallback = !screen || k > 64 || mr.n_unsafe > 64 || (hooks().hooks & MLK_HOOK_FORCE_FALLBACK);
-    if (hooks().hooks & MLK_HOOK_CAPACITY) plan.cap = (int)std::max<int64_t>(1, hooks().param);
+    const TestHooks hk = hooks_snapshot();
+    const bool all_fallback = !screen || k > 64 || mr.n_unsafe > 64 || (hk.hooks & MLK_HOOK_FORCE_FALLBACK);
+    if (hk.hooks & MLK_HOOK_CAPACITY) plan.cap = (int)std::max<int64_t>(1, hk.param);
 
     const int BQ = 4 * plan.shape.qt * 32;
     const int64_t QB = (m + BQ - 1) / BQ;
@@ -999,8 +1000,8 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
     }
     MLK_CHECK_HIP(hipGetLastError());
 
-    if ((hooks().hooks & MLK_HOOK_FORCE_UNCERTIFIED) && !hooks().forced_rows.empty()) {
-        const auto& rows = hooks().forced_rows;
+    if ((hk.hooks & MLK_HOOK_FORCE_UNCERTIFIED) && !hk.forced_rows.empty()) {
+        const auto& rows = hk.forced_rows;
         MLK_TRY(ws[7].reserve(rows.size() * sizeof(int64_t)));
         MLK_CHECK_HIP(hipMemcpyAsync(ws[7].ptr, rows.data(), rows.size() * sizeof(int64_t), hipMemcpyHostToDevice,
                                      stream));

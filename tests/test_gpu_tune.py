@@ -153,3 +153,44 @@ def test_mfma_probe_reports_a_plausible_rate(mlk):
     assert 50.0 < out.value < 2600.0
     with pytest.raises(ValueError):
         _lib.check(lib.mlk_mfma_probe(0, 10, ctypes.byref(out), s))
+
+
+def test_concurrent_queries_on_one_index(mlk):
+    """Calls on one index from several host threads serialise on its lock; tuning knobs
+    changed concurrently from another thread never change a result."""
+    import threading
+    torch = pytest.importorskip("torch")
+    from mlxlearn_amd import _lib
+    X, Q = oracle.gen_c2(seed=77, n=30_000, m=600, d=48)
+    want = oracle.knn(X, Q, 6)
+    index = mlk.DeviceNeighborIndex(X)
+    errors, done = [], threading.Event()
+
+    def worker(precision):
+        try:
+            with torch.cuda.stream(torch.cuda.Stream()):
+                for _ in range(6):
+                    d, i = index.query(Q, 6, precision=precision)
+                    torch.cuda.current_stream().synchronize()
+                    np.testing.assert_array_equal(i.cpu().numpy(), want[1])
+        except Exception as e:  # reported below
+            errors.append(e)
+
+    def knobs():
+        s = 0
+        while not done.is_set():
+            _lib.set_tuning(splits=(s % 5) * 8, rounds=[[0.5], [0.1, 0.5], []][s % 3])
+            s += 1
+        _lib.set_tuning()
+
+    threads = [threading.Thread(target=worker, args=(p,)) for p in PRECISIONS]
+    tk = threading.Thread(target=knobs)
+    tk.start()
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    done.set()
+    tk.join()
+    index.close()
+    assert not errors, errors[0]
