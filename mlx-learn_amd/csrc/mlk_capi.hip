@@ -18,12 +18,10 @@ static TestHooks g_hooks;
 static std::mutex g_config_lock;
 
 void set_error(const std::string& msg) { g_error = msg; }
-Tuning& tuning() { return g_tuning; }
 Tuning tuning_snapshot() {
     std::lock_guard<std::mutex> g(g_config_lock);
     return g_tuning;
 }
-TestHooks& hooks() { return g_hooks; }
 TestHooks hooks_snapshot() {
     std::lock_guard<std::mutex> g(g_config_lock);
     return g_hooks;

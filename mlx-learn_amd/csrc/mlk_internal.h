@@ -90,7 +90,6 @@ struct Tuning {
     int stages = 0;
     std::vector<double> rounds;
 };
-Tuning& tuning();
 Tuning tuning_snapshot();  // a copy taken under the configuration lock
 
 struct TestHooks {
@@ -98,7 +97,6 @@ struct TestHooks {
     int64_t param = 0;
     std::vector<int64_t> forced_rows;
 };
-TestHooks& hooks();
 TestHooks hooks_snapshot();  // a copy taken under the configuration lock
 mlk_stats& stats();
 

@@ -19,10 +19,11 @@ from numbers import Integral, Real
 import numpy as np
 from sklearn.base import BaseEstimator, ClusterMixin, TransformerMixin
 from sklearn.utils import check_random_state
-from sklearn.utils.validation import check_is_fitted
+from sklearn.utils.validation import check_is_fitted, validate_data
 
 from .. import _lib
 from .._index import precision_flag, require_gpu, to_device_f32
+from ..neighbors._base import prepare_input, require_dense
 
 __all__ = ["KMeans", "assign_device", "update_device"]
 
@@ -135,6 +136,13 @@ class KMeans(ClusterMixin, TransformerMixin, BaseEstimator):
             raise ValueError(f"{name} must contain finite values representable in float32")
         return arr
 
+    def _checked(self, X, *, reset):
+        """Dense, finite, nonempty float32 rows with scikit-learn's messages; n_features_in_ is set on fit
+        and checked afterwards. Runs before any device call."""
+        require_dense(X, type(self).__name__, "fit" if reset else "predict")
+        X = prepare_input(X)
+        return validate_data(self, X, accept_sparse=False, dtype=np.float32, order="C", reset=reset)
+
     def _initial_centres(self, X, rng):
         n = X.shape[0]
         k = int(self.n_clusters)
@@ -160,9 +168,9 @@ class KMeans(ClusterMixin, TransformerMixin, BaseEstimator):
         if sample_weight is not None:
             raise ValueError("sample_weight is not supported")
         self._validate_parameters()
+        Xh = self._checked(X, reset=True)
         require_gpu()
         torch = _torch()
-        Xh = self._matrix(X)
         n, d = Xh.shape
         k = int(self.n_clusters)
         if k > n:
@@ -221,7 +229,6 @@ class KMeans(ClusterMixin, TransformerMixin, BaseEstimator):
         self.n_iter_ = n_iter
         self.convergence_reason_ = reason
         self.reused_labels_ = int(sum(reused))
-        self.n_features_in_ = d
         self._n_features_out = k
         distinct = np.unique(self.labels_).size
         if distinct < k:
@@ -252,9 +259,8 @@ class KMeans(ClusterMixin, TransformerMixin, BaseEstimator):
 
     def _inference(self, X):
         check_is_fitted(self, "cluster_centers_")
-        Xh = self._matrix(X)
-        if Xh.shape[1] != self.n_features_in_:
-            raise ValueError(f"X has {Xh.shape[1]} features, expected {self.n_features_in_}")
+        Xh = self._checked(X, reset=False)
+        require_gpu()
         return Xh
 
     def predict(self, X):

@@ -97,6 +97,9 @@ class NeighborsMixin:
         if not np.all(np.isfinite(X_checked)):
             raise ValueError("Input X contains infinity or a value too large for dtype('float32').")
         self._index_ = NeighborIndex(X_checked, precision=getattr(self, "precision", "float32"))
+        # Inference writes only inside this fitted container (the backend of the last query and lazily
+        # uploaded label / target tensors), so predict never adds or rebinds an estimator attribute.
+        self._query_state_ = {"backend": None}
         self.n_samples_fit_ = int(X_checked.shape[0])
         self.effective_metric_ = "euclidean"
         self.effective_metric_params_ = {}
@@ -106,6 +109,24 @@ class NeighborsMixin:
     @property
     def _fit_X(self):
         return self._index_.X
+
+    @property
+    def last_query_backend_(self):
+        """Backend of the last query (always ``"gpu"``); AttributeError before the first query."""
+        backend = self.__dict__.get("_query_state_", {}).get("backend")
+        if backend is None:
+            raise AttributeError("last_query_backend_")
+        return backend
+
+    def _set_query_backend(self, backend):
+        self._query_state_["backend"] = backend
+
+    def _device_cache(self, key, make):
+        """A device tensor kept for the fitted state's lifetime (dropped on pickling)."""
+        state = self._query_state_
+        if state.get(key) is None:
+            state[key] = make()
+        return state[key]
 
     def _resolve_n_neighbors(self, n_neighbors, query_is_train: bool) -> int:
         """scikit-learn's messages (_base.py:466-488)."""
@@ -141,7 +162,7 @@ class NeighborsMixin:
         check_is_fitted(self, "_index_")
         Q, k, query_is_train = self._prepare_neighbor_query(X, n_neighbors)
         dist, idx = self._index_.query(Q, k, return_distance=return_distance, exclude_self=query_is_train)
-        self.last_query_backend_ = self._index_.last_query_backend
+        self._set_query_backend(self._index_.last_query_backend)
         return (dist, idx) if return_distance else idx
 
     def kneighbors_graph(self, X=None, n_neighbors=None, mode="connectivity"):
@@ -162,7 +183,10 @@ class NeighborsMixin:
         return sparse.csr_matrix((values, indices.ravel(), indptr), shape=(count, self.n_samples_fit_))
 
     def __getstate__(self):
-        return dict(self.__dict__)
+        state = dict(self.__dict__)
+        if "_query_state_" in state:
+            state["_query_state_"] = {"backend": state["_query_state_"].get("backend")}
+        return state
 
 
 class RadiusMixin:
@@ -181,7 +205,7 @@ class RadiusMixin:
         if sort_results and not return_distance:
             raise ValueError("return_distance must be True if sort_results is True.")
         csr = self._radius_csr(X, radius, need_distance=return_distance, sort_results=sort_results)
-        self.last_query_backend_ = self._index_.last_query_backend
+        self._set_query_backend(self._index_.last_query_backend)
         return csr.to_ragged(return_distance=return_distance)
 
     def radius_neighbors_graph(self, X=None, radius=None, mode="connectivity", sort_results=False):

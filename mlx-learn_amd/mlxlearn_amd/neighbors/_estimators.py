@@ -64,19 +64,11 @@ class KNeighborsClassifier(NeighborsMixin, ClassifierMixin, BaseEstimator):
         self.classes_, encoded = np.unique(y, return_inverse=True)
         self._y = encoded.astype(np.intp, copy=False)
         self.outputs_2d_ = False
-        self._codes_device = None
         return self
 
     def _codes(self):
-        if getattr(self, "_codes_device", None) is None:
-            import torch
-            self._codes_device = torch.as_tensor(self._y.astype(np.int32)).cuda()
-        return self._codes_device
-
-    def __getstate__(self):
-        state = dict(self.__dict__)
-        state["_codes_device"] = None
-        return state
+        import torch
+        return self._device_cache("codes", lambda: torch.as_tensor(self._y.astype(np.int32)).cuda())
 
     def _votes(self, X, probabilities):
         import ctypes
@@ -100,7 +92,7 @@ class KNeighborsClassifier(NeighborsMixin, ClassifierMixin, BaseEstimator):
         _lib.check(lib.mlk_knn_votes(ptr(idx), ptr(dist), m, k, ptr(self._codes()), n_classes,
                                      _lib.WEIGHTS_DISTANCE if need_dist else _lib.WEIGHTS_UNIFORM, ptr(pred),
                                      ptr(proba), stream), "mlk_knn_votes")
-        self.last_query_backend_ = "gpu"
+        self._set_query_backend("gpu")
         if probabilities:
             out = proba.cpu().numpy()
             if not np.isfinite(out).all():
@@ -161,10 +153,10 @@ def _check_weights(weights):
 def _radius_query(est, X):
     """(query rows, exclude_self, R2) of a radius estimator's prediction."""
     check_is_fitted(est, "_index_")
-    require_gpu()
     query_is_train = X is None
     Q = est._index_.X if query_is_train else est._validate_query(X, "predict")
     r = validate_radius(est.radius)
+    require_gpu()
     return Q, query_is_train, r * r
 
 
@@ -199,11 +191,9 @@ class RadiusNeighborsClassifier(NeighborsMixin, RadiusMixin, ClassifierMixin, Ba
         self._y = encoded.astype(np.intp, copy=False)
         self.outputs_2d_ = False
         self.outlier_label_ = _resolve_outlier_label(self.outlier_label, self.classes_, self._y)
-        self._codes_device = None
         return self
 
     _codes = KNeighborsClassifier._codes
-    __getstate__ = KNeighborsClassifier.__getstate__
 
     def _proba(self, X):
         weighted = _check_weights(self.weights)
@@ -213,7 +203,7 @@ class RadiusNeighborsClassifier(NeighborsMixin, RadiusMixin, ClassifierMixin, Ba
         votes, counts = self._index_.device_index.radius_reduce(
             Q, R2, codes=self._codes(), n_classes=len(self.classes_), weighted=weighted,
             exclude_self=query_is_train, precision=self._index_.precision)
-        self.last_query_backend_ = "gpu"
+        self._set_query_backend("gpu")
         votes = votes.cpu().numpy()
         counts = counts.cpu().numpy()
         outliers = np.flatnonzero(counts == 0)
@@ -272,13 +262,7 @@ class RadiusNeighborsRegressor(NeighborsMixin, RadiusMixin, RegressorMixin, Base
         del self._y_checked_
         self._y = np.array(checked, dtype=np.float64, copy=True)
         self._y_out_dtype_ = np.float32 if np.asarray(checked).dtype == np.float32 else np.float64
-        self._targets_device = None
         return self
-
-    def __getstate__(self):
-        state = dict(self.__dict__)
-        state["_targets_device"] = None
-        return state
 
     def predict(self, X):
         """float64 (float32 for float32 targets) means; NaN where no neighbour lies inside ``radius``."""
@@ -287,12 +271,11 @@ class RadiusNeighborsRegressor(NeighborsMixin, RadiusMixin, RegressorMixin, Base
         Q, query_is_train, R2 = _radius_query(self, X)
         if Q.shape[0] == 0:
             return np.empty(0, dtype=self._y_out_dtype_)
-        if getattr(self, "_targets_device", None) is None:
-            self._targets_device = torch.as_tensor(self._y).cuda()
+        targets = self._device_cache("targets", lambda: torch.as_tensor(self._y).cuda())
         pred, counts = self._index_.device_index.radius_reduce(
-            Q, R2, targets=self._targets_device, weighted=weighted, exclude_self=query_is_train,
+            Q, R2, targets=targets, weighted=weighted, exclude_self=query_is_train,
             precision=self._index_.precision)
-        self.last_query_backend_ = "gpu"
+        self._set_query_backend("gpu")
         pred = pred.cpu().numpy()
         if np.any(counts.cpu().numpy() == 0):
             warnings.warn("One or more samples have no neighbors within specified radius; predicting NaN.",

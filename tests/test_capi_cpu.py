@@ -210,3 +210,35 @@ def test_plan_cache_text_round_trip():
             _lib.import_plans(line + bad)
     assert sorted(_lib.export_plans().splitlines()) == sorted([line.strip(), radius.strip()])
     _lib.clear_plans()
+
+
+def test_tags_narrow_to_what_is_supported():
+    """Dense, single-output, no NaN (reference tests/test_sklearn_compliance.py:166-178)."""
+    from sklearn.utils import get_tags
+    import mlxlearn_amd
+    for name in ("NearestNeighbors", "KNeighborsClassifier", "KNeighborsRegressor", "RadiusNeighborsClassifier",
+                 "RadiusNeighborsRegressor", "KMeans"):
+        tags = get_tags(getattr(mlxlearn_amd, name)())
+        assert tags.input_tags.sparse is False and tags.input_tags.allow_nan is False, name
+        if tags.target_tags is not None:
+            assert tags.target_tags.multi_output is False, name
+
+
+def test_kmeans_validates_input_before_the_device():
+    """Sparse, empty, non-finite and wrong-width inputs raise scikit-learn's errors with or without a device."""
+    import scipy.sparse as sp
+    from mlxlearn_amd import KMeans
+    from mlxlearn_amd.exceptions import SparseInputError
+    X = np.random.default_rng(0).normal(size=(20, 3))
+    with pytest.raises(SparseInputError, match="sparse"):
+        KMeans(n_clusters=2).fit(sp.csr_matrix(X))
+    with pytest.raises(ValueError, match="0 sample"):
+        KMeans(n_clusters=2).fit(np.empty((0, 3)))
+    Xn = X.copy()
+    Xn[3, 1] = np.inf
+    with pytest.raises(ValueError, match="infinity"):
+        KMeans(n_clusters=2).fit(Xn)
+    with pytest.raises(ValueError, match="float32"):
+        KMeans(n_clusters=2).fit(np.array([[1e300], [1.0], [2.0]]))
+    with pytest.raises(ValueError, match="sample_weight"):
+        KMeans(n_clusters=2).fit(X, sample_weight=np.ones(20))
