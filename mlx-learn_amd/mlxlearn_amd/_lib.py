@@ -102,6 +102,8 @@ def load(path: str | None = None):
         fn.restype = res
     if path == LIB_PATH:
         _LIB = lib
+        from . import _config
+        _config.apply_tuning()
     return lib
 
 

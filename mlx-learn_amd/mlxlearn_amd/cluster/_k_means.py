@@ -22,6 +22,8 @@ from sklearn.utils import check_random_state
 from sklearn.utils.validation import check_is_fitted, validate_data
 
 from .. import _lib
+from .._config import get_config
+from .._diagnostics import record_event
 from .._index import precision_flag, require_gpu, to_device_f32
 from ..neighbors._base import prepare_input, require_dense
 
@@ -175,7 +177,7 @@ class KMeans(ClusterMixin, TransformerMixin, BaseEstimator):
         k = int(self.n_clusters)
         if k > n:
             raise ValueError(f"n_samples={n} must be >= n_clusters={k}")
-        rng = check_random_state(self.random_state)
+        rng = check_random_state(self.random_state if self.random_state is not None else get_config().random_state)
         explicit = not isinstance(self.init, str)
         if explicit and self.n_init not in ("auto", 1):
             warnings.warn("Explicit initial center position passed: performing only one init instead of "
@@ -230,6 +232,8 @@ class KMeans(ClusterMixin, TransformerMixin, BaseEstimator):
         self.convergence_reason_ = reason
         self.reused_labels_ = int(sum(reused))
         self._n_features_out = k
+        record_event(type(self).__name__, "fit", shape=Xh.shape, n_clusters=k, n_iter=n_iter,
+                     restarts=len(initials), reused_labels=self.reused_labels_, convergence=reason)
         distinct = np.unique(self.labels_).size
         if distinct < k:
             from sklearn.exceptions import ConvergenceWarning
@@ -267,6 +271,7 @@ class KMeans(ClusterMixin, TransformerMixin, BaseEstimator):
         """Exact nearest final centre of every row."""
         Xh = self._inference(X)
         labels, _ = assign_device(Xh, self.cluster_centers_, precision=self.precision)
+        record_event(type(self).__name__, "predict", shape=Xh.shape)
         return labels.cpu().numpy().astype(np.int64)
 
     def transform(self, X):
@@ -280,6 +285,7 @@ class KMeans(ClusterMixin, TransformerMixin, BaseEstimator):
         for s in range(0, Xh.shape[0], rows):
             diff = Xd[s:s + rows, None, :] - Cd[None, :, :]
             out[s:s + rows] = torch.sqrt((diff * diff).sum(-1)).to(torch.float32).cpu().numpy()
+        record_event(type(self).__name__, "transform", shape=Xh.shape, search=False)
         return out
 
     def score(self, X, y=None, sample_weight=None):
@@ -289,6 +295,7 @@ class KMeans(ClusterMixin, TransformerMixin, BaseEstimator):
         Xh = self._inference(X)
         labels, residual = assign_device(Xh, self.cluster_centers_, precision=self.precision)
         _, _, inertia = update_device(Xh, labels, self.cluster_centers_.shape[0], residual)
+        record_event(type(self).__name__, "score", shape=Xh.shape)
         return -float(inertia.item())
 
     def fit_predict(self, X, y=None, sample_weight=None):

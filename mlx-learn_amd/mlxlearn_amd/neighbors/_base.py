@@ -15,6 +15,7 @@ from numbers import Integral
 import numpy as np
 from sklearn.utils.validation import check_is_fitted, validate_data
 
+from .._diagnostics import record_event
 from .._index import NeighborIndex, RadiusCSR
 from ..exceptions import SparseInputError, UnsupportedParameterError
 
@@ -104,6 +105,8 @@ class NeighborsMixin:
         self.effective_metric_ = "euclidean"
         self.effective_metric_params_ = {}
         self._fit_method = "brute"
+        record_event(type(self).__name__, "fit", shape=X_checked.shape, search=False,
+                     detail="training rows owned; the device index is built on the first query")
         return self
 
     @property
@@ -163,6 +166,7 @@ class NeighborsMixin:
         Q, k, query_is_train = self._prepare_neighbor_query(X, n_neighbors)
         dist, idx = self._index_.query(Q, k, return_distance=return_distance, exclude_self=query_is_train)
         self._set_query_backend(self._index_.last_query_backend)
+        record_event(type(self).__name__, "kneighbors", shape=Q.shape, search=Q.shape[0] > 0, k=k)
         return (dist, idx) if return_distance else idx
 
     def kneighbors_graph(self, X=None, n_neighbors=None, mode="connectivity"):
@@ -192,27 +196,31 @@ class NeighborsMixin:
 class RadiusMixin:
     """radius_neighbors / radius_neighbors_graph with exact membership S <= fl64(r**2)."""
 
-    def _radius_csr(self, X, radius, *, need_distance, sort_results) -> RadiusCSR:
+    def _radius_csr(self, X, radius, *, need_distance, sort_results, operation) -> RadiusCSR:
         check_is_fitted(self, "_index_")
         query_is_train = X is None
         Q = self._index_.X if query_is_train else self._validate_query(X, "radius_neighbors")
         radius = self.radius if radius is None else radius
-        return self._index_.query_radius_csr(Q, radius, need_distance=need_distance or sort_results,
-                                             exclude_self=query_is_train, sort_results=sort_results)
+        csr = self._index_.query_radius_csr(Q, radius, need_distance=need_distance or sort_results,
+                                            exclude_self=query_is_train, sort_results=sort_results)
+        self._set_query_backend(self._index_.last_query_backend)
+        record_event(type(self).__name__, operation, shape=Q.shape, search=Q.shape[0] > 0, nnz=csr.nnz)
+        return csr
 
     def radius_neighbors(self, X=None, radius=None, return_distance=True, sort_results=False):
         """Rows within an inclusive radius; object arrays as in scikit-learn (_base.py:634-675)."""
         if sort_results and not return_distance:
             raise ValueError("return_distance must be True if sort_results is True.")
-        csr = self._radius_csr(X, radius, need_distance=return_distance, sort_results=sort_results)
-        self._set_query_backend(self._index_.last_query_backend)
+        csr = self._radius_csr(X, radius, need_distance=return_distance, sort_results=sort_results,
+                               operation="radius_neighbors")
         return csr.to_ragged(return_distance=return_distance)
 
     def radius_neighbors_graph(self, X=None, radius=None, mode="connectivity", sort_results=False):
         if mode not in ("connectivity", "distance"):
             raise ValueError('Unsupported mode, must be one of "connectivity", '
                              f'or "distance" but got "{mode}" instead')
-        csr = self._radius_csr(X, radius, need_distance=mode == "distance", sort_results=sort_results)
+        csr = self._radius_csr(X, radius, need_distance=mode == "distance", sort_results=sort_results,
+                               operation="radius_neighbors_graph")
         data = np.ones(csr.nnz) if mode == "connectivity" else csr.distances
         from scipy import sparse
         return sparse.csr_matrix((data, csr.indices, csr.indptr), shape=(csr.n_query, self.n_samples_fit_))

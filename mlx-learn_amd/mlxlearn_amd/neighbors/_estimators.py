@@ -16,6 +16,7 @@ from sklearn.base import BaseEstimator, ClassifierMixin, RegressorMixin
 from sklearn.utils.validation import check_is_fitted
 
 from .. import _lib
+from .._diagnostics import record_event
 from .._index import require_gpu, validate_radius
 from ._base import NeighborsMixin, RadiusMixin, compute_weights
 
@@ -93,6 +94,7 @@ class KNeighborsClassifier(NeighborsMixin, ClassifierMixin, BaseEstimator):
                                      _lib.WEIGHTS_DISTANCE if need_dist else _lib.WEIGHTS_UNIFORM, ptr(pred),
                                      ptr(proba), stream), "mlk_knn_votes")
         self._set_query_backend("gpu")
+        record_event(type(self).__name__, "predict_proba" if probabilities else "predict", shape=Q.shape, k=k)
         if probabilities:
             out = proba.cpu().numpy()
             if not np.isfinite(out).all():
@@ -204,6 +206,7 @@ class RadiusNeighborsClassifier(NeighborsMixin, RadiusMixin, ClassifierMixin, Ba
             Q, R2, codes=self._codes(), n_classes=len(self.classes_), weighted=weighted,
             exclude_self=query_is_train, precision=self._index_.precision)
         self._set_query_backend("gpu")
+        record_event(type(self).__name__, "predict", shape=Q.shape, weights=self.weights)
         votes = votes.cpu().numpy()
         counts = counts.cpu().numpy()
         outliers = np.flatnonzero(counts == 0)
@@ -276,6 +279,7 @@ class RadiusNeighborsRegressor(NeighborsMixin, RadiusMixin, RegressorMixin, Base
             Q, R2, targets=targets, weighted=weighted, exclude_self=query_is_train,
             precision=self._index_.precision)
         self._set_query_backend("gpu")
+        record_event(type(self).__name__, "predict", shape=Q.shape, weights=self.weights)
         pred = pred.cpu().numpy()
         if np.any(counts.cpu().numpy() == 0):
             warnings.warn("One or more samples have no neighbors within specified radius; predicting NaN.",

@@ -242,3 +242,91 @@ def test_kmeans_validates_input_before_the_device():
         KMeans(n_clusters=2).fit(np.array([[1e300], [1.0], [2.0]]))
     with pytest.raises(ValueError, match="sample_weight"):
         KMeans(n_clusters=2).fit(X, sample_weight=np.ones(20))
+
+
+def test_config_surface_and_scoping():
+    """Six public options, thread-local and scoped (reference _common/config.py:49-117, 278-330)."""
+    import threading
+    import mlxlearn_amd as m
+    from mlxlearn_amd._config import PUBLIC_CONFIG_FIELDS
+    assert PUBLIC_CONFIG_FIELDS == ("device", "fallback_policy", "output_type", "deterministic", "random_state",
+                                    "diagnostics")
+    with pytest.raises(TypeError, match="unknown configuration option"):
+        m.set_config(bogus=1)
+    with pytest.raises(ValueError, match="no CPU route"):
+        m.set_config(device="cpu")
+    with pytest.raises(ValueError, match="not implemented"):
+        m.set_config(output_type="mlx")
+    with pytest.raises(TypeError):
+        m.set_config(diagnostics=1)
+    assert m.get_config().device == "auto"
+    seen = []
+    with pytest.raises(RuntimeError):
+        with m.config_context(device="gpu", random_state=7):
+            assert m.get_config().random_state == 7
+            t = threading.Thread(target=lambda: seen.append(m.get_config().random_state))
+            t.start()
+            t.join()
+            raise RuntimeError("body fails")
+    assert seen == [None] and m.get_config().random_state is None and m.get_config().device == "auto"
+
+
+def test_tuning_environment_overrides(monkeypatch, tmp_path):
+    """MLXLEARN_* variables override tuning; malformed values raise (reference config.py:251-275); a plan file is
+    imported into the native plan cache."""
+    from mlxlearn_amd import _config, _lib
+    try:
+        monkeypatch.setenv("MLXLEARN_SCREEN_SPLITS", "96")
+        monkeypatch.setenv("MLXLEARN_SCREEN_ROUNDS", "0.125,0.5,1")
+        _config.reset_config()
+        t = _config.get_tuning()
+        assert t.screen_splits == 96 and t.rounds() == [0.125, 0.5, 1.0]
+        monkeypatch.setenv("MLXLEARN_SCREEN_SPLITS", "many")
+        with pytest.raises(ValueError, match="MLXLEARN_SCREEN_SPLITS"):
+            _config.reset_config()
+        monkeypatch.setenv("MLXLEARN_SCREEN_SPLITS", "0")
+        monkeypatch.setenv("MLXLEARN_SCREEN_ROUNDS", "0.5,0.25")
+        with pytest.raises(ValueError, match="screen_rounds"):
+            _config.reset_config()
+        monkeypatch.delenv("MLXLEARN_SCREEN_ROUNDS")
+        monkeypatch.setenv("MLXLEARN_QUERY_TILES", "3")
+        with pytest.raises(ValueError, match="query_tiles"):
+            _config.reset_config()
+    finally:
+        for key in ("MLXLEARN_SCREEN_SPLITS", "MLXLEARN_SCREEN_ROUNDS", "MLXLEARN_QUERY_TILES"):
+            monkeypatch.delenv(key, raising=False)
+        _config.reset_config()
+    _lib.load()
+    _lib.clear_plans()
+    line = "plan 0 1 128 10 40 27 1 2 24 48 5 0x1p-8 0x1p-5 0x1p-3 0x1p-1 0x1p+0 0x1.dp+1 0x1.fp+1 16\n"
+    path = tmp_path / "plans.txt"
+    path.write_text(line)
+    with _config.tuning_context(plan_file=str(path)):
+        assert _lib.export_plans().strip() == line.strip()
+    _lib.clear_plans()
+    with pytest.raises(TypeError, match="unknown tuning"):
+        with _config.tuning_context(bogus=1):
+            pass
+
+
+def test_diagnostics_ring_records_fit_and_is_bounded():
+    """Events are recorded while diagnostics are on, in a ring bounded by diagnostics_capacity
+    (reference _common/diagnostics.py:82-200)."""
+    import mlxlearn_amd as m
+    from mlxlearn_amd import _config
+    m.clear_backend_diagnostics()
+    X = np.random.default_rng(1).normal(size=(30, 4))
+    m.NearestNeighbors(n_neighbors=2).fit(X)
+    event = m.get_last_backend_event()
+    assert (event.estimator, event.operation, event.backend, event.shape) == ("NearestNeighbors", "fit", "gpu",
+                                                                             (30, 4))
+    with m.config_context(diagnostics=False):
+        m.NearestNeighbors(n_neighbors=2).fit(X)
+    assert len(m.get_backend_diagnostics()) == 1
+    with _config.tuning_context(diagnostics_capacity=3):
+        for _ in range(5):
+            m.NearestNeighbors(n_neighbors=2).fit(X)
+        assert len(m.get_backend_diagnostics()) == 3
+    assert m.diagnostics_summary() == {"NearestNeighbors.fit -> gpu": 3}
+    m.clear_backend_diagnostics()
+    assert m.get_last_backend_event() is None
