@@ -77,10 +77,15 @@ __device__ __forceinline__ double fast_sq(const float* __restrict__ q, const flo
 // Start threshold: for each query, the k-th smallest certified upper bound of
 // S over up to 256 evenly spaced rows is an upper bound of its k-th smallest
 // S. The block stages the sample rows in LDS, 32 features at a time, with
-// coalesced loads (32 lanes per 128-byte row segment); lane j of each wave
-// owns samples j + 64u (u < U) and takes their features into registers eight
-// at a time while the wave walks its QW queries (wave-uniform query loads), so
-// each staged value serves 4 * QW queries. float64 differences and fused
+// coalesced loads (32 lanes per 128-byte row segment), and each wave stages
+// its QW queries' features beside them; features past d, samples past the
+// sample count and padding queries are staged as zeros (they add exact zeros
+// to the sums), so the arithmetic needs no guards. The global loads of the
+// next 32 features are issued into registers before the current ones are
+// used, so their latency overlaps the arithmetic. Lane j of each wave owns
+// samples j + 64u (u < U) and takes their features into registers eight at a
+// time while it walks the wave's QW queries (broadcast LDS reads), so each
+// staged value serves 4 * QW queries. float64 differences and fused
 // squares-and-adds: any evaluation order is covered by the relative bound
 // g = f64_interval(d). Each wave also initialises its queries' candidate
 // sub-list counters and k-best bounds (screened calls only).
@@ -95,7 +100,10 @@ __global__ __launch_bounds__(256) void k_tau0(const float* __restrict__ X, int64
     constexpr int FL = 32;      // features staged per pass
     constexpr int FC = 8;       // features per register chunk
     constexpr int NS = 64 * U;  // sample slots of the block
+    constexpr int PER = NS * FL / 256;           // staged sample values per thread per pass
+    constexpr int QS = (QW * FL + 63) / 64;      // staged query values per lane per pass
     __shared__ float sx[NS][FL + 1];
+    __shared__ float sq[4][QW][FL];
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int64_t q0 = ((int64_t)blockIdx.x * 4 + w) * QW;
@@ -114,43 +122,73 @@ __global__ __launch_bounds__(256) void k_tau0(const float* __restrict__ X, int64
     }
     const int samples = n < max_samples ? (int)n : max_samples;  // <= NS
     const int64_t stride = n / samples;  // sample j is row j * stride
+    // staging registers: value i of this thread is sample j0 + 8 i, feature e of the
+    // pass (j0 = threadIdx.x / 32, e = threadIdx.x % 32); query value i of this lane is
+    // query qw0 + 2 i of the wave, feature e (qw0 = lane / 32)
+    static_assert(256 % FL == 0 && 64 % FL == 0, "staging layout");
+    float stg[PER], qstg[QS];
+    const int j0 = (int)threadIdx.x / FL, e_st = (int)threadIdx.x % FL, qw0 = lane / FL;
+    const int64_t row_step = (int64_t)(256 / FL) * stride * d;  // eight sample rows
+    const int64_t x_first = (int64_t)j0 * stride * d + e_st;
+    const int64_t q_first = (q0 + qw0) * d + e_st;
+    auto fetch = [&](int f0) {
+        // one running offset per thread, opaque to the compiler at each pass, so the
+        // per-value addresses are formed at their loads instead of being kept live
+        int64_t xo = x_first + f0, qo = q_first + f0;
+        asm volatile("" : "+v"(xo), "+v"(qo));
+        const bool fe = f0 + e_st < d;
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const bool ok = fe && j0 + (256 / FL) * i < samples;
+            stg[i] = X[ok ? xo + i * row_step : 0];  // row 0 stands in for padding (zeroed at the store)
+        }
+#pragma unroll
+        for (int i = 0; i < QS; ++i) {
+            const int qi = qw0 + (64 / FL) * i;
+            const bool ok = fe && qi < live;
+            qstg[i] = Q[ok ? qo + (int64_t)(64 / FL) * i * d : 0];
+        }
+    };
+    auto store = [&](int f0) {
+        const bool fe = f0 + e_st < d;
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int j = j0 + (256 / FL) * i;
+            sx[j][e_st] = (fe && j < samples) ? stg[i] : 0.0f;
+        }
+#pragma unroll
+        for (int i = 0; i < QS; ++i) {
+            const int qi = qw0 + (64 / FL) * i;
+            if (qi < QW) sq[w][qi][e_st] = (fe && qi < live) ? qstg[i] : 0.0f;
+        }
+    };
     double acc[QW][U];
 #pragma unroll
     for (int qi = 0; qi < QW; ++qi)
 #pragma unroll
         for (int u = 0; u < U; ++u) acc[qi][u] = 0.0;
+    fetch(0);
     for (int f0 = 0; f0 < d; f0 += FL) {
+        __syncthreads();  // every read of the previous pass is done
+        store(f0);
         __syncthreads();
-        // stage features [f0, f0 + FL) of every sample row (zeros past d / samples)
-        for (int idx = threadIdx.x; idx < NS * FL; idx += 256) {
-            const int j = idx / FL, e = idx % FL;
-            float v = 0.0f;
-            if (j < samples && f0 + e < d) v = X[(int64_t)j * stride * d + f0 + e];
-            sx[j][e] = v;
-        }
-        __syncthreads();
-        const int fl = d - f0 < FL ? d - f0 : FL;
-        for (int c0 = 0; c0 < fl; c0 += FC) {
+        if (f0 + FL < d) fetch(f0 + FL);  // in flight during this pass
+#pragma unroll 1
+        for (int c0 = 0; c0 < FL; c0 += FC) {
             double sv[U][FC];
 #pragma unroll
             for (int u = 0; u < U; ++u)
 #pragma unroll
                 for (int e = 0; e < FC; ++e) sv[u][e] = (double)sx[lane + 64 * u][c0 + e];
-            const int fe = fl - c0 < FC ? fl - c0 : FC;
 #pragma unroll
             for (int qi = 0; qi < QW; ++qi) {
-                if (qi < live) {
-                    const float* qv = Q + (q0 + qi) * d + f0 + c0;
 #pragma unroll
-                    for (int e = 0; e < FC; ++e) {
-                        if (e < fe) {
-                            const double dq = (double)qv[e];
+                for (int e = 0; e < FC; ++e) {
+                    const double dq = (double)sq[w][qi][c0 + e];
 #pragma unroll
-                            for (int u = 0; u < U; ++u) {
-                                const double t = dq - sv[u][e];
-                                acc[qi][u] = fma(t, t, acc[qi][u]);
-                            }
-                        }
+                    for (int u = 0; u < U; ++u) {
+                        const double t = dq - sv[u][e];
+                        acc[qi][u] = fma(t, t, acc[qi][u]);
                     }
                 }
             }
