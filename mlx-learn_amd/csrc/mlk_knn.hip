@@ -305,13 +305,13 @@ __global__ void k_merge(int64_t m, int k, const int* __restrict__ cnt, int* __re
         }
         int at = base + incl - mine;
         const double nqq = nq[q];
-        const size_t off = ((size_t)q * S2 + s) * cap;
+        const size_t off = (size_t)q * cap * S2 + s;  // entry j at off + j * S2
         for (int j0 = lo; j0 < hi; j0 += 8) {
             // eight entries per batch: loads and norm gathers issued together
             int2 e[8];
             float nxv[8];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) e[u] = cand[off + (j0 + u < hi ? j0 + u : hi - 1)];
+            for (int u = 0; u < 8; ++u) e[u] = cand[off + (size_t)(j0 + u < hi ? j0 + u : hi - 1) * S2];
 #pragma unroll
             for (int u = 0; u < 8; ++u) nxv[u] = nx32[e[u].x];
 #pragma unroll
@@ -379,9 +379,9 @@ __global__ void k_merge_lane(int64_t m, int k, const int* __restrict__ cnt, int*
         const int hi = cs < cap ? cs : cap;
         const int lo = seen[q * S2 + s];
         seen[q * S2 + s] = hi;
-        const size_t off = ((size_t)q * S2 + s) * cap;
+        const size_t off = (size_t)q * cap * S2 + s;  // entry j at off + j * S2
         for (int j = lo; j < hi; ++j) {
-            const int2 e = cand[off + j];
+            const int2 e = cand[off + (size_t)j * S2];
             double L, U;
             hit_bounds(__int_as_float(e.y), nx32[e.x], nqq, c, L, U);
             insert(fmaxf(__double2float_ru(U), 0.0f));
@@ -498,7 +498,7 @@ __global__ __launch_bounds__(256) void k_rank(const RankArgs a, int64_t n_items)
                 cs = cs < a.cap ? cs : a.cap;
             }
             listed += cs;
-            const size_t off = ((size_t)q * a.S + s) * a.cap;
+            const size_t off = (size_t)q * a.cap * a.S + s;  // entry j at off + j * S
             // eight entries at a time: their loads and norm gathers are issued
             // together, so a batch costs one memory latency per step
             for (int j0 = 0; j0 < cs; j0 += 8) {
@@ -507,7 +507,7 @@ __global__ __launch_bounds__(256) void k_rank(const RankArgs a, int64_t n_items)
                 int2 e[8];
                 float nxv[8];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) e[u] = a.cand[off + (j0 + u < cs ? j0 + u : cs - 1)];
+                for (int u = 0; u < 8; ++u) e[u] = a.cand[off + (size_t)(j0 + u < cs ? j0 + u : cs - 1) * a.S];
 #pragma unroll
                 for (int u = 0; u < 8; ++u) nxv[u] = a.nx32[e[u].x];
 #pragma unroll
@@ -753,9 +753,9 @@ __global__ __launch_bounds__(256) void k_rank1_lane(const RankArgs a, int64_t m)
         over |= cs > a.cap;
         cs = cs < a.cap ? cs : a.cap;
         listed += cs;
-        const int2* lst = a.cand + ((size_t)q * a.S + sl) * a.cap;
+        const int2* lst = a.cand + (size_t)q * a.cap * a.S + sl;  // entry j at lst[j * S]
         for (int j = 0; j < cs; ++j) {
-            const int2 e = lst[j];
+            const int2 e = lst[(size_t)j * a.S];
             double L, U;
             hit_bounds(__int_as_float(e.y), a.nx32[e.x], nqq, a.c, L, U);
             if (!(L <= (double)tau)) continue;

@@ -165,9 +165,13 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
     if constexpr (MODE == SCREEN_LISTS) {
 #pragma unroll
         for (int qt = 0; qt < QT; ++qt) {
-            const size_t sub = (((size_t)(qrow0 + qt * 32 + col)) * a.S + split) * 2 + h;
+            const size_t q = (size_t)(qrow0 + qt * 32 + col);
+            const size_t sub = (q * a.S + split) * 2 + h;
             cnt_r[qt] = a.cnt[sub];
-            list_base[qt] = sub * a.cap;
+            // entry j of sub-list s of query q lives at (q * cap + j) * 2S + s: the j-th
+            // entries of a query's sub-lists are adjacent, so the list walks that visit
+            // every sub-list of a query read them with coalesced loads
+            list_base[qt] = q * a.cap * (size_t)(2 * a.S) + (size_t)split * 2 + h;
         }
     }
     bool live = false;
@@ -346,7 +350,8 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
                             if (v >= thr[qt] && row < r_end && row != self) {
                                 const int slot = cnt_r[qt]++;
                                 if (slot < a.cap)
-                                    a.cand[list_base[qt] + slot] = int2{(int)row, __float_as_int(v - cB[rl])};
+                                    a.cand[list_base[qt] + (size_t)slot * (2 * a.S)] =
+                                        int2{(int)row, __float_as_int(v - cB[rl])};
                             }
                         }
                     } else {
