@@ -787,6 +787,116 @@ __global__ __launch_bounds__(256) void k_rank1_lane(const RankArgs a, int64_t m)
     }
 }
 
+// Rank for k = 1 with a group of eight lanes per query (rows of at most
+// 32 * NF4 features, d % 4 == 0, 16-byte aligned rows). The group holds its
+// query row in registers (lane gl owns the 16-byte chunks gl + 8 t), takes the
+// list entries round-robin for the certified-bound filter, and forms each
+// surviving candidate's float64 distance from coalesced 16-byte row segments,
+// reduced over the group in any order (covered by the order-free bound g).
+// The exact (S, index) comparison and the canonical distance of the result
+// are those of k_rank1_lane, so results and statistics are identical.
+template <int NF4>
+__global__ __launch_bounds__(256) void k_rank1_group(const RankArgs a, int64_t m) {
+    constexpr int G = 8;
+    const int lane = threadIdx.x & 63, gl = lane % G, gbase = lane - gl;
+    const int64_t q = ((int64_t)blockIdx.x * 256 + threadIdx.x) / G;
+    if (q >= m) return;  // whole groups leave together (G divides the block)
+    if (a.status[q]) {   // handled by the fallback
+        if (gl == 0 && a.qstats) a.qstats[3 * q] = a.qstats[3 * q + 1] = a.qstats[3 * q + 2] = 0;
+        return;
+    }
+    const int d = a.d;
+    const double g = f64_interval(d);
+    const float* qv = a.Q + q * d;
+    f32x4 qr[NF4];
+#pragma unroll
+    for (int t = 0; t < NF4; ++t) {
+        const int f = (t * G + gl) * 4;
+        qr[t] = f < d ? *(const f32x4*)(qv + f) : f32x4{0.f, 0.f, 0.f, 0.f};
+    }
+    const float tau = a.tau[q];
+    const double nqq = a.nq[q];
+    const double cut = (double)tau / a.c.scale2;
+    int listed = 0, count = 0, kept = 0;
+    bool over = false;
+    double best_s = INFINITY;
+    long long best_i = -1;
+    for (int sl = 0; sl < a.S; ++sl) {
+        int cs = a.cnt[q * a.S + sl];
+        over |= cs > a.cap;
+        cs = cs < a.cap ? cs : a.cap;
+        listed += cs;
+        const int2* lst = a.cand + (size_t)q * a.cap * a.S + sl;  // entry j at lst[j * S]
+        // 4 G entries per step: their loads, then their norm gathers, are issued together
+        for (int j0 = 0; j0 < cs; j0 += 4 * G) {
+            int2 e[4];
+            float nxv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int j = j0 + gl + G * u;
+                e[u] = lst[(size_t)(j < cs ? j : cs - 1) * a.S];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) nxv[u] = a.nx32[e[u].x];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                bool pass = false;
+                if (j0 + gl + G * u < cs) {
+                    double L, U;
+                    hit_bounds(__int_as_float(e[u].y), nxv[u], nqq, a.c, L, U);
+                    pass = L <= (double)tau;
+                }
+                uint32_t mask = (uint32_t)(ballot(pass) >> gbase) & ((1u << G) - 1);  // uniform in the group
+                count += __popc(mask);
+                while (mask) {
+                    const int b = __builtin_ctz(mask);
+                    mask &= mask - 1;
+                    const int row = __shfl(e[u].x, gbase + b);
+                    const float* xv = a.X + (size_t)row * d;
+                    double part = 0.0;
+#pragma unroll
+                    for (int t = 0; t < NF4; ++t) {
+                        const int f = (t * G + gl) * 4;
+                        if (f < d) {
+                            const f32x4 x4 = *(const f32x4*)(xv + f);
+#pragma unroll
+                            for (int c = 0; c < 4; ++c) {
+                                const double dt = (double)qr[t][c] - (double)x4[c];
+                                part = fma(dt, dt, part);
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int o = G / 2; o > 0; o >>= 1) part += __shfl_xor(part, o, G);
+                    const double sj = part;
+                    if (!(sj * (1.0 - g) <= cut)) continue;
+                    ++kept;
+                    if (best_i < 0 || exact_precedes(sj, row, best_s, best_i, g, qv, a.X, d)) {
+                        best_s = sj;
+                        best_i = row;
+                    }
+                }
+            }
+        }
+    }
+    if (gl != 0) return;
+    if (over || kept < 1) {
+        a.status[q] = ST_OVERFLOW;
+        if (a.qstats) a.qstats[3 * q] = a.qstats[3 * q + 1] = a.qstats[3 * q + 2] = 0;
+        return;
+    }
+    if (a.qstats) {
+        a.qstats[3 * q] = listed;
+        a.qstats[3 * q + 1] = count;
+        a.qstats[3 * q + 2] = kept;
+    }
+    a.out_idx[q] = best_i;
+    if (a.out_dist) {
+        const double sc = canonical_sq4(qv, a.X + (size_t)best_i * d, d);
+        a.out_dist[q] = a.squared ? sc : sqrt(sc);
+    }
+}
+
 // ---------------------------------------------------------------- fallback
 __global__ void k_fb_dist(const float* __restrict__ X, int64_t n, int d, const float* __restrict__ Q,
                           const int* __restrict__ fq, int64_t self_off, int exclude, double* __restrict__ s_out,
@@ -1119,7 +1229,12 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
         MLK_TRY(ws[19].reserve((size_t)m * 3 * sizeof(int)));
         ra.qstats = ws[19].as<int>();  // written for every query by k_rank
         const bool vec = d % 4 == 0 && ((uintptr_t)Q % 16) == 0 && ((uintptr_t)X % 16) == 0;
-        if (k == 1 && mr.n_unsafe == 0)
+        if (k == 1 && mr.n_unsafe == 0 && vec && d <= 128)
+            hipLaunchKernelGGL(d <= 64 ? k_rank1_group<2> : k_rank1_group<4>, dim3((unsigned)((m + 31) / 32)),
+                               dim3(256), 0, stream, ra, m);
+        else if (k == 1 && mr.n_unsafe == 0 && vec && d <= 256)
+            hipLaunchKernelGGL(k_rank1_group<8>, dim3((unsigned)((m + 31) / 32)), dim3(256), 0, stream, ra, m);
+        else if (k == 1 && mr.n_unsafe == 0)
             hipLaunchKernelGGL(vec ? k_rank1_lane<true> : k_rank1_lane<false>, dim3((unsigned)((m + 255) / 256)),
                                dim3(256), 0, stream, ra, m);
         else

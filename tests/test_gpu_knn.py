@@ -284,3 +284,22 @@ def test_wide_pixels_exact_screen(mlk):
     X, y, Q = oracle.gen_c3(seed=13, n=8000, m=400)
     for p in PRECISIONS:
         assert_parity(gpu_knn(mlk, X, Q, 3, p), oracle.knn(X, Q, 3))
+
+
+@pytest.mark.parametrize("d", [4, 64, 100, 200, 256])
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_nearest_row_with_ties_at_group_widths(mlk, precision, d):
+    """k = 1 with rows duplicated and queries equal to rows: the lowest index of a tied set wins at every row
+    width the one-query-per-lane-group rank handles (d % 4 == 0 up to 256), and just inside its limits."""
+    r = np.random.default_rng(4100 + d)
+    X = r.normal(size=(3000, d)).astype(np.float32)
+    X[2000:2010] = X[17]          # ten later copies of row 17
+    X[2500] = X[1999]              # one later copy of row 1999
+    Q = r.normal(size=(700, d)).astype(np.float32)
+    Q[:5] = X[17]
+    Q[5:9] = X[1999]
+    Q[9:20] = X[2000:2011]
+    want = oracle.knn(X, Q, 1)
+    got = gpu_knn(mlk, X, Q, 1, precision)
+    assert_parity(got, want)
+    assert (got[1][:5, 0] == 17).all() and (got[1][5:9, 0] == 1999).all()
