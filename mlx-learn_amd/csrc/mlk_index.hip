@@ -100,39 +100,71 @@ __device__ __forceinline__ typename ScreenType<PREC>::T to_screen(double v) {
 
 // One wave per row: screen mirror, float64 norm, safe flag, exactness flag.
 // Used for database rows (flags_out per row) and for query rows.
-template <int PREC>
-__global__ void k_mirror(const float* __restrict__ X, int64_t n, int64_t n_pad, int d, int dp,
-                         const float* __restrict__ c, double scale, typename ScreenType<PREC>::T* __restrict__ xs,
-                         double* __restrict__ norms, uint8_t* __restrict__ flags, unsigned* __restrict__ inexact) {
-    typedef typename ScreenType<PREC>::T T;
-    const int lane = threadIdx.x & 63;
-    const int64_t row = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (row >= n_pad) return;
-    // pass 1: norm, safe range, exactness; pass 2: the mirror values (zero for
-    // rows outside the safe range, whose screen products must stay finite)
+template <int PREC, int R, int T>
+__global__ __launch_bounds__(256) void k_mirror(const float* __restrict__ X, int64_t n, int64_t n_pad, int d, int dp,
+                                                const float* __restrict__ c, double scale,
+                                                typename ScreenType<PREC>::T* __restrict__ xs,
+                                                double* __restrict__ norms, uint8_t* __restrict__ flags,
+                                                unsigned* __restrict__ inexact) {
+    // R lanes per row (64 / R rows per wave); lane r holds features r + R t (t < T,
+    // T R >= dp) as float64 between the two passes, so every row is read once
+    typedef typename ScreenType<PREC>::T T_;
+    const int lane = threadIdx.x & 63, r = lane % R;
+    const int64_t row = ((int64_t)blockIdx.x * 256 + threadIdx.x) / R;
+    if (row >= n_pad) return;  // whole row groups leave together
+    const bool real = row < n;
+    // pass 1: norm (float64, any order), safe range, exactness; pass 2: the mirror
+    // values (zero for rows outside the safe range, whose screen products must stay finite)
     double s = 0.0;
     bool safe = true, exact = true;
-    for (int j = lane; j < d && row < n; j += 64) {
-        const double v = ((double)X[row * d + j] - (double)c[j]) * scale;
-        s += v * v;
-        safe &= safe_value<PREC>(v);
-        exact &= (double)(float)to_screen<PREC>(v) == v;
+    if constexpr (T == 0) {  // any width: two reads of the row, 64 lanes
+        static_assert(R == 64, "looping mirror uses whole waves");
+        for (int j = r; j < d && real; j += R) {
+            const double x = ((double)X[row * d + j] - (double)c[j]) * scale;
+            s += x * x;
+            safe &= safe_value<PREC>(x);
+            exact &= (double)(float)to_screen<PREC>(x) == x;
+        }
+        for (int o = R / 2; o > 0; o >>= 1) {
+            s += __shfl_xor(s, o, R);
+            safe = __shfl_xor((int)safe, o, R) & safe;
+            exact = __shfl_xor((int)exact, o, R) & exact;
+        }
+        for (int j = r; j < dp; j += R) {
+            T_ out = (T_)0.0f;
+            if (real && j < d && safe) out = to_screen<PREC>(((double)X[row * d + j] - (double)c[j]) * scale);
+            xs[row * dp + j] = out;
+        }
+    } else {
+    double v[T];
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+        const int j = r + R * t;
+        v[t] = 0.0;
+        if (real && j < d) {
+            v[t] = ((double)X[row * d + j] - (double)c[j]) * scale;
+            s += v[t] * v[t];
+            safe &= safe_value<PREC>(v[t]);
+            exact &= (double)(float)to_screen<PREC>(v[t]) == v[t];
+        }
     }
-    for (int o = 32; o > 0; o >>= 1) {
-        s += __shfl_xor(s, o);
-        safe = __shfl_xor((int)safe, o) & safe;
-        exact = __shfl_xor((int)exact, o) & exact;
+#pragma unroll
+    for (int o = R / 2; o > 0; o >>= 1) {
+        s += __shfl_xor(s, o, R);
+        safe = __shfl_xor((int)safe, o, R) & safe;
+        exact = __shfl_xor((int)exact, o, R) & exact;
     }
-    for (int j = lane; j < dp; j += 64) {
-        T out = (T)0.0f;
-        if (row < n && j < d && safe) out = to_screen<PREC>(((double)X[row * d + j] - (double)c[j]) * scale);
-        xs[row * dp + j] = out;
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+        const int j = r + R * t;
+        if (j < dp) xs[row * dp + j] = (real && j < d && safe) ? to_screen<PREC>(v[t]) : (T_)0.0f;
     }
-    if (lane == 0) {
-        norms[row] = row < n ? s : 0.0;
-        flags[row] = (row < n && safe) ? 1 : 0;
+    }
+    if (r == 0) {
+        norms[row] = real ? s : 0.0;
+        flags[row] = (real && safe) ? 1 : 0;
         // one flag for the whole mirror: skip the (same-address) atomic once it is set
-        if (row < n && safe && !exact && __hip_atomic_load(inexact, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u)
+        if (real && safe && !exact && __hip_atomic_load(inexact, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u)
             atomicOr(inexact, 1u);
     }
 }
@@ -164,10 +196,19 @@ template <int PREC>
 static void launch_mirror(const float* X, int64_t n, int64_t n_pad, int d, int dp, const float* c,
                           double scale, void* xs, double* norms, uint8_t* flags, unsigned* inexact,
                           hipStream_t stream) {
-    const int rows_per_block = 4;
-    dim3 grid((unsigned)((n_pad + rows_per_block - 1) / rows_per_block));
-    hipLaunchKernelGGL(k_mirror<PREC>, grid, dim3(256), 0, stream, X, n, n_pad, d, dp, c, scale,
-                       (typename ScreenType<PREC>::T*)xs, norms, flags, inexact);
+    typedef typename ScreenType<PREC>::T T_;
+    auto go = [&](auto kern, int lanes) {
+        const int64_t rows_per_block = 256 / lanes;
+        dim3 grid((unsigned)((n_pad + rows_per_block - 1) / rows_per_block));
+        hipLaunchKernelGGL(kern, grid, dim3(256), 0, stream, X, n, n_pad, d, dp, c, scale, (T_*)xs, norms, flags,
+                           inexact);
+    };
+    // lanes per row: about four features per lane, at least 16 lanes
+    if (dp <= 64) go(k_mirror<PREC, 16, 4>, 16);
+    else if (dp <= 128) go(k_mirror<PREC, 32, 4>, 32);
+    else if (dp <= 256) go(k_mirror<PREC, 64, 4>, 64);
+    else if (dp <= 832) go(k_mirror<PREC, 64, 13>, 64);
+    else go(k_mirror<PREC, 64, 0>, 64);  // wider: the row is read twice
 }
 
 int mirror_rows(int prec, const float* X, int64_t n, int64_t n_pad, int d, int dp, const float* c,
