@@ -103,11 +103,17 @@ __global__ void k_pair_keys(const int2* __restrict__ pairs, int64_t np, int rb, 
     if (i < np) keys[i] = ((unsigned long long)(unsigned)pairs[i].x << rb) | (unsigned)pairs[i].y;
 }
 
+// A classified pair: its (query << rb) | row key and its distance.
+struct KeyDist {
+    unsigned long long key;
+    double dist;
+};
+
 // VEC: d % 4 == 0 and 16-byte aligned rows (four features per load).
 template <bool VEC>
 __global__ void k_classify(const unsigned long long* __restrict__ keys, int64_t np, int rb, const float* __restrict__ X,
                            const float* __restrict__ Q, int d, RadiusConsts rc, uint8_t* __restrict__ member,
-                           double* __restrict__ dist) {
+                           KeyDist* __restrict__ kd) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
     const unsigned long long key = keys[i];
@@ -121,12 +127,12 @@ __global__ void k_classify(const unsigned long long* __restrict__ keys, int64_t 
     else if (!(s <= rc.f_out)) in = false;
     else in = exact_cmp_key(qv, xv, d, rc.key) <= 0;
     member[i] = in;
-    dist[i] = sqrt(s < rc.R2 ? s : rc.R2);
+    kd[i] = KeyDist{key, sqrt(s < rc.R2 ? s : rc.R2)};
 }
 
 // CSR row pointers from the sorted member keys: indptr[q] is the first key
 // whose query part is >= q (binary search; indptr[m] = nnz).
-__global__ void k_indptr_search(const unsigned long long* __restrict__ keys, int64_t nnz, int64_t m, int rb,
+__global__ void k_indptr_search(const KeyDist* __restrict__ kd, int64_t nnz, int64_t m, int rb,
                                 int64_t* __restrict__ indptr) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q > m) return;
@@ -134,25 +140,19 @@ __global__ void k_indptr_search(const unsigned long long* __restrict__ keys, int
     int64_t lo = 0, hi = nnz;
     while (lo < hi) {
         const int64_t mid = lo + ((hi - lo) >> 1);
-        if (keys[mid] < target) lo = mid + 1; else hi = mid;
+        if (kd[mid].key < target) lo = mid + 1; else hi = mid;
     }
     indptr[q] = lo;
 }
 
-__global__ void k_key_to_index(const unsigned long long* __restrict__ keys, int64_t nnz, int rb,
-                               int64_t* __restrict__ idx) {
+// Member records -> CSR columns (row part of the key) and, if kept, distances.
+__global__ void k_split_members(const KeyDist* __restrict__ kd, int64_t nnz, int rb, int64_t* __restrict__ idx,
+                                double* __restrict__ dist) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nnz) idx[i] = (int64_t)(keys[i] & ((1ull << rb) - 1));
-}
-
-__global__ void k_dist_bits(const double* __restrict__ dist, int64_t nnz, unsigned long long* __restrict__ bits) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nnz) bits[i] = (unsigned long long)__double_as_longlong(dist[i]);
-}
-
-__global__ void k_bits_dist(const unsigned long long* __restrict__ bits, int64_t nnz, double* __restrict__ dist) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nnz) dist[i] = __longlong_as_double((long long)bits[i]);
+    if (i >= nnz) return;
+    const KeyDist r = kd[i];
+    idx[i] = (int64_t)(r.key & ((1ull << rb) - 1));
+    if (dist) dist[i] = r.dist;
 }
 
 // All rows (except self) are members: R2 >= 2^342 or +inf.
@@ -233,34 +233,30 @@ __global__ void k_radius_mean(const int64_t* __restrict__ indptr, int64_t m, con
     pred[q] = e == b ? NAN : num / den;
 }
 
-// Stable segmented sort of every CSR row by its distance bits (non-negative
-// doubles order like their bit patterns); rows arrive in ascending index
-// order, so equal distances stay in index order.
-// Temporaries live in the index workspace (ws[14] sort scratch, ws[18..20]
-// keys and indices); the sorted indices are swapped into the result.
+// Stable segmented sort of every CSR row by its distance (non-negative
+// doubles, which the radix sort orders numerically); rows arrive in ascending
+// index order, so equal distances stay in index order.
+// Temporaries live in the index workspace (ws[14] sort scratch, ws[19..20]
+// sorted distances and indices); both are swapped into the result.
 static int sort_rows_by_distance(mlk_radius_result* res, const int64_t* indptr, int64_t m, int64_t nnz, DevBuf* ws,
                                  hipStream_t stream) {
     if (!nnz) return MLK_OK;
     DevBuf& tmp = ws[14];
-    DevBuf& bits_in = ws[18];
-    DevBuf& bits_out = ws[19];
+    DevBuf& dist_out = ws[19];
     DevBuf& idx_out = ws[20];
-    MLK_TRY(bits_in.reserve((size_t)nnz * 8));
-    MLK_TRY(bits_out.reserve((size_t)nnz * 8));
+    MLK_TRY(dist_out.reserve((size_t)nnz * 8));
     MLK_TRY(idx_out.reserve((size_t)nnz * 8));
-    hipLaunchKernelGGL(k_dist_bits, dim3(blocks_for(nnz)), dim3(256), 0, stream, res->dist.as<double>(), nnz,
-                       bits_in.as<unsigned long long>());
     size_t tmp_bytes = 0;
     MLK_CHECK_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(
-        nullptr, tmp_bytes, bits_in.as<unsigned long long>(), bits_out.as<unsigned long long>(),
-        res->indices.as<int64_t>(), idx_out.as<int64_t>(), (int)nnz, (int)m, indptr, indptr + 1, 0, 64, stream));
+        nullptr, tmp_bytes, res->dist.as<double>(), dist_out.as<double>(), res->indices.as<int64_t>(),
+        idx_out.as<int64_t>(), (int)nnz, (int)m, indptr, indptr + 1, 0, 64, stream));
     MLK_TRY(tmp.reserve(tmp_bytes + 16));
     MLK_CHECK_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(
-        tmp.ptr, tmp_bytes, bits_in.as<unsigned long long>(), bits_out.as<unsigned long long>(),
-        res->indices.as<int64_t>(), idx_out.as<int64_t>(), (int)nnz, (int)m, indptr, indptr + 1, 0, 64, stream));
-    hipLaunchKernelGGL(k_bits_dist, dim3(blocks_for(nnz)), dim3(256), 0, stream, bits_out.as<unsigned long long>(),
-                       nnz, res->dist.as<double>());
+        tmp.ptr, tmp_bytes, res->dist.as<double>(), dist_out.as<double>(), res->indices.as<int64_t>(),
+        idx_out.as<int64_t>(), (int)nnz, (int)m, indptr, indptr + 1, 0, 64, stream));
     MLK_CHECK_HIP(hipGetLastError());
+    std::swap(res->dist.ptr, dist_out.ptr);
+    std::swap(res->dist.bytes, dist_out.bytes);
     // the sorted indices become the result's; its unsorted buffer becomes workspace
     std::swap(res->indices.ptr, idx_out.ptr);
     std::swap(res->indices.bytes, idx_out.bytes);
@@ -417,7 +413,7 @@ int radius_query_impl(mlk_index* index, const float* Q, int64_t m, double R2, in
     while ((1ll << rb) < n) ++rb;
     MLK_TRY(ws[10].reserve((size_t)std::max<int64_t>(np, 1) * 8));   // keys in
     MLK_TRY(ws[11].reserve((size_t)std::max<int64_t>(np, 1) * 8));   // keys sorted
-    MLK_TRY(ws[12].reserve((size_t)std::max<int64_t>(np, 1) * 8));   // dist
+    MLK_TRY(ws[12].reserve((size_t)std::max<int64_t>(np, 1) * sizeof(KeyDist)));  // (key, distance) records
     MLK_TRY(ws[13].reserve((size_t)std::max<int64_t>(np, 1) + 16));  // member flags
     unsigned long long* keys = ws[10].as<unsigned long long>();
     unsigned long long* skeys = ws[11].as<unsigned long long>();
@@ -431,47 +427,36 @@ int radius_query_impl(mlk_index* index, const float* Q, int64_t m, double R2, in
         MLK_CHECK_HIP(hipcub::DeviceRadixSort::SortKeys(ws[14].ptr, tmp_bytes, keys, skeys, (int)np, 0, rb + qbits, stream));
         const bool vec = d % 4 == 0 && ((uintptr_t)Q % 16) == 0 && ((uintptr_t)X % 16) == 0;
         hipLaunchKernelGGL(vec ? k_classify<true> : k_classify<false>, dim3(blocks_for(np)), dim3(256), 0, stream,
-                           skeys, np, rb, X, Q, d, rc, ws[13].as<uint8_t>(), ws[12].as<double>());
+                           skeys, np, rb, X, Q, d, rc, ws[13].as<uint8_t>(), ws[12].as<KeyDist>());
         MLK_CHECK_HIP(hipGetLastError());
     }
     // compact members (order preserved)
     MLK_TRY(ws[15].reserve(64));
     int64_t* dnnz = ws[15].as<int64_t>();
     MLK_CHECK_HIP(hipMemsetAsync(dnnz, 0, sizeof(int64_t), stream));
-    DevBuf& mkeys = ws[16];  // member keys and distances (workspace, grown on demand)
-    DevBuf& mdist = ws[17];
-    MLK_TRY(mkeys.reserve((size_t)std::max<int64_t>(np, 1) * 8));
-    MLK_TRY(mdist.reserve((size_t)std::max<int64_t>(np, 1) * 8));
+    DevBuf& members = ws[16];  // member records (workspace, grown on demand)
+    MLK_TRY(members.reserve((size_t)std::max<int64_t>(np, 1) * sizeof(KeyDist)));
     if (np) {
         size_t tmp_bytes = 0;
-        MLK_CHECK_HIP(hipcub::DeviceSelect::Flagged(nullptr, tmp_bytes, skeys, ws[13].as<uint8_t>(),
-                                                    mkeys.as<unsigned long long>(), dnnz, (int)np, stream));
+        MLK_CHECK_HIP(hipcub::DeviceSelect::Flagged(nullptr, tmp_bytes, ws[12].as<KeyDist>(), ws[13].as<uint8_t>(),
+                                                    members.as<KeyDist>(), dnnz, (int)np, stream));
         MLK_TRY(ws[14].reserve(tmp_bytes + 16));
-        MLK_CHECK_HIP(hipcub::DeviceSelect::Flagged(ws[14].ptr, tmp_bytes, skeys, ws[13].as<uint8_t>(),
-                                                    mkeys.as<unsigned long long>(), dnnz, (int)np, stream));
-        MLK_CHECK_HIP(hipcub::DeviceSelect::Flagged(nullptr, tmp_bytes, ws[12].as<double>(), ws[13].as<uint8_t>(),
-                                                    mdist.as<double>(), dnnz, (int)np, stream));
-        MLK_TRY(ws[14].reserve(tmp_bytes + 16));
-        MLK_CHECK_HIP(hipcub::DeviceSelect::Flagged(ws[14].ptr, tmp_bytes, ws[12].as<double>(), ws[13].as<uint8_t>(),
-                                                    mdist.as<double>(), dnnz, (int)np, stream));
+        MLK_CHECK_HIP(hipcub::DeviceSelect::Flagged(ws[14].ptr, tmp_bytes, ws[12].as<KeyDist>(), ws[13].as<uint8_t>(),
+                                                    members.as<KeyDist>(), dnnz, (int)np, stream));
     }
     int64_t nnz = 0;
     MLK_CHECK_HIP(hipMemcpyAsync(&nnz, dnnz, sizeof(int64_t), hipMemcpyDeviceToHost, stream));
     MLK_CHECK_HIP(hipStreamSynchronize(stream));
 
     // CSR row pointers
-    hipLaunchKernelGGL(k_indptr_search, dim3(blocks_for(m + 1)), dim3(256), 0, stream, mkeys.as<unsigned long long>(),
-                       nnz, m, rb, indptr);
+    hipLaunchKernelGGL(k_indptr_search, dim3(blocks_for(m + 1)), dim3(256), 0, stream, members.as<KeyDist>(), nnz, m,
+                       rb, indptr);
     MLK_CHECK_HIP(hipGetLastError());
     MLK_TRY(res->indices.reserve((size_t)std::max<int64_t>(nnz, 1) * sizeof(int64_t)));
+    if (res->has_dist) MLK_TRY(res->dist.reserve((size_t)std::max<int64_t>(nnz, 1) * sizeof(double)));
     if (nnz)
-        hipLaunchKernelGGL(k_key_to_index, dim3(blocks_for(nnz)), dim3(256), 0, stream, mkeys.as<unsigned long long>(), nnz,
-                           rb, res->indices.as<int64_t>());
-    if (res->has_dist) {
-        MLK_TRY(res->dist.reserve((size_t)std::max<int64_t>(nnz, 1) * sizeof(double)));
-        if (nnz)
-            MLK_CHECK_HIP(hipMemcpyAsync(res->dist.ptr, mdist.ptr, nnz * sizeof(double), hipMemcpyDeviceToDevice, stream));
-    }
+        hipLaunchKernelGGL(k_split_members, dim3(blocks_for(nnz)), dim3(256), 0, stream, members.as<KeyDist>(), nnz,
+                           rb, res->indices.as<int64_t>(), res->has_dist ? res->dist.as<double>() : nullptr);
     if (sort_rows) MLK_TRY(sort_rows_by_distance(res, indptr, m, nnz, ws, stream));
     MLK_CHECK_HIP(hipGetLastError());
     MLK_CHECK_HIP(hipStreamSynchronize(stream));
