@@ -65,7 +65,8 @@ def timed(fn, steps, warmup):
         per.append(time.perf_counter() - ts)
     total = time.perf_counter() - t0
     q1, med, q3 = np.percentile(np.array(per) * 1e3, [25, 50, 75])
-    spread = {"median_ms": round(float(med), 3), "iqr_ms": round(float(q3 - q1), 3), "runs": steps}
+    spread = {"median_ms": round(float(med), 3), "iqr_ms": round(float(q3 - q1), 3), "max_ms": round(max(per) * 1e3, 3),
+              "runs": steps}
     return total / steps, out, spread
 
 
@@ -324,10 +325,12 @@ def run_c5(args):
     C1 = torch.where((counts > 0)[:, None], sums / counts.clamp(min=1).to(torch.float64)[:, None],
                      Cd.to(torch.float64)).to(torch.float32).contiguous()
     reused = []
-    t_reuse, (l_reuse, _), _ = timed(lambda: assign_device(Xd, C1, precision="bfloat16", previous=labels,
-                                                           out_reused=reused), 3, 1)
-    t_full, (l_full, _), _ = timed(lambda: assign_device(Xd, C1, precision="bfloat16"), 3, 1)
-    line["lloyd_iteration_2"] = {"assign_reuse_ms": round(t_reuse * 1e3, 3), "assign_full_ms": round(t_full * 1e3, 3),
+    _, (l_reuse, _), s_reuse = timed(lambda: assign_device(Xd, C1, precision="bfloat16", previous=labels,
+                                                           out_reused=reused), 5, 1)
+    _, (l_full, _), s_full = timed(lambda: assign_device(Xd, C1, precision="bfloat16"), 5, 1)
+    # medians of five calls, with the slowest call beside them
+    line["lloyd_iteration_2"] = {"assign_reuse_ms": s_reuse["median_ms"], "assign_reuse_max_ms": s_reuse["max_ms"],
+                                 "assign_full_ms": s_full["median_ms"], "assign_full_max_ms": s_full["max_ms"],
                                  "reused_rows": reused[-1], "reused_fraction": round(reused[-1] / n, 4),
                                  "labels_equal": bool(torch.equal(l_reuse, l_full))}
     log(f"c5 iteration 2: {line['lloyd_iteration_2']}")
