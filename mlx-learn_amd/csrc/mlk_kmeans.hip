@@ -35,20 +35,49 @@ __global__ void k_labels_u32(const int64_t* __restrict__ labels, int64_t n, int6
     atomicAdd(&counts[l], 1ull);
 }
 
-// One wave per cluster: float64 sums of its rows in ascending row order.
+// One wave per cluster: float64 sums of its rows in ascending row order. Lane
+// l owns features l + 64 t (t < NJ) of a 64 NJ-feature block; the wave reads
+// 64 row numbers at a time (one coalesced load) and keeps eight rows' values
+// in flight, while each feature's additions stay one row at a time in order.
+template <int NJ>
 __global__ void k_cluster_sums(const float* __restrict__ X, int d, const int* __restrict__ order,
                                const unsigned long long* __restrict__ offsets, int64_t kc, double* __restrict__ sums,
                                int64_t* __restrict__ counts_out) {
+    constexpr int U = 8;  // rows in flight
     const int lane = threadIdx.x & 63;
     const int64_t c = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (c >= kc) return;
     const unsigned long long b = offsets[c], e = offsets[c + 1];
-    for (int j0 = 0; j0 < d; j0 += 64) {
-        const int j = j0 + lane;
-        double s = 0.0;
-        if (j < d)
-            for (unsigned long long r = b; r < e; ++r) s += (double)X[(int64_t)order[r] * d + j];
-        if (j < d) sums[c * d + j] = s;
+    for (int j0 = 0; j0 < d; j0 += 64 * NJ) {
+        double s[NJ];
+#pragma unroll
+        for (int t = 0; t < NJ; ++t) s[t] = 0.0;
+        for (unsigned long long r0 = b; r0 < e; r0 += 64) {
+            const int cnt = e - r0 < 64 ? (int)(e - r0) : 64;
+            const int my_row = lane < cnt ? order[r0 + lane] : 0;
+            for (int u0 = 0; u0 < cnt; u0 += U) {
+                float v[U][NJ];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int row = __shfl(my_row, u0 + u < cnt ? u0 + u : 0);
+#pragma unroll
+                    for (int t = 0; t < NJ; ++t) {
+                        const int j = j0 + lane + 64 * t;
+                        v[u][t] = (u0 + u < cnt && j < d) ? X[(int64_t)row * d + j] : 0.0f;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    if (u0 + u < cnt)
+#pragma unroll
+                        for (int t = 0; t < NJ; ++t) s[t] += (double)v[u][t];
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < NJ; ++t) {
+            const int j = j0 + lane + 64 * t;
+            if (j < d) sums[c * d + j] = s[t];
+        }
     }
     if (lane == 0) counts_out[c] = (int64_t)(e - b);
 }
@@ -382,8 +411,9 @@ extern "C" int mlk_kmeans_update(const float* X_dev, int64_t n, int32_t d, const
         MLK_CHECK_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.ptr, tb, cnt.as<unsigned long long>(),
                                                        off.as<unsigned long long>(), (int)(kc + 1), stream));
     }
-    hipLaunchKernelGGL(k_cluster_sums, dim3(nblk(kc, 4)), dim3(256), 0, stream, X_dev, d, rows2.as<int>(),
-                       off.as<unsigned long long>(), kc, sums_dev, counts_dev);
+    hipLaunchKernelGGL(d <= 64 ? k_cluster_sums<1> : d <= 128 ? k_cluster_sums<2> : k_cluster_sums<4>,
+                       dim3(nblk(kc, 4)), dim3(256), 0, stream, X_dev, d, rows2.as<int>(), off.as<unsigned long long>(),
+                       kc, sums_dev, counts_dev);
     MLK_CHECK_HIP(hipGetLastError());
     if (inertia_dev) {
         if (!residual_dev && n) {
