@@ -50,6 +50,7 @@ constexpr int ST_UNSAFE = 1;     // query coordinates outside the screen's safe 
 constexpr int ST_OVERFLOW = 2;   // a candidate list overflowed
 constexpr int ST_FORCED = 4;     // certificate withdrawn by a test hook
 constexpr int ST_FALLBACK = 8;   // resolved by the exact fallback
+constexpr int ST_TIE = 16;       // k = 1 group rank: a comparison needs the exact comparator (resolved by the lane rank)
 
 constexpr int LCAP = 512;        // candidates ranked in LDS per query
 constexpr int MCAP = 2048;       // upper bounds held in LDS per query by the merge
@@ -731,10 +732,13 @@ __device__ __forceinline__ double fast_sq4(const float* __restrict__ q, const fl
 // (S, index) order; the result's distance is the canonical sum. Statuses and
 // statistics are those of k_rank (no database row is unsafe on this route).
 template <bool VEC>
-__global__ __launch_bounds__(256) void k_rank1_lane(const RankArgs a, int64_t m) {
+__global__ __launch_bounds__(256) void k_rank1_lane(const RankArgs a, int64_t m, int ties_only) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= m) return;
-    if (a.status[q]) {  // handled by the fallback
+    if (ties_only) {  // after the group rank: only its undecided queries
+        if (a.status[q] != ST_TIE) return;
+        a.status[q] = 0;
+    } else if (a.status[q]) {  // handled by the fallback
         if (a.qstats) a.qstats[3 * q] = a.qstats[3 * q + 1] = a.qstats[3 * q + 2] = 0;
         return;
     }
@@ -793,8 +797,10 @@ __global__ __launch_bounds__(256) void k_rank1_lane(const RankArgs a, int64_t m)
 // list entries round-robin for the certified-bound filter, and forms each
 // surviving candidate's float64 distance from coalesced 16-byte row segments,
 // reduced over the group in any order (covered by the order-free bound g).
-// The exact (S, index) comparison and the canonical distance of the result
-// are those of k_rank1_lane, so results and statistics are identical.
+// Candidates are compared by their float64 intervals only; a query with an
+// undecided comparison (a near or exact tie) is marked ST_TIE and ranked again
+// by k_rank1_lane, whose exact comparator resolves it. Results and statistics
+// are those of k_rank1_lane.
 template <int NF4>
 __global__ __launch_bounds__(256) void k_rank1_group(const RankArgs a, int64_t m) {
     constexpr int G = 8;
@@ -818,7 +824,7 @@ __global__ __launch_bounds__(256) void k_rank1_group(const RankArgs a, int64_t m
     const double nqq = a.nq[q];
     const double cut = (double)tau / a.c.scale2;
     int listed = 0, count = 0, kept = 0;
-    bool over = false;
+    bool over = false, tie = false;
     double best_s = INFINITY;
     long long best_i = -1;
     for (int sl = 0; sl < a.S; ++sl) {
@@ -871,9 +877,13 @@ __global__ __launch_bounds__(256) void k_rank1_group(const RankArgs a, int64_t m
                     const double sj = part;
                     if (!(sj * (1.0 - g) <= cut)) continue;
                     ++kept;
-                    if (best_i < 0 || exact_precedes(sj, row, best_s, best_i, g, qv, a.X, d)) {
+                    // the interval decides almost every comparison; an undecided one (a
+                    // near or exact tie) leaves the query to the lane rank's exact comparator
+                    if (best_i < 0 || sj * (1.0 + g) < best_s * (1.0 - g)) {
                         best_s = sj;
                         best_i = row;
+                    } else if (!(best_s * (1.0 + g) < sj * (1.0 - g))) {
+                        tie = true;
                     }
                 }
             }
@@ -883,6 +893,10 @@ __global__ __launch_bounds__(256) void k_rank1_group(const RankArgs a, int64_t m
     if (over || kept < 1) {
         a.status[q] = ST_OVERFLOW;
         if (a.qstats) a.qstats[3 * q] = a.qstats[3 * q + 1] = a.qstats[3 * q + 2] = 0;
+        return;
+    }
+    if (tie) {
+        a.status[q] = ST_TIE;
         return;
     }
     if (a.qstats) {
@@ -1229,14 +1243,13 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
         MLK_TRY(ws[19].reserve((size_t)m * 3 * sizeof(int)));
         ra.qstats = ws[19].as<int>();  // written for every query by k_rank
         const bool vec = d % 4 == 0 && ((uintptr_t)Q % 16) == 0 && ((uintptr_t)X % 16) == 0;
-        if (k == 1 && mr.n_unsafe == 0 && vec && d <= 128)
-            hipLaunchKernelGGL(d <= 64 ? k_rank1_group<2> : k_rank1_group<4>, dim3((unsigned)((m + 31) / 32)),
-                               dim3(256), 0, stream, ra, m);
-        else if (k == 1 && mr.n_unsafe == 0 && vec && d <= 256)
-            hipLaunchKernelGGL(k_rank1_group<8>, dim3((unsigned)((m + 31) / 32)), dim3(256), 0, stream, ra, m);
-        else if (k == 1 && mr.n_unsafe == 0)
+        if (k == 1 && mr.n_unsafe == 0 && vec && d <= 256) {
+            hipLaunchKernelGGL(d <= 64 ? k_rank1_group<2> : d <= 128 ? k_rank1_group<4> : k_rank1_group<8>,
+                               dim3((unsigned)((m + 31) / 32)), dim3(256), 0, stream, ra, m);
+            hipLaunchKernelGGL(k_rank1_lane<true>, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, stream, ra, m, 1);
+        } else if (k == 1 && mr.n_unsafe == 0)
             hipLaunchKernelGGL(vec ? k_rank1_lane<true> : k_rank1_lane<false>, dim3((unsigned)((m + 255) / 256)),
-                               dim3(256), 0, stream, ra, m);
+                               dim3(256), 0, stream, ra, m, 0);
         else
             hipLaunchKernelGGL(k_rank, dim3((unsigned)((m + 3) / 4)), dim3(256), 0, stream, ra, m);
         MLK_CHECK_HIP(hipGetLastError());
