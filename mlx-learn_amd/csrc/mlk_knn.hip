@@ -86,10 +86,10 @@ __device__ __forceinline__ double fast_sq(const float* __restrict__ q, const flo
 // used, so their latency overlaps the arithmetic. Lane j of each wave owns
 // samples j + 64u (u < U) and takes their features into registers eight at a
 // time while it walks the wave's QW queries (broadcast LDS reads), so each
-// staged value serves 4 * QW queries. float64 differences and fused
-// squares-and-adds: any evaluation order is covered by the relative bound
-// g = f64_interval(d). Each wave also initialises its queries' candidate
-// sub-list counters and k-best bounds (screened calls only).
+// staged value serves 4 * QW queries. float32 differences and fused
+// squares-and-adds, turned into a certified upper bound of S by the relative
+// and absolute error terms below. Each wave also initialises its queries'
+// candidate sub-list counters and k-best bounds (screened calls only).
 template <int U, int QW>
 __global__ __launch_bounds__(256) void k_tau0(const float* __restrict__ X, int64_t n, int d,
                                               const float* __restrict__ Q, int64_t m, int64_t m_pad, int k,
@@ -163,11 +163,11 @@ __global__ __launch_bounds__(256) void k_tau0(const float* __restrict__ X, int64
             if (qi < QW) sq[w][qi][e_st] = (fe && qi < live) ? qstg[i] : 0.0f;
         }
     };
-    double acc[QW][U];
+    float acc[QW][U];
 #pragma unroll
     for (int qi = 0; qi < QW; ++qi)
 #pragma unroll
-        for (int u = 0; u < U; ++u) acc[qi][u] = 0.0;
+        for (int u = 0; u < U; ++u) acc[qi][u] = 0.0f;
     fetch(0);
     for (int f0 = 0; f0 < d; f0 += FL) {
         __syncthreads();  // every read of the previous pass is done
@@ -176,26 +176,33 @@ __global__ __launch_bounds__(256) void k_tau0(const float* __restrict__ X, int64
         if (f0 + FL < d) fetch(f0 + FL);  // in flight during this pass
 #pragma unroll 1
         for (int c0 = 0; c0 < FL; c0 += FC) {
-            double sv[U][FC];
+            float sv[U][FC];
 #pragma unroll
             for (int u = 0; u < U; ++u)
 #pragma unroll
-                for (int e = 0; e < FC; ++e) sv[u][e] = (double)sx[lane + 64 * u][c0 + e];
+                for (int e = 0; e < FC; ++e) sv[u][e] = sx[lane + 64 * u][c0 + e];
 #pragma unroll
             for (int qi = 0; qi < QW; ++qi) {
 #pragma unroll
                 for (int e = 0; e < FC; ++e) {
-                    const double dq = (double)sq[w][qi][c0 + e];
+                    const float dq = sq[w][qi][c0 + e];
 #pragma unroll
                     for (int u = 0; u < U; ++u) {
-                        const double t = dq - sv[u][e];
-                        acc[qi][u] = fma(t, t, acc[qi][u]);
+                        const float t = dq - sv[u][e];
+                        acc[qi][u] = fmaf(t, t, acc[qi][u]);
                     }
                 }
             }
         }
     }
-    const double g = f64_interval(d);
+    // float32 rounding: t = fl(q - x) = (q - x)(1 + delta) and a_i = fma(t, t, a_(i-1))
+    // = (t^2 + a_(i-1))(1 + eps) with |delta|, |eps| <= 2^-24, and at most 2^-126
+    // lost to a subnormal or flushed result per step, give
+    // a_d >= (1 - 2^-24)^(d+2) S - d 2^-125, hence
+    // S <= (a_d + (d + 1) 2^-125)(1 + (d + 4) 2^-23), evaluated in float64 (its own
+    // rounding is inside the spare 2^-22) and rounded up; an overflow gives +inf
+    const double pad = (double)(d + 1) * 0x1p-125;
+    const double rel = 1.0 + (double)(d + 4) * 0x1p-23;
 #pragma unroll
     for (int qi = 0; qi < QW; ++qi) {
         const int64_t q = q0 + qi;
@@ -210,7 +217,7 @@ __global__ __launch_bounds__(256) void k_tau0(const float* __restrict__ X, int64
             const int64_t row = (int64_t)j * stride;
             bits[u] = 0x7f800000u;
             if (j < samples && !(exclude && row == self_off + q))
-                bits[u] = __float_as_uint(__double2float_ru(acc[qi][u] * (1.0 + g) * c.scale2));
+                bits[u] = __float_as_uint(__double2float_ru(((double)acc[qi][u] + pad) * rel * c.scale2));
         }
         uint32_t kb;
         if (k == 1) {  // the minimum (non-negative float bit patterns order as integers)

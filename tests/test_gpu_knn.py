@@ -303,3 +303,20 @@ def test_nearest_row_with_ties_at_group_widths(mlk, precision, d):
     got = gpu_knn(mlk, X, Q, 1, precision)
     assert_parity(got, want)
     assert (got[1][:5, 0] == 17).all() and (got[1][5:9, 0] == 1999).all()
+
+
+def test_merge_with_more_new_bounds_than_it_holds(mlk, rng):
+    """A query with about 6,000 rows inside the screen's error band: one merge receives more new upper bounds than
+    it holds in LDS (it keeps its threshold), and the rank's list overflows into the exact fallback; every other
+    query is unaffected. Results equal the oracle's."""
+    from mlxlearn_amd import _lib
+    Q = rng.normal(size=(40, 32)).astype(np.float32)
+    X = rng.normal(size=(20_000, 32)).astype(np.float32)
+    X[:6000] = Q[0] + rng.normal(scale=0.01, size=(6000, 32)).astype(np.float32)
+    want = oracle.knn(X, Q, 10)
+    _lib.set_tuning(splits=64, capacity=64, rounds=[0.5])
+    try:
+        for p in PRECISIONS:
+            assert_parity(gpu_knn(mlk, X, Q, 10, p), want)
+    finally:
+        _lib.set_tuning()
