@@ -103,7 +103,11 @@ def load(path: str | None = None):
     if path == LIB_PATH:
         _LIB = lib
         from . import _config
-        _config.apply_tuning()
+        try:
+            _config.apply_tuning()
+        except Exception:
+            _LIB = None  # a rejected MLXLEARN_* value fails every load until it is fixed
+            raise
     return lib
 
 

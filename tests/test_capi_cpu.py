@@ -330,3 +330,22 @@ def test_diagnostics_ring_records_fit_and_is_bounded():
     assert m.diagnostics_summary() == {"NearestNeighbors.fit -> gpu": 3}
     m.clear_backend_diagnostics()
     assert m.get_last_backend_event() is None
+
+
+def test_bad_environment_tuning_fails_every_load(monkeypatch):
+    """A malformed MLXLEARN_* value is an error at each library load, never a silent default."""
+    from mlxlearn_amd import _config, _lib
+    monkeypatch.setenv("MLXLEARN_SCREEN_STAGES", "1")
+    saved = _lib._LIB
+    try:
+        _lib._LIB = None
+        with pytest.raises(ValueError, match="screen_stages"):
+            _config.reset_config()
+        for _ in range(2):
+            with pytest.raises(ValueError, match="screen_stages"):
+                _lib.load()
+            assert _lib._LIB is None
+    finally:
+        monkeypatch.delenv("MLXLEARN_SCREEN_STAGES")
+        _config.reset_config()
+        _lib._LIB = saved
