@@ -233,27 +233,125 @@ __global__ void k_radius_mean(const int64_t* __restrict__ indptr, int64_t m, con
     pred[q] = e == b ? NAN : num / den;
 }
 
-// Stable segmented sort of every CSR row by its distance (non-negative
-// doubles, which the radix sort orders numerically); rows arrive in ascending
-// index order, so equal distances stay in index order.
-// Temporaries live in the index workspace (ws[14] sort scratch, ws[19..20]
-// sorted distances and indices); both are swapped into the result.
+// Sort keys of the CSR entries by distance: the float32 rounding of each
+// (non-negative) distance, a monotone map that never inverts two distances,
+// and the entry's position.
+__global__ void k_row_sort_keys(const double* __restrict__ dist, int64_t nnz, uint32_t* __restrict__ keys,
+                                uint32_t* __restrict__ pos) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nnz) return;
+    keys[i] = __float_as_uint(__double2float_rn(dist[i]));
+    pos[i] = (uint32_t)i;
+}
+
+// First entry of every non-empty row (runs never cross a row boundary).
+__global__ void k_row_starts(const int64_t* __restrict__ indptr, int64_t m, uint8_t* __restrict__ start) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < m && indptr[q] < indptr[q + 1]) start[indptr[q]] = 1;
+}
+
+__global__ void k_row_gather(const uint32_t* __restrict__ pos, int64_t nnz, const double* __restrict__ dist,
+                             const int64_t* __restrict__ idx, double* __restrict__ dist_out,
+                             int64_t* __restrict__ idx_out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nnz) return;
+    const uint32_t p = pos[i];
+    dist_out[i] = dist[p];
+    idx_out[i] = idx[p];
+}
+
+__device__ __forceinline__ bool dist_idx_less(double da, int64_t ia, double db, int64_t ib) {
+    return da < db || (da == db && ia < ib);
+}
+
+// Entries of a row whose float32 keys are equal keep their index order from
+// the stable key sort; the thread at the start of each such run orders it by
+// (distance, index): insertion for short runs, an in-place heap sort for long
+// ones (many distinct distances inside one float32 step).
+__global__ void k_row_fix_runs(const uint32_t* __restrict__ keys, const uint8_t* __restrict__ start, int64_t nnz,
+                               double* __restrict__ dist, int64_t* __restrict__ idx) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nnz) return;
+    if (!(i == 0 || start[i] || keys[i] != keys[i - 1])) return;
+    int64_t e = i + 1;
+    while (e < nnz && !start[e] && keys[e] == keys[i]) ++e;
+    const int64_t L = e - i;
+    if (L < 2) return;
+    double* D = dist + i;
+    int64_t* I = idx + i;
+    if (L <= 32) {
+        for (int64_t a = 1; a < L; ++a) {
+            const double dv = D[a];
+            const int64_t iv = I[a];
+            int64_t b = a - 1;
+            while (b >= 0 && dist_idx_less(dv, iv, D[b], I[b])) {
+                D[b + 1] = D[b];
+                I[b + 1] = I[b];
+                --b;
+            }
+            D[b + 1] = dv;
+            I[b + 1] = iv;
+        }
+        return;
+    }
+    auto sift = [&](int64_t root, int64_t end) {
+        while (2 * root + 1 < end) {
+            int64_t child = 2 * root + 1;
+            if (child + 1 < end && dist_idx_less(D[child], I[child], D[child + 1], I[child + 1])) ++child;
+            if (!dist_idx_less(D[root], I[root], D[child], I[child])) return;
+            const double td = D[root]; D[root] = D[child]; D[child] = td;
+            const int64_t ti = I[root]; I[root] = I[child]; I[child] = ti;
+            root = child;
+        }
+    };
+    for (int64_t r = L / 2 - 1; r >= 0; --r) sift(r, L);
+    for (int64_t end2 = L - 1; end2 > 0; --end2) {
+        const double td = D[0]; D[0] = D[end2]; D[end2] = td;
+        const int64_t ti = I[0]; I[0] = I[end2]; I[end2] = ti;
+        sift(0, end2);
+    }
+}
+
+// Every CSR row ordered by (distance, index). A stable segmented radix sort on
+// 32-bit keys (the float32 rounding of the distance) and 32-bit positions
+// orders the rows up to runs of equal keys, whose entries arrive in ascending
+// index order; the runs are then ordered exactly. Temporaries live in the
+// index workspace (ws[14] sort scratch, ws[10..11] keys, ws[21..22] positions,
+// ws[23] row starts, ws[19..20] sorted distances and indices); the sorted
+// arrays are swapped into the result.
 static int sort_rows_by_distance(mlk_radius_result* res, const int64_t* indptr, int64_t m, int64_t nnz, DevBuf* ws,
                                  hipStream_t stream) {
     if (!nnz) return MLK_OK;
     DevBuf& tmp = ws[14];
     DevBuf& dist_out = ws[19];
     DevBuf& idx_out = ws[20];
+    MLK_TRY(ws[10].reserve((size_t)nnz * 4));
+    MLK_TRY(ws[11].reserve((size_t)nnz * 4));
+    MLK_TRY(ws[21].reserve((size_t)nnz * 4));
+    MLK_TRY(ws[22].reserve((size_t)nnz * 4));
+    MLK_TRY(ws[23].reserve((size_t)nnz));
     MLK_TRY(dist_out.reserve((size_t)nnz * 8));
     MLK_TRY(idx_out.reserve((size_t)nnz * 8));
+    uint32_t* keys = ws[10].as<uint32_t>();
+    uint32_t* keys_out = ws[11].as<uint32_t>();
+    uint32_t* pos = ws[21].as<uint32_t>();
+    uint32_t* pos_out = ws[22].as<uint32_t>();
+    uint8_t* start = ws[23].as<uint8_t>();
+    hipLaunchKernelGGL(k_row_sort_keys, dim3(blocks_for(nnz)), dim3(256), 0, stream, res->dist.as<double>(), nnz, keys,
+                       pos);
+    MLK_CHECK_HIP(hipMemsetAsync(start, 0, (size_t)nnz, stream));
+    hipLaunchKernelGGL(k_row_starts, dim3(blocks_for(m)), dim3(256), 0, stream, indptr, m, start);
     size_t tmp_bytes = 0;
-    MLK_CHECK_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(
-        nullptr, tmp_bytes, res->dist.as<double>(), dist_out.as<double>(), res->indices.as<int64_t>(),
-        idx_out.as<int64_t>(), (int)nnz, (int)m, indptr, indptr + 1, 0, 64, stream));
+    MLK_CHECK_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(nullptr, tmp_bytes, keys, keys_out, pos, pos_out,
+                                                              (int)nnz, (int)m, indptr, indptr + 1, 0, 31, stream));
     MLK_TRY(tmp.reserve(tmp_bytes + 16));
-    MLK_CHECK_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(
-        tmp.ptr, tmp_bytes, res->dist.as<double>(), dist_out.as<double>(), res->indices.as<int64_t>(),
-        idx_out.as<int64_t>(), (int)nnz, (int)m, indptr, indptr + 1, 0, 64, stream));
+    MLK_CHECK_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(tmp.ptr, tmp_bytes, keys, keys_out, pos, pos_out,
+                                                              (int)nnz, (int)m, indptr, indptr + 1, 0, 31, stream));
+    hipLaunchKernelGGL(k_row_gather, dim3(blocks_for(nnz)), dim3(256), 0, stream, pos_out, nnz,
+                       res->dist.as<double>(), res->indices.as<int64_t>(), dist_out.as<double>(),
+                       idx_out.as<int64_t>());
+    hipLaunchKernelGGL(k_row_fix_runs, dim3(blocks_for(nnz)), dim3(256), 0, stream, keys_out, start, nnz,
+                       dist_out.as<double>(), idx_out.as<int64_t>());
     MLK_CHECK_HIP(hipGetLastError());
     std::swap(res->dist.ptr, dist_out.ptr);
     std::swap(res->dist.bytes, dist_out.bytes);

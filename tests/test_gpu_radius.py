@@ -166,3 +166,20 @@ def test_misaligned_queries_take_the_scalar_path(mlk, rng):
             assert_csr((indptr.cpu().numpy(), indices.cpu().numpy(), dist.cpu().numpy()), want)
         finally:
             index.close()
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_sorted_rows_with_distances_inside_one_float32_step(mlk, precision):
+    """Distances that differ only below float32 resolution: rows (1, j 2^-20) lie at 1 + j^2 2^-41 (one float32
+    value, 100 distinct doubles) and rows (2, j 2^-20) at 2 + j^2 2^-42 (a run of 12), shuffled, with exact
+    duplicates; sorted rows must still follow (distance, index) exactly."""
+    r = np.random.default_rng(404)
+    long_run = np.stack([np.ones(100), np.arange(1, 101) * 2.0 ** -20], 1)
+    short_run = np.stack([np.full(12, 2.0), np.arange(1, 13) * 2.0 ** -20], 1)
+    X = np.concatenate([long_run, short_run, long_run[[7, 50]], short_run[[3]]]).astype(np.float32)
+    X = X[r.permutation(len(X))]
+    Q = np.zeros((3, 2), np.float32)
+    Q[2] = [0.0, 2.0 ** -20]
+    R2 = 9.0
+    want = oracle.radius(X, Q, R2, sort_results=True)
+    assert_csr(gpu_radius(mlk, X, Q, R2, precision, sort_results=True), want, "float32-step runs")
