@@ -388,12 +388,22 @@ __global__ void k_merge_lane(int64_t m, int k, const int* __restrict__ cnt, int*
         const int lo = seen[q * S2 + s];
         seen[q * S2 + s] = hi;
         const size_t off = (size_t)q * cap * S2 + s;  // entry j at off + j * S2
-        for (int j = lo; j < hi; ++j) {
-            const int2 e = cand[off + (size_t)j * S2];
-            double L, U;
-            hit_bounds(__int_as_float(e.y), nx32[e.x], nqq, c, L, U);
-            insert(fmaxf(__double2float_ru(U), 0.0f));
-            ++fresh;
+        // four entries per step: their loads, then their norm gathers, issued together
+        for (int j0 = lo; j0 < hi; j0 += 4) {
+            int2 e[4];
+            float nxv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) e[u] = cand[off + (size_t)(j0 + u < hi ? j0 + u : hi - 1) * S2];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) nxv[u] = nx32[e[u].x];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (j0 + u >= hi) continue;
+                double L, U;
+                hit_bounds(__int_as_float(e[u].y), nxv[u], nqq, c, L, U);
+                insert(fmaxf(__double2float_ru(U), 0.0f));
+                ++fresh;
+            }
         }
     }
     if (over) {
@@ -1204,9 +1214,9 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
         const bool lane_merge = S2 <= 16 && k <= 8;
         auto merge = [&]() -> int {
             if (lane_merge)
-                hipLaunchKernelGGL(k_merge_lane<8>, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, stream, m, k,
-                                   sa.cnt, ws[17].as<int>(), sa.cand, sa.nx32, S2, cap, ws[18].as<float>(),
-                                   ws[1].as<double>(), c, tau, thr, status);
+                hipLaunchKernelGGL(k == 1 ? k_merge_lane<1> : k_merge_lane<8>, dim3((unsigned)((m + 255) / 256)),
+                                   dim3(256), 0, stream, m, k, sa.cnt, ws[17].as<int>(), sa.cand, sa.nx32, S2, cap,
+                                   ws[18].as<float>(), ws[1].as<double>(), c, tau, thr, status);
             else
                 hipLaunchKernelGGL(k_merge, dim3((unsigned)((m + 3) / 4)), dim3(256), 0, stream, m, k, sa.cnt,
                                    ws[17].as<int>(), sa.cand, sa.nx32, S2, cap, ws[18].as<float>(),
