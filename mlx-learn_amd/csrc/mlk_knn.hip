@@ -268,11 +268,22 @@ __global__ void k_sum_qstats(const int* __restrict__ qs, int64_t m, unsigned lon
 // are folded in, the k-th smallest value of the union becomes the new tau
 // (an upper bound of the query's k-th smallest S), and kbest is rewritten with
 // the k smallest values of the union.
+//
+// With rows (kbest_row != nullptr, k <= 32) the merge also keeps the row of
+// every kbest entry and replaces each entry's value by the smaller of it and
+// the row's float64 distance raised by the order-free bound g (rounded up):
+// still k upper bounds of k distinct rows, so their largest is a valid tau,
+// now free of the screen's rounding band, which shrinks the next rounds'
+// candidate sets.
 __global__ void k_merge(int64_t m, int k, const int* __restrict__ cnt, int* __restrict__ seen,
                         const int2* __restrict__ cand, const float* __restrict__ nx32, int S2, int cap,
                         float* __restrict__ kbest, const double* __restrict__ nq, Consts c, float* __restrict__ tau,
-                        float* __restrict__ thr, int* __restrict__ status) {
+                        float* __restrict__ thr, int* __restrict__ status, int* __restrict__ kbest_row,
+                        const float* __restrict__ X, const float* __restrict__ Q, int d) {
     __shared__ uint32_t sU[4][MCAP];
+    __shared__ int sR[4][MCAP];
+    __shared__ float sKV[4][32];
+    __shared__ int sKR[4][32];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int64_t q = (int64_t)blockIdx.x * 4 + w;
     if (q >= m) return;
@@ -294,7 +305,12 @@ __global__ void k_merge(int64_t m, int k, const int* __restrict__ cnt, int* __re
     }
     if (fresh == 0 || k + fresh > MCAP) return;  // tau stays valid (just not tighter)
     uint32_t* buf = sU[w];
-    for (int j = lane; j < k; j += 64) buf[j] = __float_as_uint(kbest[q * k + j]);
+    int* rbuf = sR[w];
+    const bool rows = kbest_row != nullptr && k <= 32;
+    for (int j = lane; j < k; j += 64) {
+        buf[j] = __float_as_uint(kbest[q * k + j]);
+        if (rows) rbuf[j] = kbest_row[q * k + j];
+    }
     int base = k;
     for (int s0 = 0; s0 < S2; s0 += 64) {
         const int s = s0 + lane;
@@ -327,6 +343,7 @@ __global__ void k_merge(int64_t m, int k, const int* __restrict__ cnt, int* __re
                 if (j0 + u >= hi) continue;
                 double L, U;
                 hit_bounds(__int_as_float(e[u].y), nxv[u], nqq, c, L, U);
+                if (rows) rbuf[at] = e[u].x;
                 buf[at++] = __float_as_uint(fmaxf(__double2float_ru(U), 0.0f));
             }
         }
@@ -336,19 +353,81 @@ __global__ void k_merge(int64_t m, int k, const int* __restrict__ cnt, int* __re
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     const int total = base;
     const uint32_t kb = wave_kth_bits(total, k, [&](int j) { return buf[j]; });
-    // kbest <- every value below the k-th, then copies of the k-th
-    int put = 0;
-    for (int j0 = 0; j0 < total; j0 += 64) {
-        const int j = j0 + lane;
-        const bool below = j < total && buf[j] < kb;
-        const unsigned long long mask = ballot(below);
-        const int pos = put + __popcll(mask & ((1ull << lane) - 1));
-        if (below) kbest[q * k + pos] = __uint_as_float(buf[j]);
-        put += __popcll(mask);
+    if (!rows) {
+        // kbest <- every value below the k-th, then copies of the k-th
+        int put = 0;
+        for (int j0 = 0; j0 < total; j0 += 64) {
+            const int j = j0 + lane;
+            const bool below = j < total && buf[j] < kb;
+            const unsigned long long mask = ballot(below);
+            const int pos = put + __popcll(mask & ((1ull << lane) - 1));
+            if (below) kbest[q * k + pos] = __uint_as_float(buf[j]);
+            put += __popcll(mask);
+        }
+        for (int j = put + lane; j < k; j += 64) kbest[q * k + j] = __uint_as_float(kb);
+        if (lane == 0) {
+            const float t_new = fminf(tau[q], __uint_as_float(kb));
+            tau[q] = t_new;
+            thr[q] = thr_from_tau((double)t_new, nq[q], c);
+        }
+        return;
     }
-    for (int j = put + lane; j < k; j += 64) kbest[q * k + j] = __uint_as_float(kb);
+    // the k entries with their rows: every value below the k-th, then entries
+    // equal to it (at least enough exist), then empty slots
+    float* kv = sKV[w];
+    int* kr = sKR[w];
+    int put = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int j0 = 0; j0 < total && put < k; j0 += 64) {
+            const int j = j0 + lane;
+            const bool sel = j < total && (pass == 0 ? buf[j] < kb : buf[j] == kb);
+            const unsigned long long mask = ballot(sel);
+            const int pos = put + __popcll(mask & ((1ull << lane) - 1));
+            if (sel && pos < k) {
+                kv[pos] = __uint_as_float(buf[j]);
+                kr[pos] = rbuf[j];
+            }
+            put += __popcll(mask);
+        }
+    }
+    for (int j = put + lane; j < k; j += 64) {
+        kv[j] = INFINITY;
+        kr[j] = -1;
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    // float64 distances of the kept rows, 16 lanes per row (any summation order)
+    const double g = f64_interval(d);
+    const int gl = lane & 15, grp = lane >> 4;
+    const float* qv = Q + q * d;
+    for (int j0 = 0; j0 < k; j0 += 4) {
+        const int j = j0 + grp;
+        const int r = j < k ? kr[j] : -1;
+        double part = 0.0;
+        if (r >= 0) {
+            const float* xv = X + (size_t)r * d;
+            for (int e = gl; e < d; e += 16) {
+                const double t = (double)qv[e] - (double)xv[e];
+                part += t * t;
+            }
+        }
+        for (int o = 8; o > 0; o >>= 1) part += __shfl_xor(part, o, 16);
+        if (r >= 0 && gl == 0) {
+            const float up = __double2float_ru(part * (1.0 + g) * c.scale2);
+            if (up < kv[j]) kv[j] = up;
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    float kth = 0.0f;
+    for (int j = lane; j < k; j += 64) {
+        kbest[q * k + j] = kv[j];
+        kbest_row[q * k + j] = kr[j];
+        kth = fmaxf(kth, kv[j]);
+    }
+    for (int o = 32; o > 0; o >>= 1) kth = fmaxf(kth, __shfl_xor(kth, o));
     if (lane == 0) {
-        const float t_new = fminf(tau[q], __uint_as_float(kb));
+        const float t_new = fminf(tau[q], kth);
         tau[q] = t_new;
         thr[q] = thr_from_tau((double)t_new, nq[q], c);
     }
@@ -1212,6 +1291,16 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
         // one wave per query in general; one lane per query when a query has
         // few sub-lists and k is small
         const bool lane_merge = S2 <= 16 && k <= 8;
+        // rows of the kbest entries: the wave merge tightens tau with their float64
+        // distances, which pays where the screen's band is wide (relative error
+        // constant >= 2^-11: bf16, and fp16 on inexact inputs); float32 screens and
+        // inputs that are exact in the screen type have a narrow band already
+        int* kbest_row = nullptr;
+        if (!lane_merge && k <= 32 && c.H >= 0x1p-11) {
+            MLK_TRY(ws[21].reserve((size_t)m_pad * k * sizeof(int)));
+            kbest_row = ws[21].as<int>();
+            MLK_CHECK_HIP(hipMemsetAsync(kbest_row, 0xff, (size_t)m_pad * k * sizeof(int), stream));  // -1: none
+        }
         auto merge = [&]() -> int {
             if (lane_merge)
                 hipLaunchKernelGGL(k == 1 ? k_merge_lane<1> : k_merge_lane<8>, dim3((unsigned)((m + 255) / 256)),
@@ -1220,7 +1309,7 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
             else
                 hipLaunchKernelGGL(k_merge, dim3((unsigned)((m + 3) / 4)), dim3(256), 0, stream, m, k, sa.cnt,
                                    ws[17].as<int>(), sa.cand, sa.nx32, S2, cap, ws[18].as<float>(),
-                                   ws[1].as<double>(), c, tau, thr, status);
+                                   ws[1].as<double>(), c, tau, thr, status, kbest_row, X, Q, d);
             MLK_CHECK_HIP(hipGetLastError());
             return MLK_OK;
         };
