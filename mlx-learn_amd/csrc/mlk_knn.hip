@@ -270,11 +270,12 @@ __global__ void k_sum_qstats(const int* __restrict__ qs, int64_t m, unsigned lon
 // the k smallest values of the union.
 //
 // With rows (kbest_row != nullptr, k <= 32) the merge also keeps the row of
-// every kbest entry and replaces each entry's value by the smaller of it and
-// the row's float64 distance raised by the order-free bound g (rounded up):
-// still k upper bounds of k distinct rows, so their largest is a valid tau,
-// now free of the screen's rounding band, which shrinks the next rounds'
-// candidate sets.
+// every entry: it takes the min(2k, 32) smallest bounds, replaces each by the
+// smaller of it and its row's float64 distance raised by the order-free bound
+// g (rounded up), and keeps the k smallest results as kbest. These are upper
+// bounds of k distinct rows, so the k-th is a valid tau, now close to the
+// exact k-th distance instead of a screen band above it, which shrinks the
+// next rounds' candidate sets.
 __global__ void k_merge(int64_t m, int k, const int* __restrict__ cnt, int* __restrict__ seen,
                         const int2* __restrict__ cand, const float* __restrict__ nx32, int S2, int cap,
                         float* __restrict__ kbest, const double* __restrict__ nq, Consts c, float* __restrict__ tau,
@@ -352,8 +353,8 @@ __global__ void k_merge(int64_t m, int k, const int* __restrict__ cnt, int* __re
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     const int total = base;
-    const uint32_t kb = wave_kth_bits(total, k, [&](int j) { return buf[j]; });
     if (!rows) {
+        const uint32_t kb = wave_kth_bits(total, k, [&](int j) { return buf[j]; });
         // kbest <- every value below the k-th, then copies of the k-th
         int put = 0;
         for (int j0 = 0; j0 < total; j0 += 64) {
@@ -372,37 +373,40 @@ __global__ void k_merge(int64_t m, int k, const int* __restrict__ cnt, int* __re
         }
         return;
     }
-    // the k entries with their rows: every value below the k-th, then entries
-    // equal to it (at least enough exist), then empty slots
+    // the K2 = min(2k, 32) smallest bounds with their rows (every value below the
+    // K2-th, then entries equal to it, then empty slots); their float64 distances
+    // replace the bounds where smaller, and the k smallest results become kbest
+    const int K2 = 2 * k < 32 ? 2 * k : 32;
+    const uint32_t kb2 = wave_kth_bits(total, K2, [&](int j) { return buf[j]; });
     float* kv = sKV[w];
     int* kr = sKR[w];
     int put = 0;
     for (int pass = 0; pass < 2; ++pass) {
-        for (int j0 = 0; j0 < total && put < k; j0 += 64) {
+        for (int j0 = 0; j0 < total && put < K2; j0 += 64) {
             const int j = j0 + lane;
-            const bool sel = j < total && (pass == 0 ? buf[j] < kb : buf[j] == kb);
+            const bool sel = j < total && (pass == 0 ? buf[j] < kb2 : buf[j] == kb2);
             const unsigned long long mask = ballot(sel);
             const int pos = put + __popcll(mask & ((1ull << lane) - 1));
-            if (sel && pos < k) {
+            if (sel && pos < K2) {
                 kv[pos] = __uint_as_float(buf[j]);
                 kr[pos] = rbuf[j];
             }
             put += __popcll(mask);
         }
     }
-    for (int j = put + lane; j < k; j += 64) {
+    for (int j = put + lane; j < K2; j += 64) {
         kv[j] = INFINITY;
         kr[j] = -1;
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    // float64 distances of the kept rows, 16 lanes per row (any summation order)
+    // float64 distances of those rows, 16 lanes per row (any summation order)
     const double g = f64_interval(d);
     const int gl = lane & 15, grp = lane >> 4;
     const float* qv = Q + q * d;
-    for (int j0 = 0; j0 < k; j0 += 4) {
+    for (int j0 = 0; j0 < K2; j0 += 4) {
         const int j = j0 + grp;
-        const int r = j < k ? kr[j] : -1;
+        const int r = j < K2 ? kr[j] : -1;
         double part = 0.0;
         if (r >= 0) {
             const float* xv = X + (size_t)r * d;
@@ -419,11 +423,20 @@ __global__ void k_merge(int64_t m, int k, const int* __restrict__ cnt, int* __re
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    // rank of each entry by (value, slot); the k lowest become kbest, the k-th is tau
     float kth = 0.0f;
-    for (int j = lane; j < k; j += 64) {
-        kbest[q * k + j] = kv[j];
-        kbest_row[q * k + j] = kr[j];
-        kth = fmaxf(kth, kv[j]);
+    if (lane < K2) {
+        const float v = kv[lane];
+        int rank = 0;
+        for (int i = 0; i < K2; ++i) {
+            const float u = kv[i];
+            rank += (u < v) || (u == v && i < lane);
+        }
+        if (rank < k) {
+            kbest[q * k + rank] = v;
+            kbest_row[q * k + rank] = kr[lane];
+        }
+        if (rank == k - 1) kth = v;
     }
     for (int o = 32; o > 0; o >>= 1) kth = fmaxf(kth, __shfl_xor(kth, o));
     if (lane == 0) {
