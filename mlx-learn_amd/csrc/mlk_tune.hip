@@ -401,6 +401,15 @@ int autotune_impl(mlk_index* index, const float* Q, int64_t m, int k, double R2,
             p.rounds = rs;
             MLK_TRY(check(consider(p)));
         }
+        // calls of well under a millisecond are bound by launches: fewer rounds
+        // (fewer screen and merge launches) can win despite a looser first threshold
+        if (best_ms < 1.0) {
+            for (const auto& rs : std::vector<std::vector<double>>{{1.0 / 4, 1.0}, {1.0}}) {
+                Plan p = best;
+                p.rounds = rs;
+                MLK_TRY(check(consider(p)));
+            }
+        }
         // 4. capacity, when candidate lists overflowed into the exact fallback
         for (int step = 0; step < 2 && best_fb > 0; ++step) {
             Plan p = best;
