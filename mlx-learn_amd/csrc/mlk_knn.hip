@@ -53,7 +53,7 @@ constexpr int ST_FALLBACK = 8;   // resolved by the exact fallback
 constexpr int ST_TIE = 16;       // k = 1 group rank: a comparison needs the exact comparator (resolved by the lane rank)
 
 constexpr int LCAP = 512;        // candidates ranked in LDS per query
-constexpr int MCAP = 1024;       // upper bounds held in LDS per query by the merge
+constexpr int MCAP = 512;        // upper bounds held in LDS per query by the merge
 
 // ---------------------------------------------------------------- tau0
 constexpr int TAU0_SAMPLES = 256;
