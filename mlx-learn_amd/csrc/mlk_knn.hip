@@ -241,26 +241,6 @@ __global__ __launch_bounds__(256) void k_tau0(const float* __restrict__ X, int64
     }
 }
 
-// Sums of the per-query rank statistics (three counts per query): block
-// partials added into out (zeroed by the caller); integer sums, any order.
-__global__ void k_sum_qstats(const int* __restrict__ qs, int64_t m, unsigned long long* __restrict__ out) {
-    __shared__ unsigned long long sh[3][256];
-    unsigned long long a0 = 0, a1 = 0, a2 = 0;
-    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < m; q += (int64_t)gridDim.x * blockDim.x) {
-        a0 += (unsigned)qs[3 * q];
-        a1 += (unsigned)qs[3 * q + 1];
-        a2 += (unsigned)qs[3 * q + 2];
-    }
-    sh[0][threadIdx.x] = a0; sh[1][threadIdx.x] = a1; sh[2][threadIdx.x] = a2;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (threadIdx.x < o)
-            for (int c = 0; c < 3; ++c) sh[c][threadIdx.x] += sh[c][threadIdx.x + o];
-        __syncthreads();
-    }
-    if (threadIdx.x < 3 && sh[threadIdx.x][0]) atomicAdd(&out[threadIdx.x], sh[threadIdx.x][0]);
-}
-
 // ---------------------------------------------------------------- merge
 // Incremental merge after a round. Per query, kbest holds the k smallest
 // certified upper bounds over every candidate merged so far; the entries
@@ -1114,13 +1094,35 @@ __global__ void k_fb_collect(const double* __restrict__ s_in, int64_t n, const d
     }
 }
 
-__global__ void k_qprep_status(int64_t m, const int* __restrict__ status, int* __restrict__ fq,
-                               unsigned* __restrict__ nfq) {
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < m && status[q]) {
-        const unsigned at = atomicAdd(nfq, 1u);
-        fq[at] = (int)q;
+// After the rank: queries without a certificate are listed for the exact
+// fallback, and the rank's statistics (three counts per query, when qs is
+// given) are summed into out (zeroed by the caller; block partials, then one
+// atomic per count; integer sums, any order).
+__global__ void k_finish_status(int64_t m, const int* __restrict__ status, int* __restrict__ fq,
+                                unsigned* __restrict__ nfq, const int* __restrict__ qs,
+                                unsigned long long* __restrict__ out) {
+    __shared__ unsigned long long sh[3][256];
+    unsigned long long a0 = 0, a1 = 0, a2 = 0;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < m; q += (int64_t)gridDim.x * blockDim.x) {
+        if (status[q]) {
+            const unsigned at = atomicAdd(nfq, 1u);
+            fq[at] = (int)q;
+        }
+        if (qs) {
+            a0 += (unsigned)qs[3 * q];
+            a1 += (unsigned)qs[3 * q + 1];
+            a2 += (unsigned)qs[3 * q + 2];
+        }
     }
+    if (!qs) return;
+    sh[0][threadIdx.x] = a0; sh[1][threadIdx.x] = a1; sh[2][threadIdx.x] = a2;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o)
+            for (int c = 0; c < 3; ++c) sh[c][threadIdx.x] += sh[c][threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x < 3 && sh[threadIdx.x][0]) atomicAdd(&out[threadIdx.x], sh[threadIdx.x][0]);
 }
 
 __global__ void k_force(int64_t m, const int64_t* __restrict__ rows, int64_t count, int* __restrict__ status,
@@ -1283,6 +1285,7 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
 
     MLK_CHECK_HIP(hipEventRecord(ev1, stream));
     std::vector<hipEvent_t> round_events;
+    const int* rank_stats = nullptr;  // per-query counts written by the rank (screened calls)
     if (!all_fallback) {
         ScreenArgs sa{};
         sa.xs = mr.xs.ptr;
@@ -1372,9 +1375,7 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
         else
             hipLaunchKernelGGL(k_rank, dim3((unsigned)((m + 3) / 4)), dim3(256), 0, stream, ra, m);
         MLK_CHECK_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_sum_qstats, dim3((unsigned)std::min<int64_t>(1024, (m + 255) / 256)), dim3(256), 0,
-                           stream, ra.qstats, m, (unsigned long long*)(dflag + 4));
-        MLK_CHECK_HIP(hipGetLastError());
+        rank_stats = ra.qstats;
     } else {
         MLK_CHECK_HIP(hipEventRecord(ev2, stream));
         std::vector<int> all(m, ST_FALLBACK);
@@ -1385,8 +1386,8 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
     // queries without a certificate -> exact fallback
     MLK_TRY(ws[11].reserve((size_t)m * sizeof(int)));
     int* fq = ws[11].as<int>();
-    hipLaunchKernelGGL(k_qprep_status, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, stream, m, status, fq,
-                       dflag + 2);
+    hipLaunchKernelGGL(k_finish_status, dim3((unsigned)std::min<int64_t>(1024, (m + 255) / 256)), dim3(256), 0,
+                       stream, m, status, fq, dflag + 2, rank_stats, (unsigned long long*)(dflag + 4));
     MLK_CHECK_HIP(hipGetLastError());
     // one read-back: the fallback count and the statistics sums
     struct {
