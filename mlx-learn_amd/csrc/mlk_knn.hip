@@ -97,7 +97,8 @@ __global__ __launch_bounds__(256) void k_tau0(const float* __restrict__ X, int64
                                               const double* __restrict__ nq, const uint8_t* __restrict__ qflags,
                                               Consts c, float* __restrict__ tau, float* __restrict__ thr,
                                               int* __restrict__ status, int S2, int* __restrict__ cnt,
-                                              int* __restrict__ seen, float* __restrict__ kbest) {
+                                              int* __restrict__ seen, float* __restrict__ kbest,
+                                              int* __restrict__ kbest_row) {
     constexpr int FL = 32;      // features staged per pass
     constexpr int FC = 8;       // features per register chunk
     constexpr int NS = 64 * U;  // sample slots of the block
@@ -118,7 +119,10 @@ __global__ __launch_bounds__(256) void k_tau0(const float* __restrict__ X, int64
                 cnt[q * S2 + j] = 0;
                 seen[q * S2 + j] = 0;
             }
-            for (int j = lane; j < k; j += 64) kbest[q * k + j] = INFINITY;
+            for (int j = lane; j < k; j += 64) {
+                kbest[q * k + j] = INFINITY;
+                if (kbest_row) kbest_row[q * k + j] = -1;
+            }
         }
     }
     const int samples = n < max_samples ? (int)n : max_samples;  // <= NS
@@ -1227,11 +1231,22 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
     // sampled start threshold: 32 rows per neighbour asked for, 64 to 256 rows
     const int tau_samples = std::min(TAU0_SAMPLES, std::max(64, 32 * k));
     const int S2 = 2 * S;  // candidate sub-lists per query (split x lane half)
+    // one lane per query merges when a query has few sub-lists and k is small
+    const bool lane_merge = S2 <= 16 && k <= 8;
+    // rows of the kbest entries: the wave merge tightens tau with their float64
+    // distances, which pays where the screen's band is wide (relative error
+    // constant >= 2^-11: bf16, and fp16 on inexact inputs); float32 screens and
+    // inputs that are exact in the screen type have a narrow band already
+    int* kbest_row = nullptr;
     if (!all_fallback) {
         MLK_TRY(ws[8].reserve((size_t)m_pad * S2 * sizeof(int)));          // counts
         MLK_TRY(ws[9].reserve((size_t)m_pad * S2 * cap * sizeof(int2)));   // (row, dot-product estimate)
         MLK_TRY(ws[17].reserve((size_t)m_pad * S2 * sizeof(int)));         // merged counts
         MLK_TRY(ws[18].reserve((size_t)m_pad * k * sizeof(float)));        // k smallest upper bounds
+        if (!lane_merge && k <= 32 && c.H >= 0x1p-11) {
+            MLK_TRY(ws[21].reserve((size_t)m_pad * k * sizeof(int)));     // their rows (cleared by tau0)
+            kbest_row = ws[21].as<int>();
+        }
     }
     {
         // U sample rows per lane (64 U samples), QW queries per wave: U * QW = 32
@@ -1242,7 +1257,7 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
             hipLaunchKernelGGL(kern, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, stream, X, n, d, Q, m, m_pad,
                                k, tau_samples, q_offset, (int)exclude, ws[1].as<double>(), ws[2].as<uint8_t>(), c,
                                tau, thr, status, S2, all_fallback ? nullptr : ws[8].as<int>(), ws[17].as<int>(),
-                               ws[18].as<float>());
+                               ws[18].as<float>(), kbest_row);
         };
         // the most queries per wave (register reuse of the samples) that still
         // leaves about 4096 waves (4 per SIMD) to fill the device
@@ -1304,19 +1319,6 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
         sa.exclude = exclude;
         sa.xcd_map = (S % 8) == 0;
         const unsigned blocks = (unsigned)(QB * S);
-        // one wave per query in general; one lane per query when a query has
-        // few sub-lists and k is small
-        const bool lane_merge = S2 <= 16 && k <= 8;
-        // rows of the kbest entries: the wave merge tightens tau with their float64
-        // distances, which pays where the screen's band is wide (relative error
-        // constant >= 2^-11: bf16, and fp16 on inexact inputs); float32 screens and
-        // inputs that are exact in the screen type have a narrow band already
-        int* kbest_row = nullptr;
-        if (!lane_merge && k <= 32 && c.H >= 0x1p-11) {
-            MLK_TRY(ws[21].reserve((size_t)m_pad * k * sizeof(int)));
-            kbest_row = ws[21].as<int>();
-            MLK_CHECK_HIP(hipMemsetAsync(kbest_row, 0xff, (size_t)m_pad * k * sizeof(int), stream));  // -1: none
-        }
         auto merge = [&]() -> int {
             if (lane_merge)
                 hipLaunchKernelGGL(k == 1 ? k_merge_lane<1> : k_merge_lane<8>, dim3((unsigned)((m + 255) / 256)),
