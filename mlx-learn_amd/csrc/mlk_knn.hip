@@ -51,8 +51,10 @@ constexpr int ST_OVERFLOW = 2;   // a candidate list overflowed
 constexpr int ST_FORCED = 4;     // certificate withdrawn by a test hook
 constexpr int ST_FALLBACK = 8;   // resolved by the exact fallback
 constexpr int ST_TIE = 16;       // k = 1 group rank: a comparison needs the exact comparator (resolved by the lane rank)
+constexpr int ST_BIG = 32;       // more than LCAP rows survive the bound filter (ranked again with LCAP_BIG)
 
 constexpr int LCAP = 512;        // candidates ranked in LDS per query
+constexpr int LCAP_BIG = 4096;   // the same for queries with more survivors (one wave per block)
 constexpr int MCAP = 512;        // upper bounds held in LDS per query by the merge
 
 // ---------------------------------------------------------------- tau0
@@ -546,30 +548,18 @@ __device__ __forceinline__ Cand shfl_cand(Cand v, int o) {
     return r;
 }
 
-// One wave per query: exact (S, index) order of the candidate set.
-__global__ __launch_bounds__(256) void k_rank(const RankArgs a, int64_t n_items) {
-    __shared__ int sI[4][LCAP];
-    __shared__ double sS[4][LCAP];
-    __shared__ int sN[4];
-    __shared__ int sO[4][64];  // result rows by rank
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int64_t item = (int64_t)blockIdx.x * 4 + w;
-    if (item >= n_items) return;
+// One wave ranks one query (item `item` of the call; query q): exact (S, index)
+// order of its candidate set, held in LDS (li / ls: LC rows; *sN: the wave's
+// counter; orow: 64 result rows). In mode 0, a query whose bound filter keeps
+// more than LC rows is marked ST_BIG when LC < LCAP_BIG (ranked again with
+// LCAP_BIG rows), else ST_OVERFLOW (exact fallback).
+template <int LC>
+__device__ void rank_query(const RankArgs& a, int64_t item, int64_t q, int* __restrict__ li, double* __restrict__ ls,
+                           int* sN, int* orow) {
+    const int lane = threadIdx.x & 63;
     const int d = a.d, k = a.k;
     const double g = f64_interval(d);
-    int64_t q;
-    if (a.mode == 0) {
-        q = item;
-        if (a.status[q]) {  // handled by the fallback
-            if (lane == 0 && a.qstats) a.qstats[3 * q] = a.qstats[3 * q + 1] = a.qstats[3 * q + 2] = 0;
-            return;
-        }
-    } else {
-        q = a.fq[item];
-    }
     const float* qv = a.Q + q * d;
-    int* li = sI[w];
-    double* ls = sS[w];
     int count = 0;
     if (a.mode == 0) {
         const float tau = a.tau[q];
@@ -578,7 +568,7 @@ __global__ __launch_bounds__(256) void k_rank(const RankArgs a, int64_t n_items)
         // Rows whose certified lower bound does not exceed tau, appended in one
         // pass through an LDS counter (the rank below does not depend on the
         // order of the candidates).
-        if (lane == 0) sN[w] = 0;
+        if (lane == 0) *sN = 0;
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         int listed = 0;
@@ -610,8 +600,8 @@ __global__ __launch_bounds__(256) void k_rank(const RankArgs a, int64_t n_items)
                     double L, U;
                     hit_bounds(__int_as_float(e[u].y), nxv[u], nqq, a.c, L, U);
                     if (L <= (double)tau) {
-                        const int at = atomicAdd(&sN[w], 1);
-                        if (at < LCAP) li[at] = e[u].x;
+                        const int at = atomicAdd(sN, 1);
+                        if (at < LC) li[at] = e[u].x;
                     }
                 }
             }
@@ -619,12 +609,12 @@ __global__ __launch_bounds__(256) void k_rank(const RankArgs a, int64_t n_items)
         for (int o = 32; o > 0; o >>= 1) over = __shfl_xor((int)over, o) | over;
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        const int total = sN[w];
+        const int total = *sN;
         int extra = 0;
         for (int u = 0; u < a.n_unsafe; ++u) extra += a.unsafe[u] != self;
-        if (over || total + extra > LCAP) {
+        if (over || total + extra > LC) {
             if (lane == 0) {
-                a.status[q] = ST_OVERFLOW;
+                a.status[q] = (!over && LC < LCAP_BIG) ? ST_BIG : ST_OVERFLOW;
                 if (a.qstats) a.qstats[3 * q] = a.qstats[3 * q + 1] = a.qstats[3 * q + 2] = 0;
             }
             return;
@@ -704,7 +694,7 @@ __global__ __launch_bounds__(256) void k_rank(const RankArgs a, int64_t n_items)
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     } else {
         count = a.pcount[item];
-        if (count <= LCAP) {
+        if (count <= LC) {
             for (int j = lane; j < count; j += 64) {
                 li[j] = a.plist[item * a.pstride + j];
                 ls[j] = a.ps[item * a.pstride + j];
@@ -740,7 +730,6 @@ __global__ __launch_bounds__(256) void k_rank(const RankArgs a, int64_t n_items)
         }
     }
     // rank counting: position of each kept candidate in the exact (S, index) order
-    int* orow = sO[w];
     if (lane < 64) orow[lane] = 0;  // every slot holds a valid row even if a rank stays unset
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -773,7 +762,7 @@ __global__ __launch_bounds__(256) void k_rank(const RankArgs a, int64_t n_items)
     // coalesced loads; then one lane per row forms the left-to-right sum.
     float* stage = (float*)ls;
     const int rs = d + 1;  // padded row stride
-    const int ch = d < LCAP * 2 ? (LCAP * 2 - d) / rs : 0;
+    const int ch = d < LC * 2 ? (LC * 2 - d) / rs : 0;
     if (ch == 0) {
         for (int r = lane; r < k; r += 64) {
             const double sc = canonical_sq(qv, a.X + (size_t)orow[r] * d, d);
@@ -802,6 +791,54 @@ __global__ __launch_bounds__(256) void k_rank(const RankArgs a, int64_t n_items)
         }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+}
+
+// One wave per query, four per block.
+__global__ __launch_bounds__(256) void k_rank(const RankArgs a, int64_t n_items) {
+    __shared__ int sI[4][LCAP];
+    __shared__ double sS[4][LCAP];
+    __shared__ int sN[4];
+    __shared__ int sO[4][64];  // result rows by rank
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t item = (int64_t)blockIdx.x * 4 + w;
+    if (item >= n_items) return;
+    int64_t q;
+    if (a.mode == 0) {
+        q = item;
+        if (a.status[q]) {  // handled by the fallback
+            if (lane == 0 && a.qstats) a.qstats[3 * q] = a.qstats[3 * q + 1] = a.qstats[3 * q + 2] = 0;
+            return;
+        }
+    } else {
+        q = a.fq[item];
+    }
+    rank_query<LCAP>(a, item, q, sI[w], sS[w], &sN[w], sO[w]);
+}
+
+// Queries of the listed set (fq, n_items) marked ST_BIG by k_rank: ranked again
+// with room for LCAP_BIG rows, one wave per block. A query that still does not
+// fit keeps ST_OVERFLOW (exact fallback); the others are resolved (status 0) and
+// their list statistics are added to sums[0..3).
+__global__ __launch_bounds__(64) void k_rank_big(const RankArgs a, const int* __restrict__ fq, int64_t n_items,
+                                                 unsigned long long* __restrict__ sums) {
+    __shared__ int sI[LCAP_BIG];
+    __shared__ double sS[LCAP_BIG];
+    __shared__ int sN;
+    __shared__ int sO[64];
+    const int64_t item = blockIdx.x;
+    if (item >= n_items) return;
+    const int64_t q = fq[item];
+    if (a.status[q] != ST_BIG) return;
+    __builtin_amdgcn_wave_barrier();
+    if (threadIdx.x == 0) a.status[q] = 0;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    rank_query<LCAP_BIG>(a, q, q, sI, sS, &sN, sO);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (threadIdx.x == 0 && a.qstats && a.status[q] == 0) {
+        for (int c = 0; c < 3; ++c) atomicAdd(&sums[c], (unsigned long long)(unsigned)a.qstats[3 * q + c]);
     }
 }
 
@@ -1301,6 +1338,8 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
     MLK_CHECK_HIP(hipEventRecord(ev1, stream));
     std::vector<hipEvent_t> round_events;
     const int* rank_stats = nullptr;  // per-query counts written by the rank (screened calls)
+    bool big_rank = false;            // k_rank ran: queries marked ST_BIG are ranked again
+    RankArgs ra{};
     if (!all_fallback) {
         ScreenArgs sa{};
         sa.xs = mr.xs.ptr;
@@ -1318,6 +1357,7 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
         sa.self_off = q_offset;
         sa.exclude = exclude;
         sa.xcd_map = (S % 8) == 0;
+        sa.nchunk = screen_chunks(dp);
         const unsigned blocks = (unsigned)(QB * S);
         auto merge = [&]() -> int {
             if (lane_merge)
@@ -1356,7 +1396,6 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
         MLK_TRY(merge());
         MLK_CHECK_HIP(hipEventRecord(ev2, stream));
 
-        RankArgs ra{};
         ra.X = X; ra.n = n; ra.d = d; ra.Q = Q; ra.m = m; ra.k = k;
         ra.cnt = sa.cnt; ra.cand = sa.cand; ra.nx32 = sa.nx32; ra.nq = ws[1].as<double>();
         ra.S = S2; ra.cap = cap;
@@ -1374,8 +1413,10 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
         } else if (k == 1 && mr.n_unsafe == 0)
             hipLaunchKernelGGL(vec ? k_rank1_lane<true> : k_rank1_lane<false>, dim3((unsigned)((m + 255) / 256)),
                                dim3(256), 0, stream, ra, m, 0);
-        else
+        else {
             hipLaunchKernelGGL(k_rank, dim3((unsigned)((m + 3) / 4)), dim3(256), 0, stream, ra, m);
+            big_rank = true;
+        }
         MLK_CHECK_HIP(hipGetLastError());
         rank_stats = ra.qstats;
     } else {
@@ -1399,13 +1440,31 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
     static_assert(sizeof(words) == 32, "flag words 2..10");
     MLK_CHECK_HIP(hipMemcpyAsync(&words, dflag + 2, sizeof(words), hipMemcpyDeviceToHost, stream));
     MLK_CHECK_HIP(hipStreamSynchronize(stream));
-    const unsigned nf = words.nf;
+    unsigned nf = words.nf;
+    std::vector<int> fq_host;
+    if (nf) {
+        fq_host.resize(nf);
+        MLK_CHECK_HIP(hipMemcpyAsync(fq_host.data(), fq, nf * sizeof(int), hipMemcpyDeviceToHost, stream));
+        MLK_CHECK_HIP(hipStreamSynchronize(stream));
+    }
+    if (nf && big_rank) {
+        // queries with more survivors than one wave's LDS list: rank them again
+        // with the larger list, then keep only what still needs the exact scan
+        hipLaunchKernelGGL(k_rank_big, dim3(nf), dim3(64), 0, stream, ra, fq, (int64_t)nf,
+                           (unsigned long long*)(dflag + 4));
+        MLK_CHECK_HIP(hipGetLastError());
+        std::vector<int> st_host(m);
+        MLK_CHECK_HIP(hipMemcpyAsync(st_host.data(), status, m * sizeof(int), hipMemcpyDeviceToHost, stream));
+        MLK_CHECK_HIP(hipMemcpyAsync(&words, dflag + 2, sizeof(words), hipMemcpyDeviceToHost, stream));
+        MLK_CHECK_HIP(hipStreamSynchronize(stream));
+        fq_host.erase(std::remove_if(fq_host.begin(), fq_host.end(), [&](int q) { return st_host[q] == 0; }),
+                      fq_host.end());
+        nf = (unsigned)fq_host.size();
+    }
     st.fallback_queries = nf;
     st.candidates = (int64_t)words.cnts[0];
     st.reranked = (int64_t)words.cnts[2];
     if (nf) {
-        std::vector<int> fq_host(nf);
-        MLK_CHECK_HIP(hipMemcpy(fq_host.data(), fq, nf * sizeof(int), hipMemcpyDeviceToHost));
         std::sort(fq_host.begin(), fq_host.end());
         MLK_CHECK_HIP(hipMemcpy(fq, fq_host.data(), nf * sizeof(int), hipMemcpyHostToDevice));
         // batches bounded by ~1 GiB of float64 distances + candidate lists
@@ -1444,12 +1503,12 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
             MLK_CHECK_HIP(hipMemsetAsync(pcount, 0, nb * sizeof(int), stream));
             hipLaunchKernelGGL(k_fb_collect, g2, dim3(256), 0, stream, ws[12].as<double>(), n, cut, pcount,
                                ws[13].as<int>(), ws[14].as<double>(), n);
-            RankArgs ra{};
-            ra.X = X; ra.n = n; ra.d = d; ra.Q = Q; ra.m = m; ra.k = k;
-            ra.fq = fq + f0; ra.plist = ws[13].as<int>(); ra.ps = ws[14].as<double>(); ra.pcount = pcount;
-            ra.pstride = n; ra.mode = 1; ra.self_off = q_offset; ra.exclude = exclude; ra.status = status;
-            ra.out_idx = idx; ra.out_dist = dist; ra.squared = (flags & MLK_SQUARED_DIST) != 0;
-            hipLaunchKernelGGL(k_rank, dim3((unsigned)((nb + 3) / 4)), dim3(256), 0, stream, ra, nb);
+            RankArgs fa{};
+            fa.X = X; fa.n = n; fa.d = d; fa.Q = Q; fa.m = m; fa.k = k;
+            fa.fq = fq + f0; fa.plist = ws[13].as<int>(); fa.ps = ws[14].as<double>(); fa.pcount = pcount;
+            fa.pstride = n; fa.mode = 1; fa.self_off = q_offset; fa.exclude = exclude; fa.status = status;
+            fa.out_idx = idx; fa.out_dist = dist; fa.squared = (flags & MLK_SQUARED_DIST) != 0;
+            hipLaunchKernelGGL(k_rank, dim3((unsigned)((nb + 3) / 4)), dim3(256), 0, stream, fa, nb);
             MLK_CHECK_HIP(hipGetLastError());
         }
     }

@@ -476,13 +476,14 @@ int radius_query_impl(mlk_index* index, const float* Q, int64_t m, double R2, in
             sa.self_off = q_offset;
             sa.exclude = exclude;
             sa.xcd_map = (S % 8) == 0;
+            sa.nchunk = screen_chunks(dp);
             sa.pairs = ws[9].as<int2>();
             sa.pair_count = dcount;
             sa.pair_cap = cap;
             hipLaunchKernelGGL(screen, dim3((unsigned)(QB * S)), dim3(256), 0, stream, sa);
             MLK_CHECK_HIP(hipGetLastError());
         } else {
-            set_error("radius search supports at most 800 features");
+            set_error("no radius screen kernel for this row width");
             return MLK_EINVAL;
         }
         MLK_CHECK_HIP(hipEventRecord(e1, stream));

@@ -96,7 +96,13 @@ struct ScreenArgs {
     int2* pairs;
     unsigned long long* pair_count;
     unsigned long long pair_cap;
+    // rows wider than the widest register-held query (deep kernels): the row
+    // is DEEP_CHUNK features per chunk, nchunk chunks; 1 for every other kernel
+    int nchunk;
 };
+
+constexpr int WIDE_MAX = 800;    // widest row whose query fragments stay in registers
+constexpr int DEEP_CHUNK = 256;  // features per chunk of the deep kernels
 
 constexpr int SCREEN_LISTS = 0;  // k-NN: per-(query, split) candidate lists
 constexpr int SCREEN_PAIRS = 1;  // radius: global (query, row) pair buffer
@@ -114,12 +120,16 @@ struct ScreenShape {
 ScreenShape default_shape(int prec, int dp);
 
 // Padded width of the screen mirror: a multiple of 32 up to 128, then the
-// wide-kernel widths 256, 384, 512, 640, 800 (bf16 / fp16 only).
+// wide-kernel widths 256, 384, 512, 640, 800, then multiples of DEEP_CHUNK
+// (bf16 / fp16 only above 128).
 int screen_dp(int prec, int d);
+
+// Chunks per row of a padded width: dp / DEEP_CHUNK for the deep kernels, else 1.
+inline int screen_chunks(int dp) { return dp > WIDE_MAX ? dp / DEEP_CHUNK : 1; }
 
 // Screen precision actually used: float32 has no wide kernel, so rows wider
 // than 128 features are screened in bf16 (results do not depend on it).
-inline int effective_prec(int prec, int d) { return (prec == PREC_FP32 && d > 128 && d <= 800) ? PREC_BF16 : prec; }
+inline int effective_prec(int prec, int d) { return (prec == PREC_FP32 && d > 128) ? PREC_BF16 : prec; }
 
 // Launch function of the screen for a padded width dp in {32, 64, 96, 128};
 // nullptr when the shape is not built for that mode and precision. Blocks

@@ -90,7 +90,7 @@ __device__ __forceinline__ void tile_barrier() { asm volatile("s_barrier" ::: "m
 // Per tile a wave issues its MFMAs (accumulators start at the rows' biases
 // -a_x), then one branch-free max-tree test of every accumulator against its
 // query's threshold; only a hit enters the append path.
-template <int PREC, int DP, int QT, int RTW, int NS, int MODE>
+template <int PREC, int DP, int QT, int RTW, int NS, int MODE, bool CH = false>
 __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
     // the body uses device-only types (buffer resources): compile it in the
     // device pass only, so the host pass still emits the kernel's launch stub
@@ -110,6 +110,7 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
     static_assert(NS >= 2 && NS <= 4, "stages");
     static_assert(QT == 1 || QT == 2, "query tiles");
     static_assert(RTW == 1 || RTW == 2, "row tiles per step");
+    static_assert(!CH || NS == 2, "deep rows: two stages (each step waits for all of its loads)");
     static_assert(NS * (ABYTES + 512) + (MODE == SCREEN_PAIRS ? NW * PAIR_STAGE * 8 : 0) <= 160 * 1024,
                   "LDS of one block");
     // distinct buffers (not one array) so the compiler can tell the tile
@@ -134,6 +135,12 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
     int64_t r_end = s_lo + (a.off_hi < a.split_rows ? a.off_hi : a.split_rows);
     if (r_end > a.n) r_end = a.n;
     if (r_begin >= r_end) return;
+    // Deep rows (CH): a row is nc chunks of DP features; step u of the walk is
+    // chunk u % nc of tile u / nc, and the accumulators carry over the chunks
+    // of a tile. Otherwise nc = 1 and a step is a tile.
+    const int nc = CH ? a.nchunk : 1;
+    const uint32_t rbg = (uint32_t)RB * (uint32_t)nc;  // bytes per database row in memory
+    const size_t qstride = (size_t)DP * nc;          // elements per query row
 
     if constexpr (MODE == SCREEN_PAIRS) {
         if (threadIdx.x < NW) sPairN[threadIdx.x] = 0;
@@ -182,22 +189,27 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
     constexpr int NB = PREC == PREC_FP32 ? DP / 2 : DP / 16;
     typedef typename std::conditional<PREC == PREC_FP32, float, i32x4>::type BFrag;
     BFrag bq[QT][NB];
+    // fragments of feature chunk cc (the whole row unless CH)
+    auto load_query = [&](int cc) {
 #pragma unroll
-    for (int qt = 0; qt < QT; ++qt) {
-        const T* qp = (const T*)a.qs + (size_t)(qrow0 + qt * 32 + col) * DP + h * (DP / 2);
-        if constexpr (PREC == PREC_FP32) {
+        for (int qt = 0; qt < QT; ++qt) {
+            const T* qp = (const T*)a.qs + (size_t)(qrow0 + qt * 32 + col) * qstride + (size_t)cc * DP + h * (DP / 2);
+            if constexpr (PREC == PREC_FP32) {
 #pragma unroll
-            for (int j = 0; j < NB; j += 4) {
-                f32x4 v = *(const f32x4*)(qp + j);
-                bq[qt][j] = v[0]; bq[qt][j + 1] = v[1]; bq[qt][j + 2] = v[2]; bq[qt][j + 3] = v[3];
+                for (int j = 0; j < NB; j += 4) {
+                    f32x4 v = *(const f32x4*)(qp + j);
+                    bq[qt][j] = v[0]; bq[qt][j + 1] = v[1]; bq[qt][j + 2] = v[2]; bq[qt][j + 3] = v[3];
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < NB; ++j) bq[qt][j] = *(const i32x4*)(qp + 8 * j);
             }
-        } else {
-#pragma unroll
-            for (int j = 0; j < NB; ++j) bq[qt][j] = *(const i32x4*)(qp + 8 * j);
         }
-    }
+    };
+    load_query(0);
 
     const int64_t ntiles = (r_end - r_begin + BN - 1) / BN;
+    const int64_t nsteps = ntiles * nc;
     // Load plan: LDS chunk L (of NLOAD * 64 per tile) is row L / (CPR + 1),
     // position L % (CPR + 1); the last position of each row and every chunk
     // past the tile are padding (loaded out of range, i.e. zero). Source
@@ -207,7 +219,7 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
     auto chunk_off = [&](int i) -> uint32_t {
         const int L = (wave + i * NW) * 64 + lane;
         const int row = L / (CPR + 1), pos = L % (CPR + 1);
-        return (pos < CPR && row < BN) ? (uint32_t)(row * RB + pos * 16) : 0x7ffffff0u;
+        return (pos < CPR && row < BN) ? (uint32_t)row * rbg + (uint32_t)(pos * 16) : 0x7ffffff0u;
     };
     // narrow rows: the few offsets stay in registers; wide rows recompute them
     constexpr bool KEEP_OFF = LPW <= 4;
@@ -229,16 +241,19 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
         if constexpr (i == 0) return sB0; else if constexpr (i == 1) return sB1;
         else if constexpr (i == 2) return sB2; else return sB3;
     };
-    auto load_tile = [&](int64_t t, char* dstA, float* dstB) {
+    auto load_tile = [&](int64_t u, char* dstA, float* dstB) {
+        const int64_t t = CH ? u / nc : u;
+        const uint32_t coff = CH ? (uint32_t)(u - t * nc) * RB : 0u;  // byte offset of the chunk in a row
         const int64_t r0 = r_begin + t * BN;
         const int64_t rows = r_end - r0 < BN ? r_end - r0 : BN;
-        const __amdgpu_buffer_rsrc_t ra = buffer_rsrc((const char*)a.xs + (size_t)r0 * RB, (uint32_t)(rows * RB));
+        const __amdgpu_buffer_rsrc_t ra = buffer_rsrc((const char*)a.xs + (size_t)r0 * rbg, (uint32_t)rows * rbg);
 #pragma unroll
         for (int i = 0; i < LPW; ++i) {
             const int g = wave + i * NW;
             if (g < NLOAD) {
                 uint32_t off;
                 if constexpr (KEEP_OFF) off = voff[i]; else off = chunk_off(i);
+                if constexpr (CH) off = off == 0x7ffffff0u ? off : off + coff;
                 __builtin_amdgcn_raw_ptr_buffer_load_lds(ra, (lds_char*)(dstA + g * 1024), 16, off, 0, 0, 0);
             }
         }
@@ -267,19 +282,23 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
         tile_barrier();
     };
 
-    // one step: prefetch tile t + NS - 1 into the buffer freed by step t - 1,
-    // compute tile t from buffer CUR, wait for tile t + 1
-    auto step = [&](auto CUR, int64_t t) {
+    f32x16 acc_keep[RTW][QT];  // deep rows: sums carried over the chunks of a tile
+    // one step: prefetch step u + NS - 1 into the buffer freed by step u - 1,
+    // compute step u from buffer CUR, wait for step u + 1
+    auto step = [&](auto CUR, int64_t u) {
         constexpr int c = decltype(CUR)::value;
         constexpr int nxt = (c + NS - 1) % NS;
-        const bool full = t + NS - 1 < ntiles;
-        if (full) load_tile(t + NS - 1, bufA(std::integral_constant<int, nxt>{}), bufB(std::integral_constant<int, nxt>{}));
+        const bool full = u + NS - 1 < nsteps;
+        if (full) load_tile(u + NS - 1, bufA(std::integral_constant<int, nxt>{}), bufB(std::integral_constant<int, nxt>{}));
         const char* cA = bufA(CUR);
         const float* cB = bufB(CUR);
+        const int64_t t = CH ? u / nc : u;
+        const int cc = CH ? (int)(u - t * nc) : 0;  // feature chunk of this step
         bool stored = false;
         if (wave_live) {
             const int64_t r0 = r_begin + t * BN;
-            f32x16 acc[RTW][QT];
+            f32x16 acc_step[RTW][QT];
+            f32x16(&acc)[RTW][QT] = *(CH ? &acc_keep : &acc_step);
 #pragma unroll
             for (int rt = 0; rt < RTW; ++rt) {
                 // accumulator start = -a_x of this lane's 16 rows (4 runs of 4 rows)
@@ -290,8 +309,10 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
                                      b2[0], b2[1], b2[2], b2[3], b3[0], b3[1], b3[2], b3[3]};
                 const char* arow = cA + (rt * 32 + col) * RS + h * (CPR / 2) * 16;
                 if constexpr (PREC == PREC_FP32) {
+                    if (cc == 0) {
 #pragma unroll
-                    for (int qt = 0; qt < QT; ++qt) acc[rt][qt] = init;
+                        for (int qt = 0; qt < QT; ++qt) acc[rt][qt] = init;
+                    }
 #pragma unroll
                     for (int j = 0; j < NB; j += 4) {
                         const f32x4 av = *(const f32x4*)(arow + (j / 4) * 16);
@@ -303,9 +324,11 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
                     }
                 } else {
                     // the first k-step reads the bias registers as its C operand
+                    // (deep rows: only in the first chunk of the tile)
                     const i32x4 a0 = *(const i32x4*)(arow);
 #pragma unroll
-                    for (int qt = 0; qt < QT; ++qt) acc[rt][qt] = mfma16<PREC>(a0, bq[qt][0], init);
+                    for (int qt = 0; qt < QT; ++qt)
+                        acc[rt][qt] = mfma16<PREC>(a0, bq[qt][0], cc == 0 ? init : acc[rt][qt]);
 #pragma unroll
                     for (int j = 1; j < NB; ++j) {
                         const i32x4 av = *(const i32x4*)(arow + j * 16);
@@ -314,7 +337,14 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
                     }
                 }
             }
+            // deep rows: the next chunk's query fragments (in flight during the
+            // test below and the step's final wait)
+            if constexpr (CH) {
+                if (u + 1 < nsteps) load_query(cc + 1 == nc ? 0 : cc + 1);
+            }
             // Hit test: acc >= thr  <=>  L(x) <= tau. Branch-free over the tile.
+            // Deep rows: only after the tile's last chunk.
+            if (!CH || cc == nc - 1) {
             bool hit_rq[RTW][QT];
             bool any = false;
 #pragma unroll
@@ -385,22 +415,23 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
                     }
                 }
             }
+            }  // hit test
         }
         // Stores of the hit path also count in vmcnt: after one, wait for
         // everything rather than rely on the order of loads and stores.
         sync_tiles(full && !stored ? NS - 2 : 0);
     };
 
-    // prologue: tiles 0 .. NS-2 in flight, wait for tile 0
+    // prologue: steps 0 .. NS-2 in flight, wait for step 0
     load_tile(0, sA0, sB0);
-    if constexpr (NS > 2) if (1 < ntiles) load_tile(1, sA1, sB1);
-    if constexpr (NS > 3) if (2 < ntiles) load_tile(2, sA2, sB2);
-    sync_tiles(ntiles >= NS - 1 ? NS - 2 : 0);
-    for (int64_t t = 0; t < ntiles; t += NS) {
-        step(std::integral_constant<int, 0>{}, t);
-        if (t + 1 < ntiles) step(std::integral_constant<int, 1>{}, t + 1);
-        if constexpr (NS > 2) if (t + 2 < ntiles) step(std::integral_constant<int, 2>{}, t + 2);
-        if constexpr (NS > 3) if (t + 3 < ntiles) step(std::integral_constant<int, 3>{}, t + 3);
+    if constexpr (NS > 2) if (1 < nsteps) load_tile(1, sA1, sB1);
+    if constexpr (NS > 3) if (2 < nsteps) load_tile(2, sA2, sB2);
+    sync_tiles(nsteps >= NS - 1 ? NS - 2 : 0);
+    for (int64_t u = 0; u < nsteps; u += NS) {
+        step(std::integral_constant<int, 0>{}, u);
+        if (u + 1 < nsteps) step(std::integral_constant<int, 1>{}, u + 1);
+        if constexpr (NS > 2) if (u + 2 < nsteps) step(std::integral_constant<int, 2>{}, u + 2);
+        if constexpr (NS > 3) if (u + 3 < nsteps) step(std::integral_constant<int, 3>{}, u + 3);
     }
     if constexpr (MODE == SCREEN_LISTS) {
 #pragma unroll
@@ -429,6 +460,8 @@ static ScreenFn pick_dp(int dp) {
 // Wide rows (bf16 / fp16, padded width in {256, 384, 512, 640, 800}): one
 // query tile per wave (its fragments for the whole width stay in registers),
 // 32-row tiles; three LDS stages for k-NN lists, two beside the pair staging.
+// Deeper rows (multiples of 256 features): the deep kernel walks each 64-row
+// tile in 256-feature chunks, reloading the query fragments of each chunk.
 template <int PREC, int MODE>
 static ScreenFn pick_wide(int dp) {
     constexpr int NS = MODE == SCREEN_PAIRS ? 2 : 3;
@@ -438,7 +471,10 @@ static ScreenFn pick_wide(int dp) {
         case 512: return k_screen<PREC, 512, 1, 1, NS, MODE>;
         case 640: return k_screen<PREC, 640, 1, 1, NS, MODE>;
         case 800: return k_screen<PREC, 800, 1, 1, NS, MODE>;
-        default: return nullptr;
+        default:
+            // deeper rows: 256-feature chunks, 64-row tiles, two stages
+            if (dp > WIDE_MAX && dp % DEEP_CHUNK == 0) return k_screen<PREC, DEEP_CHUNK, 1, 2, 2, MODE, true>;
+            return nullptr;
     }
 }
 
@@ -468,16 +504,16 @@ static ScreenFn pick_shape(int dp, int qt, int ns) {
 int screen_dp(int prec, int d) {
     const int p32 = (d + 31) / 32 * 32;
     if (p32 <= 128 || prec == PREC_FP32) return p32;
-    for (int w : {256, 384, 512, 640, 800})
+    for (int w : {256, 384, 512, 640, WIDE_MAX})
         if (d <= w) return w;
-    return p32;  // wider than any screen: exact fallback
+    return (d + DEEP_CHUNK - 1) / DEEP_CHUNK * DEEP_CHUNK;  // deep kernels
 }
 
 // Measured on the 1M x 128 mixture workload: with the candidate appends of a
 // real query, one query tile per wave (occupancy 4) beats two (occupancy 3).
 ScreenShape default_shape(int prec, int dp) {
     (void)prec;
-    return dp > 128 ? ScreenShape{1, 3} : ScreenShape{1, 2};
+    return (dp > 128 && dp <= WIDE_MAX) ? ScreenShape{1, 3} : ScreenShape{1, 2};
 }
 
 ScreenFn pick_screen(int mode, int prec, int dp, ScreenShape shape) {

@@ -279,6 +279,42 @@ def test_wide_screen_matches_oracle(mlk, precision, d):
     assert_parity(gpu_knn(mlk, X, Q, 7, precision, screened=True), oracle.knn(X, Q, 7))
 
 
+@pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize("d", [801, 1024, 1100, 1536])
+def test_deep_screen_matches_oracle(mlk, precision, d):
+    """Rows wider than 800 features: the deep MFMA screen walks each row tile in 256-feature chunks
+    (float32 requests are screened in bf16). Ragged database (not a multiple of the 64-row tile),
+    duplicated rows, queries equal to rows, and no query may need the exact fallback."""
+    from mlxlearn_amd import _lib
+    r = np.random.default_rng(9000 + d)
+    X = r.normal(size=(2999, d)).astype(np.float32)
+    X[50:55] = X[49]
+    Q = np.concatenate([X[:3], X[50:51], r.normal(size=(150, d)).astype(np.float32)])
+    assert_parity(gpu_knn(mlk, X, Q, 7, precision, screened=True), oracle.knn(X, Q, 7))
+    assert _lib.stats()["screen_precision"] == (0x20 if precision == "float16" else 0x10)
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_deep_screen_nearest_and_self_query(mlk, precision):
+    """k = 1 and the self query (own row excluded) on 1,280-feature rows, through the screen."""
+    r = np.random.default_rng(9101)
+    X = r.normal(size=(4100, 1280)).astype(np.float32)
+    X[7] = X[3]
+    Q = X[:300]
+    assert_parity(gpu_knn(mlk, X, Q, 1, precision, screened=True), oracle.knn(X, Q, 1))
+    assert_parity(gpu_knn(mlk, X, Q, 4, precision, exclude_self=True, screened=True),
+                  oracle.knn(X, Q, 4, exclude_self=True))
+
+
+def test_deep_pixels_exact_screen(mlk):
+    """Integer pixels on 1,024-feature rows: exact in fp16, so the screen's band is the accumulation's alone."""
+    r = np.random.default_rng(9202)
+    X = r.integers(0, 256, size=(3000, 1024)).astype(np.float32)
+    Q = np.concatenate([X[:4], r.integers(0, 256, size=(96, 1024)).astype(np.float32)])
+    for p in PRECISIONS:
+        assert_parity(gpu_knn(mlk, X, Q, 3, p, screened=True), oracle.knn(X, Q, 3))
+
+
 def test_wide_pixels_exact_screen(mlk):
     """Workload c3 generator (784 integer pixels) through the wide fp16 screen, no fallback."""
     X, y, Q = oracle.gen_c3(seed=13, n=8000, m=400)
@@ -318,5 +354,28 @@ def test_merge_with_more_new_bounds_than_it_holds(mlk, rng):
     try:
         for p in PRECISIONS:
             assert_parity(gpu_knn(mlk, X, Q, 10, p), want)
+    finally:
+        _lib.set_tuning()
+
+
+@pytest.mark.parametrize("near", [1500, 3900, 5000])
+def test_rank_with_more_survivors_than_one_wave_holds(mlk, rng, near):
+    """`near` rows packed around the first queries, far inside the bf16 screen's error band, with lists large enough
+    to hold them all: more rows survive the bound filter than one wave's 512-row LDS list. Up to 4,096 survivors the
+    query is ranked again with the larger list (no exact fallback); beyond that it takes the exact fallback.
+    Results equal the oracle's either way."""
+    from mlxlearn_amd import _lib
+    X = (rng.normal(size=(near + 1500, 48)) + 10.0).astype(np.float32)
+    base = rng.normal(size=48).astype(np.float32)
+    X[:near] = base + rng.normal(scale=1e-4, size=(near, 48)).astype(np.float32)
+    Q = np.concatenate([base + rng.normal(scale=1e-4, size=(3, 48)), rng.normal(size=(61, 48)) + 10.0]).astype(np.float32)
+    want = oracle.knn(X, Q, 5)
+    _lib.set_tuning(splits=8, capacity=1024)
+    try:
+        for p in PRECISIONS:
+            assert_parity(gpu_knn(mlk, X, Q, 5, p), want)
+        assert_parity(gpu_knn(mlk, X, Q, 5, "bfloat16"), want)
+        st = _lib.stats()
+        assert st["fallback_queries"] == (0 if near <= 3900 else 3), st
     finally:
         _lib.set_tuning()

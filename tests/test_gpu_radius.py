@@ -137,7 +137,7 @@ def test_estimator_radius_neighbors(mlk, rng):
     assert math.isclose(float(np.mean([len(v) for v in i_self])), (wp[-1]) / 2000)
 
 
-@pytest.mark.parametrize("d", [200, 784])
+@pytest.mark.parametrize("d", [200, 784, 1024, 1300])
 def test_wide_rows(mlk, d):
     r = np.random.default_rng(8100 + d)
     X = r.normal(size=(2500, d)).astype(np.float32)
