@@ -120,8 +120,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5"],
-                    help="c2 (default) is the headline; c1, c3, c4, c5 are the other configurations")
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "deep"],
+                    help="c2 (default) is the headline; c1, c3, c4, c5 are the other configurations; "
+                         "deep is 1,024-feature rows (the deep screen)")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="time the heuristic plans instead of autotuned ones")
     ap.add_argument("--mfma-calibration", action="store_true",

@@ -14,6 +14,9 @@ bounded sample on the host cores.
       rows sorted by (distance, index).
 * c5  KMeans: one exact assignment of 10,000,000 x 96 rows to 65,536 centroids
       (bf16 screen, k = 1) + one float64 centroid update.
+* deep  NearestNeighbors(k=10) on 250,000 x 1,024 rows (the c2 mixture generator
+      at 1,024 features), 10,000 queries: rows too wide for register-held queries,
+      screened by the deep kernel in 256-feature chunks (bf16; fp16 reported beside).
 """
 from __future__ import annotations
 
@@ -350,5 +353,55 @@ def finish_c5(args, line, X, C, kc):
     return line
 
 
+def run_deep(args):
+    import torch
+    import mlxlearn_amd
+    import oracle
+    n, d, m, k = 250_000, 1024, 10_000, 10
+    X, Q = oracle.gen_c2(seed=args.seed, n=n, m=m, d=d)
+    index = mlxlearn_amd.DeviceNeighborIndex(torch.from_numpy(X).cuda())
+    Qd = torch.from_numpy(Q).cuda()
+    res, spread, tuned = {}, {}, {}
+    for prec in ("bfloat16", "float16"):
+        first = cold(knn_step(index, Qd, k, prec))
+        if args.autotune:
+            tuned[prec] = tune(index, Qd, k, precision=prec, max_trials=16)
+        t, (dd, ii, st), spread[prec] = timed(knn_step(index, Qd, k, prec), args.steps, args.warmup)
+        spread[prec]["first_call_ms"] = first
+        res[prec] = (t, dd.cpu().numpy(), ii.cpu().numpy(), st)
+        log(f"deep {prec}: {t * 1e3:.3f} ms/step, stats {st}")
+    sample = np.random.default_rng(args.seed).choice(m, args.oracle_sample, replace=False)
+    wd, wi = oracle.knn(X, Q[sample], k)
+    t, dd, ii, st = res["bfloat16"]
+    dp = (d + 255) // 256 * 256
+    flops = 2.0 * m * n * dp
+    line = base_line(args, "exact kNN queries/s 250k x 1024 k=10 (deep screen, bf16 + fp64 rerank)", m / t,
+                     "queries/s", t * 1e3, "bf16", "synthetic (seeded Gaussian mixture, 256 centres, 1% duplicates)",
+                     {"workload": "deep: NearestNeighbors(k=10), database 250,000x1024 float32, 10,000 queries, "
+                                  "bf16 deep MFMA screen (256-feature chunks) + fp64 exact rerank",
+                      "n": n, "d": d, "queries": m, "k": k})
+    line["roofline"] = roofline(flops, st["screen_ms"], "bfloat16", "k_screen<bf16,256,1,2,2,deep>", st["rounds"])
+    line["step_ms"] = spread
+    line["autotune"] = tuned or None
+    line["fp16_path"] = {"value": round(m / res["float16"][0], 1), "ms_per_step": round(res["float16"][0] * 1e3, 3),
+                         "screen_frac": round(flops / (res["float16"][3]["screen_ms"] * 1e-3) / 1e12 / PEAK["float16"], 4)}
+    line["screen_stats"] = {"candidates": st["candidates"], "reranked": st["reranked"],
+                            "fallback_queries": st["fallback_queries"], "rounds": st["rounds"], "splits": st["splits"]}
+    line["index_match"] = {"bf16_vs_oracle": float((ii[sample] == wi).all(axis=1).mean()),
+                           "fp16_vs_oracle": float((res["float16"][2][sample] == wi).all(axis=1).mean()),
+                           "oracle_sample": int(args.oracle_sample),
+                           "max_rel_dist_err": float(np.max(np.abs(dd[sample] - wd) / np.maximum(wd, 1e-300)))}
+    if not args.no_cpu:
+        port = oracle.GramPort(X)
+        qs = Q[:64]
+        t0 = time.perf_counter()
+        port.query(qs, k, block=64)
+        tc = time.perf_counter() - t0
+        line["cpu_baseline"] = {"value": round(len(qs) / tc, 1), "unit": "queries/s",
+                                "cores": len(os.sched_getaffinity(0)), "kind": "port",
+                                "sample": "64 of the 10,000 queries (numpy float64 BLAS CPUIndex._gram restatement)"}
+    return line
+
+
 def run(args):
-    return {"c1": run_c1, "c3": run_c3, "c4": run_c4, "c5": run_c5}[args.workload](args)
+    return {"c1": run_c1, "c3": run_c3, "c4": run_c4, "c5": run_c5, "deep": run_deep}[args.workload](args)
