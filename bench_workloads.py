@@ -373,7 +373,7 @@ def run_deep(args):
     sample = np.random.default_rng(args.seed).choice(m, args.oracle_sample, replace=False)
     wd, wi = oracle.knn(X, Q[sample], k)
     t, dd, ii, st = res["bfloat16"]
-    dp = (d + 255) // 256 * 256
+    dp = (d + 255) // 256 * 256  # deep kernels pad rows to 256-feature chunks
     flops = 2.0 * m * n * dp
     line = base_line(args, "exact kNN queries/s 250k x 1024 k=10 (deep screen, bf16 + fp64 rerank)", m / t,
                      "queries/s", t * 1e3, "bf16", "synthetic (seeded Gaussian mixture, 256 centres, 1% duplicates)",

@@ -477,6 +477,7 @@ int radius_query_impl(mlk_index* index, const float* Q, int64_t m, double R2, in
             sa.exclude = exclude;
             sa.xcd_map = (S % 8) == 0;
             sa.nchunk = screen_chunks(dp);
+            sa.qgroup = sa.nchunk > 1 ? DEEP_QGROUP : 1;
             sa.pairs = ws[9].as<int2>();
             sa.pair_count = dcount;
             sa.pair_cap = cap;

@@ -99,10 +99,14 @@ struct ScreenArgs {
     // rows wider than the widest register-held query (deep kernels): the row
     // is DEEP_CHUNK features per chunk, nchunk chunks; 1 for every other kernel
     int nchunk;
+    // with xcd_map: query blocks per group of an XCD's consecutive blocks
+    // (0 or 1: every query block of a split before the next split)
+    int qgroup;
 };
 
 constexpr int WIDE_MAX = 800;    // widest row whose query fragments stay in registers
 constexpr int DEEP_CHUNK = 256;  // features per chunk of the deep kernels
+constexpr int DEEP_QGROUP = 8;   // query blocks per XCD block group of the deep kernels
 
 constexpr int SCREEN_LISTS = 0;  // k-NN: per-(query, split) candidate lists
 constexpr int SCREEN_PAIRS = 1;  // radius: global (query, row) pair buffer

@@ -37,7 +37,17 @@ __device__ __forceinline__ f32x16 mfma16(const i32x4& a, const i32x4& b, const f
 // XCDs), so they stream the same rows through that XCD's L2 together.
 __device__ __forceinline__ void block_item(const ScreenArgs& a, int& qb, int& split) {
     const int b = blockIdx.x;
-    if (a.xcd_map) {
+    if (a.xcd_map && a.qgroup > 1) {
+        // an XCD's consecutive blocks cover qgroup query blocks x its splits, so
+        // the blocks resident on it at one time share both their query rows
+        // and their database rows in its L2 (deep rows reload query fragments)
+        const int xcd = b & 7, slot = b >> 3, sg = a.S >> 3;
+        const int g = slot / (a.qgroup * sg);
+        const int gsize = min(a.qgroup, a.QB - g * a.qgroup);
+        const int local = slot - g * a.qgroup * sg;
+        qb = g * a.qgroup + local % gsize;
+        split = (local / gsize) * 8 + xcd;
+    } else if (a.xcd_map) {
         const int xcd = b & 7, slot = b >> 3;
         qb = slot % a.QB;
         split = (slot / a.QB) * 8 + xcd;
@@ -110,7 +120,7 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
     static_assert(NS >= 2 && NS <= 4, "stages");
     static_assert(QT == 1 || QT == 2, "query tiles");
     static_assert(RTW == 1 || RTW == 2, "row tiles per step");
-    static_assert(!CH || NS == 2, "deep rows: two stages (each step waits for all of its loads)");
+    static_assert(!CH || NS == 2, "deep rows: two stages (each step waits for its query reload)");
     static_assert(NS * (ABYTES + 512) + (MODE == SCREEN_PAIRS ? NW * PAIR_STAGE * 8 : 0) <= 160 * 1024,
                   "LDS of one block");
     // distinct buffers (not one array) so the compiler can tell the tile
@@ -189,8 +199,8 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
     constexpr int NB = PREC == PREC_FP32 ? DP / 2 : DP / 16;
     typedef typename std::conditional<PREC == PREC_FP32, float, i32x4>::type BFrag;
     BFrag bq[QT][NB];
-    // fragments of feature chunk cc (the whole row unless CH)
-    auto load_query = [&](int cc) {
+    // fragments of feature chunk cc (the whole row unless CH) into dst
+    auto load_query = [&](BFrag(&dst)[QT][NB], int cc) {
 #pragma unroll
         for (int qt = 0; qt < QT; ++qt) {
             const T* qp = (const T*)a.qs + (size_t)(qrow0 + qt * 32 + col) * qstride + (size_t)cc * DP + h * (DP / 2);
@@ -198,15 +208,15 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
 #pragma unroll
                 for (int j = 0; j < NB; j += 4) {
                     f32x4 v = *(const f32x4*)(qp + j);
-                    bq[qt][j] = v[0]; bq[qt][j + 1] = v[1]; bq[qt][j + 2] = v[2]; bq[qt][j + 3] = v[3];
+                    dst[qt][j] = v[0]; dst[qt][j + 1] = v[1]; dst[qt][j + 2] = v[2]; dst[qt][j + 3] = v[3];
                 }
             } else {
 #pragma unroll
-                for (int j = 0; j < NB; ++j) bq[qt][j] = *(const i32x4*)(qp + 8 * j);
+                for (int j = 0; j < NB; ++j) dst[qt][j] = *(const i32x4*)(qp + 8 * j);
             }
         }
     };
-    load_query(0);
+    load_query(bq, 0);
 
     const int64_t ntiles = (r_end - r_begin + BN - 1) / BN;
     const int64_t nsteps = ntiles * nc;
@@ -289,11 +299,11 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
         constexpr int c = decltype(CUR)::value;
         constexpr int nxt = (c + NS - 1) % NS;
         const bool full = u + NS - 1 < nsteps;
+        const int64_t t = CH ? u / nc : u;
+        const int cc = CH ? (int)(u - t * nc) : 0;  // feature chunk of this step
         if (full) load_tile(u + NS - 1, bufA(std::integral_constant<int, nxt>{}), bufB(std::integral_constant<int, nxt>{}));
         const char* cA = bufA(CUR);
         const float* cB = bufB(CUR);
-        const int64_t t = CH ? u / nc : u;
-        const int cc = CH ? (int)(u - t * nc) : 0;  // feature chunk of this step
         bool stored = false;
         if (wave_live) {
             const int64_t r0 = r_begin + t * BN;
@@ -337,10 +347,10 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
                     }
                 }
             }
-            // deep rows: the next chunk's query fragments (in flight during the
-            // test below and the step's final wait)
+            // deep rows: the next step's fragments replace this step's once its
+            // MFMAs are issued (two stages: the step's final wait covers them)
             if constexpr (CH) {
-                if (u + 1 < nsteps) load_query(cc + 1 == nc ? 0 : cc + 1);
+                if (u + 1 < nsteps) load_query(bq, cc + 1 == nc ? 0 : cc + 1);
             }
             // Hit test: acc >= thr  <=>  L(x) <= tau. Branch-free over the tile.
             // Deep rows: only after the tile's last chunk.
@@ -460,8 +470,9 @@ static ScreenFn pick_dp(int dp) {
 // Wide rows (bf16 / fp16, padded width in {256, 384, 512, 640, 800}): one
 // query tile per wave (its fragments for the whole width stay in registers),
 // 32-row tiles; three LDS stages for k-NN lists, two beside the pair staging.
-// Deeper rows (multiples of 256 features): the deep kernel walks each 64-row
-// tile in 256-feature chunks, reloading the query fragments of each chunk.
+// Deeper rows (multiples of DEEP_CHUNK features): the deep kernel walks each
+// 64-row tile in DEEP_CHUNK-feature chunks, reloading the query fragments of
+// each chunk (they do not fit in registers for the whole row).
 template <int PREC, int MODE>
 static ScreenFn pick_wide(int dp) {
     constexpr int NS = MODE == SCREEN_PAIRS ? 2 : 3;
@@ -472,7 +483,7 @@ static ScreenFn pick_wide(int dp) {
         case 640: return k_screen<PREC, 640, 1, 1, NS, MODE>;
         case 800: return k_screen<PREC, 800, 1, 1, NS, MODE>;
         default:
-            // deeper rows: 256-feature chunks, 64-row tiles, two stages
+            // deeper rows: DEEP_CHUNK-feature chunks, 64-row tiles, two stages
             if (dp > WIDE_MAX && dp % DEEP_CHUNK == 0) return k_screen<PREC, DEEP_CHUNK, 1, 2, 2, MODE, true>;
             return nullptr;
     }
