@@ -51,9 +51,10 @@ def check_update(X, labels, kc):
 
 
 @pytest.mark.parametrize("precision", PRECISIONS)
-@pytest.mark.parametrize("n,k,d", [(7, 3, 1), (33, 17, 9), (129, 65, 31), (83, 11, 128), (5000, 700, 96)])
+@pytest.mark.parametrize("n,k,d", [(7, 3, 1), (33, 17, 9), (129, 65, 31), (83, 11, 128), (5000, 700, 96),
+                                   (3000, 300, 784), (2000, 257, 1100)])
 def test_assignment_matches_oracle(mlk, precision, n, k, d):
-    """Shapes of tests/unit/test_e_means.py:101-113 plus a wider one."""
+    """Shapes of tests/unit/test_e_means.py:101-113 plus wider ones (the wide and the deep screens)."""
     r = np.random.default_rng(2600 + d)
     X = r.normal(size=(n, d)).astype(np.float32)
     C = r.normal(size=(k, d)).astype(np.float32)

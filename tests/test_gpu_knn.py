@@ -244,14 +244,15 @@ def test_exact_order_across_float32_range(mlk):
         np.testing.assert_array_equal(i, want[1])
 
 
-def test_batched_equals_separate(mlk, rng):
-    X = rng.normal(size=(8000, 32)).astype(np.float32)
+@pytest.mark.parametrize("d", [32, 1000])
+def test_batched_equals_separate(mlk, rng, d):
+    X = rng.normal(size=(8000, d)).astype(np.float32)
     sizes = [1, 17, 300, 5, 64]
-    batches = [rng.normal(size=(s, 32)).astype(np.float32) for s in sizes]
+    batches = [rng.normal(size=(s, d)).astype(np.float32) for s in sizes]
     index = mlk.DeviceNeighborIndex(X)
     res = index.query_batched(batches, 7, precision="bfloat16")
-    for b, (d, i) in zip(batches, res):
-        assert_parity((d.cpu().numpy(), i.cpu().numpy()), oracle.knn(X, b, 7))
+    for b, (dd, i) in zip(batches, res):
+        assert_parity((dd.cpu().numpy(), i.cpu().numpy()), oracle.knn(X, b, 7))
     index.close()
 
 
