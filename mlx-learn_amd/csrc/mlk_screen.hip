@@ -199,7 +199,7 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
     constexpr int NB = PREC == PREC_FP32 ? DP / 2 : DP / 16;
     typedef typename std::conditional<PREC == PREC_FP32, float, i32x4>::type BFrag;
     BFrag bq[QT][NB];
-    // fragments of feature chunk cc (the whole row unless CH) into dst
+    // deep rows: fragments of feature chunk cc into dst
     auto load_query = [&](BFrag(&dst)[QT][NB], int cc) {
 #pragma unroll
         for (int qt = 0; qt < QT; ++qt) {
@@ -216,7 +216,23 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
             }
         }
     };
-    load_query(bq, 0);
+    // the first chunk (the whole row unless CH), written out rather than through
+    // the lambda: the narrow kernels then keep their register budget (128 at
+    // two query tiles of 96 features, occupancy 4)
+#pragma unroll
+    for (int qt = 0; qt < QT; ++qt) {
+        const T* qp = (const T*)a.qs + (size_t)(qrow0 + qt * 32 + col) * qstride + h * (DP / 2);
+        if constexpr (PREC == PREC_FP32) {
+#pragma unroll
+            for (int j = 0; j < NB; j += 4) {
+                f32x4 v = *(const f32x4*)(qp + j);
+                bq[qt][j] = v[0]; bq[qt][j + 1] = v[1]; bq[qt][j + 2] = v[2]; bq[qt][j + 3] = v[3];
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < NB; ++j) bq[qt][j] = *(const i32x4*)(qp + 8 * j);
+        }
+    }
 
     const int64_t ntiles = (r_end - r_begin + BN - 1) / BN;
     const int64_t nsteps = ntiles * nc;
