@@ -17,6 +17,36 @@ static Tuning g_tuning;
 static TestHooks g_hooks;
 static std::mutex g_config_lock;
 
+// Event pool: (device, event) pairs, at most EVENT_POOL_MAX kept.
+static std::mutex g_event_lock;
+static std::vector<std::pair<int, hipEvent_t>> g_event_pool;
+constexpr size_t EVENT_POOL_MAX = 256;
+
+hipError_t event_acquire(int device, hipEvent_t* out) {
+    {
+        std::lock_guard<std::mutex> g(g_event_lock);
+        for (size_t i = g_event_pool.size(); i-- > 0;) {
+            if (g_event_pool[i].first == device) {
+                *out = g_event_pool[i].second;
+                g_event_pool.erase(g_event_pool.begin() + (std::ptrdiff_t)i);
+                return hipSuccess;
+            }
+        }
+    }
+    return hipEventCreate(out);
+}
+
+void event_release(int device, hipEvent_t e) {
+    {
+        std::lock_guard<std::mutex> g(g_event_lock);
+        if (g_event_pool.size() < EVENT_POOL_MAX) {
+            g_event_pool.emplace_back(device, e);
+            return;
+        }
+    }
+    (void)hipEventDestroy(e);
+}
+
 void set_error(const std::string& msg) { g_error = msg; }
 Tuning tuning_snapshot() {
     std::lock_guard<std::mutex> g(g_config_lock);

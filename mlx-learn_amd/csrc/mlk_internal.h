@@ -36,16 +36,30 @@ struct Status {
 
 // HIP events owned by one call: every event made here is destroyed when the
 // owner goes out of scope, on every return path.
+// Events come from a per-device pool (creating and destroying a dozen events
+// per call costs more than a small call's kernels); released events are
+// re-recorded by later calls.
+hipError_t event_acquire(int device, hipEvent_t* out);
+void event_release(int device, hipEvent_t e);
+
 struct Events {
     std::vector<hipEvent_t> ev;
+    int device = -1;
     Events() = default;
     Events(const Events&) = delete;
     Events& operator=(const Events&) = delete;
     ~Events() {
-        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+        for (hipEvent_t e : ev) event_release(device, e);
     }
     hipError_t make(hipEvent_t* out) {
-        const hipError_t r = hipEventCreate(out);
+        if (device < 0) {
+            const hipError_t g = hipGetDevice(&device);
+            if (g != hipSuccess) {
+                device = -1;
+                return g;
+            }
+        }
+        const hipError_t r = event_acquire(device, out);
         if (r == hipSuccess) ev.push_back(*out);
         return r;
     }
