@@ -10,7 +10,8 @@ bounded sample on the host cores.
 * c3  KNeighborsClassifier(k=3, uniform) on 60,000 x 784 integer pixels, 10,000
       queries: fp16 screen (integer data are exact in fp16) + exact rank + device votes.
 * c4  radius_neighbors on 2,000,000 x 32 (uniform cube + 20% clumps), 256
-      batches of 1..4096 queries in one call (CSR split per batch), fp32 screen,
+      batches of 1..4096 queries in one call (CSR split per batch), float32 request
+      (screened in bf16 by the library; results do not depend on it),
       rows sorted by (distance, index).
 * c5  KMeans: one exact assignment of 10,000,000 x 96 rows to 65,536 centroids
       (bf16 screen, k = 1) + one float64 centroid update.
@@ -251,12 +252,15 @@ def run_c4(args):
         ok += int(np.array_equal(got_i, wi) and np.allclose(got_d, wd, rtol=1e-12, atol=0))
     m = Q.shape[0]
     flops = 2.0 * m * X.shape[0] * X.shape[1]
+    # the library screens float32 radius requests in bf16 (results do not depend on it)
+    sp = {0x00: ("float32", "fp32"), 0x10: ("bfloat16", "bf16"), 0x20: ("float16", "fp16")}[int(st["screen_precision"])]
     line = base_line(args, "radius_neighbors queries/s 2M x 32, 256 batches, sorted CSR", m / t, "queries/s", t * 1e3,
-                     "fp32", "synthetic (seeded uniform cube + 20% Gaussian clumps, calibrated radius)",
+                     sp[1], "synthetic (seeded uniform cube + 20% Gaussian clumps, calibrated radius)",
                      {"workload": f"c4: radius_neighbors(r={radius}, sort_results=True), database 2,000,000x32 float32, "
-                                  f"{len(sizes)} batches ({m} queries) in one call, fp32 MFMA screen + exact band",
+                                  f"{len(sizes)} batches ({m} queries) in one call, {sp[1]} MFMA screen of a float32 "
+                                  "request + exact float64 band",
                       "n": X.shape[0], "d": X.shape[1], "queries": m, "batches": len(sizes)})
-    line["roofline"] = roofline(flops, st["screen_ms"], "float32", "k_screen<fp32,32,pairs>", 1)
+    line["roofline"] = roofline(flops, st["screen_ms"], sp[0], f"k_screen<{sp[1]},32,pairs>", 1)
     line["step_ms"] = spread
     line["autotune"] = tuned
     line["result"] = {"nnz": int(indptr[-1]), "median_per_query": float(np.median(np.diff(indptr))),

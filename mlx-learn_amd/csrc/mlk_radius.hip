@@ -410,7 +410,7 @@ int radius_query_impl(mlk_index* index, const float* Q, int64_t m, double R2, in
         return MLK_OK;
     }
 
-    const int prec = effective_prec((int)((flags & MLK_PREC_MASK) >> 4), d);
+    const int prec = radius_prec((int)((flags & MLK_PREC_MASK) >> 4), d);
     st.screen_precision = (uint32_t)prec << 4;  // the precision actually screened
     MLK_TRY(prepare_mirror(index, prec, stream));
     Mirror& mr = index->mirror[prec];

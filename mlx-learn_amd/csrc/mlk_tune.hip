@@ -146,7 +146,8 @@ static double list_bytes(const Plan& p, int64_t m) {
 
 int plan_info_impl(const mlk_index* index, int64_t m, int k, uint32_t flags, mlk_plan* out) {
     const int kind = (flags & MLK_TUNE_RADIUS) ? PLAN_RADIUS : PLAN_KNN;
-    const int prec = effective_prec((int)((flags & MLK_PREC_MASK) >> 4), index->d);
+    const int req = (int)((flags & MLK_PREC_MASK) >> 4);
+    const int prec = kind == PLAN_RADIUS ? radius_prec(req, index->d) : effective_prec(req, index->d);
     const int dp = screen_dp(prec, index->d);
     *out = mlk_plan{};
     bool tuned = false;
@@ -260,7 +261,8 @@ int autotune_impl(mlk_index* index, const float* Q, int64_t m, int k, double R2,
                   mlk_plan* out, hipStream_t stream) {
     const int kind = (flags & MLK_TUNE_RADIUS) ? PLAN_RADIUS : PLAN_KNN;
     const uint32_t qflags = flags & (MLK_PREC_MASK | MLK_SORT_RESULTS | MLK_NEED_DISTANCE);
-    const int prec = effective_prec((int)((flags & MLK_PREC_MASK) >> 4), index->d);
+    const int req = (int)((flags & MLK_PREC_MASK) >> 4);
+    const int prec = kind == PLAN_RADIUS ? radius_prec(req, index->d) : effective_prec(req, index->d);
     const int dp = screen_dp(prec, index->d);
     const int64_t n = index->n;
     const int mode = kind == PLAN_KNN ? SCREEN_LISTS : SCREEN_PAIRS;
