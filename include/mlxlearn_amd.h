@@ -56,6 +56,8 @@ typedef struct mlk_radius_result mlk_radius_result;
 #define MLK_HOOK_CAPACITY 0x2u         /* candidate capacity per (query, split) = param     */
 #define MLK_HOOK_FAIL 0x4u             /* the next library call fails with MLK_EDEVICE      */
 #define MLK_HOOK_FORCE_UNCERTIFIED 0x8u /* rows listed by mlk_set_forced_rows lose certificates */
+#define MLK_HOOK_SORT_LIMIT 0x10u      /* radius: counts sorted or selected in one pass must be <= param
+                                          (default INT32_MAX, the limit of the library sorts)       */
 
 /* Per-call statistics of the last query on this thread. */
 typedef struct {

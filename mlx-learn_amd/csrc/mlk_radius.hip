@@ -312,6 +312,22 @@ __global__ void k_row_fix_runs(const uint32_t* __restrict__ keys, const uint8_t*
     }
 }
 
+// The library sorts and selects take int counts (and the row sort 32-bit
+// positions): a count above INT32_MAX (or the test hook's lower limit) is
+// refused with MLK_ECAPACITY instead of sorting a wrapped count.
+static int check_sort_count(int64_t count, const char* what) {
+    int64_t limit = INT32_MAX;
+    const TestHooks hk = hooks_snapshot();
+    if (hk.hooks & MLK_HOOK_SORT_LIMIT) limit = std::min<int64_t>(limit, std::max<int64_t>(hk.param, 0));
+    if (count > limit) {
+        set_error(std::string("radius search: ") + what + " (" + std::to_string(count) +
+                  ") exceeds the one-pass sort limit (" + std::to_string(limit) +
+                  "); split the queries into smaller calls");
+        return MLK_ECAPACITY;
+    }
+    return MLK_OK;
+}
+
 // Every CSR row ordered by (distance, index). A stable segmented radix sort on
 // 32-bit keys (the float32 rounding of the distance) and 32-bit positions
 // orders the rows up to runs of equal keys, whose entries arrive in ascending
@@ -322,6 +338,8 @@ __global__ void k_row_fix_runs(const uint32_t* __restrict__ keys, const uint8_t*
 static int sort_rows_by_distance(mlk_radius_result* res, const int64_t* indptr, int64_t m, int64_t nnz, DevBuf* ws,
                                  hipStream_t stream) {
     if (!nnz) return MLK_OK;
+    MLK_TRY(check_sort_count(nnz, "result entries"));
+    MLK_TRY(check_sort_count(m, "query rows"));
     DevBuf& tmp = ws[14];
     DevBuf& dist_out = ws[19];
     DevBuf& idx_out = ws[20];
@@ -395,6 +413,7 @@ int radius_query_impl(mlk_index* index, const float* Q, int64_t m, double R2, in
     if (!r2_key(R2, rc.key)) {
         const int64_t per = n - (exclude ? 1 : 0);
         const int64_t total = m * per;
+        if (sort_rows) MLK_TRY(check_sort_count(total, "result entries"));
         MLK_TRY(res->indices.reserve((size_t)std::max<int64_t>(total, 1) * sizeof(int64_t)));
         if (res->has_dist) MLK_TRY(res->dist.reserve((size_t)std::max<int64_t>(total, 1) * sizeof(double)));
         hipLaunchKernelGGL(k_everything, dim3(blocks_for(total)), dim3(256), 0, stream, X, n, d, Q, m, q_offset,
@@ -507,6 +526,7 @@ int radius_query_impl(mlk_index* index, const float* Q, int64_t m, double R2, in
     if (hipEventElapsedTime(&ms, e0, e1) == hipSuccess) st.screen_ms = ms;
     const int64_t np = (int64_t)count;
     st.candidates = np;
+    MLK_TRY(check_sort_count(np, "candidate pairs"));
 
     // sort pairs by (query, row): keys (query << rb) | row, rb = bits of n
     int rb = 1;

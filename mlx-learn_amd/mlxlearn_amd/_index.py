@@ -102,13 +102,25 @@ class DeviceNeighborIndex:
         except Exception:
             pass
 
+    def _check_width(self, Q):
+        if hasattr(Q, "ndim") and (Q.ndim != 2 or Q.shape[1] != self.n_features):
+            raise ValueError(f"queries must be a two-dimensional matrix with {self.n_features} features, "
+                             f"got shape {tuple(Q.shape)}")
+
+    def _queries(self, Q):
+        """Q as a float32 device matrix of n_features columns; the C ABI cannot check the width."""
+        self._check_width(Q)
+        Qd = to_device_f32(Q)
+        if Qd.ndim != 2 or Qd.shape[1] != self.n_features:
+            raise ValueError(f"queries must be a two-dimensional matrix with {self.n_features} features, "
+                             f"got shape {tuple(Qd.shape)}")
+        return Qd
+
     def query(self, Q, k, *, return_distance=True, exclude_self=False, q_offset=0, precision="float32",
               squared=False):
         """(float64 distances | None, int64 indices) as CUDA tensors, ordered by (S, index)."""
         torch = _torch()
-        Qd = to_device_f32(Q)
-        if Qd.ndim != 2 or Qd.shape[1] != self.n_features:
-            raise ValueError(f"queries must have {self.n_features} features")
+        Qd = self._queries(Q)
         m, k = int(Qd.shape[0]), int(k)
         idx = torch.empty((m, k), dtype=torch.int64, device=self._device)
         dist = torch.empty((m, k), dtype=torch.float64, device=self._device) if return_distance else None
@@ -121,7 +133,9 @@ class DeviceNeighborIndex:
     def query_batched(self, batches, k, *, return_distance=True, precision="float32"):
         """Several query sets answered by one screen launch per round; list of results."""
         torch = _torch()
-        parts = [to_device_f32(b) for b in batches]
+        for b in batches:
+            self._check_width(b)
+        parts = [self._queries(b) for b in batches]
         sizes = [int(p.shape[0]) for p in parts]
         offsets = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
         Qd = torch.cat(parts) if parts else torch.empty((0, self.n_features), device=self._device)
@@ -141,7 +155,7 @@ class DeviceNeighborIndex:
                precision="float32"):
         """Exact radius CSR as CUDA tensors: (indptr int64, indices int64, distances float64 | None)."""
         torch = _torch()
-        Qd = to_device_f32(Q)
+        Qd = self._queries(Q)
         m = int(Qd.shape[0])
         indptr = torch.empty(m + 1, dtype=torch.int64, device=self._device)
         flags = precision_flag(precision) | (_lib.EXCLUDE_SELF if exclude_self else 0)
@@ -170,7 +184,7 @@ class DeviceNeighborIndex:
         torch = _torch()
         if (codes is None) == (targets is None):
             raise ValueError("give exactly one of codes and targets")
-        Qd = to_device_f32(Q)
+        Qd = self._queries(Q)
         m = int(Qd.shape[0])
         indptr = torch.empty(m + 1, dtype=torch.int64, device=self._device)
         flags = precision_flag(precision) | (_lib.EXCLUDE_SELF if exclude_self else 0)
@@ -200,9 +214,7 @@ class DeviceNeighborIndex:
         k-NN with ``k``; radius search with ``radius_sq`` (R2) instead. Returns the
         chosen plan with its time and the untuned plan's time (milliseconds).
         """
-        Qd = to_device_f32(Q)
-        if Qd.ndim != 2 or Qd.shape[1] != self.n_features:
-            raise ValueError(f"queries must have {self.n_features} features")
+        Qd = self._queries(Q)
         if (k is None) == (radius_sq is None):
             raise ValueError("give exactly one of k and radius_sq")
         flags = precision_flag(precision)

@@ -27,6 +27,7 @@ HOOK_FORCE_FALLBACK = 0x1
 HOOK_CAPACITY = 0x2
 HOOK_FAIL = 0x4
 HOOK_FORCE_UNCERTIFIED = 0x8
+HOOK_SORT_LIMIT = 0x10
 TUNE_RADIUS = 0x1000
 MAX_ROUNDS = 8
 

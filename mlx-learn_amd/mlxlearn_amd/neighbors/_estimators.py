@@ -13,7 +13,7 @@ import warnings
 
 import numpy as np
 from sklearn.base import BaseEstimator, ClassifierMixin, RegressorMixin
-from sklearn.utils.validation import check_is_fitted
+from sklearn.utils.validation import _is_arraylike_not_scalar, check_is_fitted
 
 from .. import _lib
 from .._diagnostics import record_event
@@ -207,20 +207,7 @@ class RadiusNeighborsClassifier(NeighborsMixin, RadiusMixin, ClassifierMixin, Ba
             exclude_self=query_is_train, precision=self._index_.precision)
         self._set_query_backend("gpu")
         record_event(type(self).__name__, "predict", shape=Q.shape, weights=self.weights)
-        votes = votes.cpu().numpy()
-        counts = counts.cpu().numpy()
-        outliers = np.flatnonzero(counts == 0)
-        if outliers.size > 0 and self.outlier_label_ is None:
-            raise ValueError(f"No neighbors found for test samples {outliers!r}, you can try using larger radius, "
-                             "giving a label for outliers, or considering removing them from your dataset.")
-        if outliers.size > 0:
-            label = self.outlier_label_[0]
-            match = np.flatnonzero(self.classes_ == label)
-            if match.size == 1:
-                votes[outliers, match[0]] = 1.0
-            else:
-                warnings.warn(f"Outlier label {label} is not in training classes. All class probabilities of "
-                              "outliers will be assigned with 0.", stacklevel=3)
+        votes = _fill_outliers(votes.cpu().numpy(), counts.cpu().numpy(), self.outlier_label_, self.classes_)
         normalizer = votes.sum(axis=1)[:, None]
         normalizer[normalizer == 0.0] = 1.0
         return votes / normalizer
@@ -288,26 +275,50 @@ class RadiusNeighborsRegressor(NeighborsMixin, RadiusMixin, RegressorMixin, Base
 
 
 def _resolve_outlier_label(outlier_label, classes, encoded_y):
-    """``outlier_label_`` for a single output, as scikit-learn fills it (reference _radius.py:341-380)."""
+    """``outlier_label_`` of a single-output classifier: None or a one-element list.
+
+    scikit-learn's documented contract for ``outlier_label``: ``None`` makes an
+    empty neighbourhood an error at prediction; ``"most_frequent"`` takes the
+    training class with the largest count (argmax of the class counts, so the
+    smallest class on a tie); anything else is a manual label, a scalar or a
+    one-element list of scalars (one entry per output), whose type must not
+    widen the classes' dtype. Bad values raise scikit-learn's errors.
+    """
     if outlier_label is None:
         return None
     if isinstance(outlier_label, str) and outlier_label == "most_frequent":
         counts = np.bincount(encoded_y, minlength=len(classes))
-        return [classes[int(counts.argmax())]]
-
-    def arraylike(value):
-        return (hasattr(value, "__len__") or hasattr(value, "shape") or hasattr(value, "__array__")) \
-            and not isinstance(value, (str, bytes))
-
-    if arraylike(outlier_label):
-        if len(outlier_label) != 1:
-            raise ValueError(f"The length of outlier_label: {outlier_label} is inconsistent with the output "
-                             "length: 1")
-        label = list(outlier_label)[0]
-    else:
-        label = outlier_label
-    if arraylike(label):
+        return [classes[int(np.argmax(counts))]]
+    given = list(outlier_label) if _is_arraylike_not_scalar(outlier_label) else [outlier_label]
+    if len(given) != 1:
+        raise ValueError(f"The length of outlier_label: {outlier_label} is inconsistent with the output "
+                         "length: 1")
+    label = given[0]
+    if _is_arraylike_not_scalar(label):
         raise TypeError(f"The outlier_label of classes {classes} is supposed to be a scalar, got {label}.")
-    if np.append(classes, label).dtype != classes.dtype:
+    if np.promote_types(classes.dtype, np.asarray(label).dtype) != classes.dtype:
         raise TypeError(f"The dtype of outlier_label {label} is inconsistent with classes {classes} in y.")
     return [label]
+
+
+def _fill_outliers(votes, counts, outlier_label_, classes):
+    """Votes of queries without neighbours, in place (scikit-learn's contract).
+
+    No outlier label: an error naming the empty queries. A label among the
+    classes: a one-hot vote for it. A label outside the classes: all-zero
+    votes (every class probability 0) and a warning.
+    """
+    empty = np.flatnonzero(counts == 0)
+    if empty.size == 0:
+        return votes
+    if outlier_label_ is None:
+        raise ValueError(f"No neighbors found for test samples {empty!r}, you can try using larger radius, "
+                         "giving a label for outliers, or considering removing them from your dataset.")
+    label = outlier_label_[0]
+    column = int(np.searchsorted(classes, label)) if classes.size else 0
+    if column < classes.size and classes[column] == label:
+        votes[empty, column] = 1.0
+    else:
+        warnings.warn(f"Outlier label {label} is not in training classes. All class probabilities of "
+                      "outliers will be assigned with 0.", stacklevel=4)
+    return votes

@@ -12,13 +12,13 @@ Contents
   coordinates, lower index on exact ties.
 * ``integer_key`` / ``integer_knn`` -- pure-Python big-integer restatement of
   ``src/mlxlearn/_kernels/neighbors_cpu.py:39-47, 174-181`` for tiny cases.
-* ``direct_oracle`` -- restates ``tests/unit/test_neighbors_precision.py:25-39``
-  (float64 direct differences + lexsort); not exact on near-ties, used only
-  where the float64 order is provably unambiguous.
-* ``gram_port`` -- numpy restatement of the reference CPU path
-  ``CPUIndex._gram`` (``neighbors_cpu.py:314-364``), used as the timed CPU
+* ``radius_band`` -- the outward-rounded float64 band of
+  ``radius_contract.thresholds`` (``radius_contract.py:129-153``), via the C
+  oracle.
+* ``GramPort`` -- numpy restatement of the reference CPU path
+  ``CPUIndex._gram`` / ``_direct`` / ``_exact_order``
+  (``neighbors_cpu.py:174-181, 213-262, 314-364``), used as the timed CPU
   baseline.
-* ``radius_thresholds`` -- restates ``radius_contract.thresholds``.
 * ``gen_*`` -- seeded synthetic generators for the five benchmark workloads.
 """
 from __future__ import annotations
@@ -223,77 +223,163 @@ def integer_knn(X, Q, k, *, exclude_self=False, q_offset=0):
     return out
 
 
-def direct_oracle(train, queries, k, *, exclude_self=False):
-    """Float64 direct differences + lexsort (tests/unit/test_neighbors_precision.py:25-39)."""
-    train64 = np.asarray(train, dtype=np.float64)
-    query64 = np.asarray(queries, dtype=np.float64)
-    ids = np.arange(len(train), dtype=np.int64)
-    distances, indices = [], []
-    for row, query in enumerate(query64):
-        delta = train64 - query
-        squared = np.sum(delta * delta, axis=1, dtype=np.float64)
-        if exclude_self:
-            squared[row] = np.inf
-        order = np.lexsort((ids, squared))[:k]
-        distances.append(np.sqrt(squared[order]))
-        indices.append(order)
-    return np.asarray(distances), np.asarray(indices)
-
-
-def radius_thresholds(d, R2):
-    """(f64_in, f64_out) band of radius_contract.thresholds (radius_contract.py:129-153)."""
-    d = int(d)
-    R2 = float(R2)
-    if math.isinf(R2):
-        return math.inf, math.inf
-    if R2 == 0.0:
-        return 0.0, 0.0
-    u = 2.0 ** -53
-    g = (d + 2) * u / (1.0 - (d + 2) * u)
-    dn = lambda v: math.nextafter(v, -math.inf)  # noqa: E731
-    upv = lambda v: math.nextafter(v, math.inf)  # noqa: E731
-    return max(dn(dn(R2 * dn(1.0 - 2.0 * g))), 0.0), upv(upv(R2 * upv(1.0 + 2.0 * g)))
-
-
 # ------------------------------------------------------- reference CPU path port
 
 
-class GramPort:
-    """Numpy restatement of ``CPUIndex._gram`` (neighbors_cpu.py:314-364).
+def _stable_minima(work, take):
+    """The `take` smallest entries per row, lower column first on equal values,
+    by repeated row argmin (restates ``_extract``, neighbors_cpu.py:49-59)."""
+    rows = np.arange(len(work))
+    ids = np.empty((len(work), take), np.intp)
+    values = np.empty((len(work), take), work.dtype)
+    for slot in range(take):
+        best = np.argmin(work, axis=1)
+        ids[:, slot] = best
+        values[:, slot] = work[rows, best]
+        work[rows, best] = np.inf
+    return values, ids
 
-    Float64 BLAS Gram proposal with the omitted-point lower bound
-    ``(1 - 64(D+1)eps)(|q|^2 + |x|^2) - 2 q.x``; take k+2 plus one barrier;
-    float64 direct refinement; interval ``16(D+1)eps``; exact repair of
-    ambiguous rows; uncertain rows fall back to a full direct scan. Timed as the
-    CPU baseline; exact repair here uses the C oracle.
+
+def _partition_minima(work, take):
+    """Partition-based prefix, stable minima for rows whose prefix ends at zero
+    (restates ``_extract_partition``, neighbors_cpu.py:62-77)."""
+    rows = np.arange(len(work))[:, None]
+    ids = np.argpartition(work, take - 1, axis=1)[:, :take]
+    values = work[rows, ids]
+    order = np.lexsort((ids, values), axis=1)
+    values, ids = values[rows, order], ids[rows, order]
+    zero = np.flatnonzero(values[:, -1] == 0)
+    if len(zero):
+        values[zero], ids[zero] = _stable_minima(work[zero].copy(), take)
+    work[rows, ids] = np.inf
+    return values, ids
+
+
+class GramPort:
+    """Numpy restatement of the reference CPU path ``CPUIndex`` (neighbors_cpu.py).
+
+    ``query`` follows ``CPUIndex.query`` (:402-448) for the methods the benchmark
+    shapes reach, in query blocks of 256 rows:
+
+    * ``_gram`` (:314-364): float64 BLAS Gram lower bound
+      ``(1 - 64(D+1)eps)(|q|^2 + |x|^2) - 2 q.x``, the k+2 smallest as candidates
+      plus the next as barrier, float64 direct refinement, interval
+      ``16(D+1)eps``; ambiguous certified rows are ordered exactly over their
+      k+2 candidates only (``_exact_order``); uncertain rows (barrier not above
+      the k-th upper value) rerun through ``_direct`` alone.
+    * ``_direct`` (:213-262): float64 squared distances to every row, the k+1
+      smallest, exact repair of overlapping intervals over the retained prefix
+      plus every row whose lower value reaches the k-th upper one.
+    * ``_integer_gram`` (:264-287): float32 Gram arithmetic for small integer
+      data (every term exact), used when the reference would use it.
+    * ``_exact_order`` (:174-181): exact order by Python-integer keys
+      (coordinates in units of 2^-149).
+
+    Not restated: the compiled D <= 32 scan (``_neighbors_cpu_native.query``,
+    reached only for D <= 32 and n <= 16384), the k-d tree proposal of
+    ``_tree_candidates`` (n <= 2048), explicit train tiling (``_tiled``) and the
+    batched int64 dyadic order (``_native_exact_order``, a faster route to the
+    same exact order as ``_exact_order``). Timed as the CPU baseline.
     """
+
+    QUERY_BLOCK = 256
 
     def __init__(self, X):
         self.X = _c32(X)
         self.train64 = self.X.astype(np.float64)
         self.norm64 = np.einsum("ij,ij->i", self.train64, self.train64)
+        self._norm32 = None
+        self._integral = None
 
-    def query(self, Q, k, block=64):
+    # -------------------------------------------------------------- routing
+    def _integer_query(self, Q):
+        """Small integer data whose float32 Gram arithmetic is exact (:148-172)."""
+        n, features = self.X.shape
+        if n > 2048 or features > 64:
+            return False
+        if self._integral is None:
+            top = float(np.max(np.abs(self.X)))
+            self._top = top
+            self._integral = bool(4 * features * top * top <= 2 ** 24
+                                  and np.equal(self.X.view(np.uint32), np.trunc(self.X).view(np.uint32)).all())
+        if not self._integral:
+            return False
+        top = max(self._top, float(np.max(np.abs(Q))))
+        return bool(4 * features * top * top <= 2 ** 24
+                    and np.equal(Q.view(np.uint32), np.trunc(Q).view(np.uint32)).all())
+
+    def query(self, Q, k, block=None, *, exclude_self=False):
+        """(float64 distances, int64 indices) ordered by (S, index)."""
         Q = _c32(Q)
+        block = block or self.QUERY_BLOCK
+        exact = self._integer_query(Q) if len(Q) else False
         out_d = np.empty((len(Q), k))
         out_i = np.empty((len(Q), k), np.int64)
         for s in range(0, len(Q), block):
-            d, i = self._gram(Q[s:s + block], k)
+            part = Q[s:s + block]
+            if exact and len(Q) >= 16 and self.X.shape[1] >= 32 and len(self.X) >= 512:
+                d, i = self._integer_gram(part, k, exclude_self, s)
+            elif len(Q) >= 16 and len(self.X) >= 256:
+                d, i = self._gram(part, k, exclude_self, s)
+            else:
+                d, i = self._direct(part, k, exclude_self, s, exact=exact)
             out_d[s:s + block] = np.sqrt(d)
             out_i[s:s + block] = i
         return out_d, out_i
 
-    def _gram(self, Q, k):
+    # -------------------------------------------------------------- pieces
+    def _exact_order(self, query, candidates, k):
+        keys = sorted((integer_key(query, self.X[c]), int(c)) for c in candidates)
+        return np.asarray([c for _, c in keys[:k]], np.intp)
+
+    def _direct(self, Q, k, exclude_self=False, offset=0, exact=False):
+        from scipy.spatial.distance import cdist
+        train, query = self.train64, Q.astype(np.float64)
+        work = cdist(query, train, "sqeuclidean")
+        if exclude_self:
+            work[np.arange(len(Q)), np.arange(offset, offset + len(Q))] = np.inf
+        take = min(k + 1, len(train) - int(exclude_self))
+        values, ids = (_partition_minima(work, take) if not exact and len(train) <= 256
+                       else _stable_minima(work, take))
+        if not exact:
+            error = 16 * (self.X.shape[1] + 1) * np.finfo(np.float64).eps
+            lower, upper = values * (1 - error), values * (1 + error)
+            ambiguous = np.any(lower[:, 1:] <= upper[:, :-1], axis=1) & (values[:, k - 1] != 0)
+            for row in np.flatnonzero(ambiguous):
+                if take > k and lower[row, k] <= upper[row, k - 1]:
+                    cutoff = np.nextafter(upper[row, k - 1] / (1 - error), np.inf)
+                    cands = np.concatenate((ids[row], np.flatnonzero(work[row] <= cutoff)))
+                else:
+                    cands = ids[row, :k]
+                chosen = self._exact_order(Q[row], cands, k)
+                ids[row, :k] = chosen
+                values[row, :k] = np.sum((train[chosen] - query[row]) ** 2, axis=1)
+        return values[:, :k], ids[:, :k]
+
+    def _integer_gram(self, Q, k, exclude_self=False, offset=0):
+        if self._norm32 is None:
+            self._norm32 = np.einsum("ij,ij->i", self.X, self.X)
+        work = Q @ self.X.T
+        work *= -2
+        work += self._norm32[None, :]
+        work += np.einsum("ij,ij->i", Q, Q)[:, None]
+        if exclude_self:
+            work[np.arange(len(Q)), np.arange(offset, offset + len(Q))] = np.inf
+        values, ids = _stable_minima(work, k)
+        return values.astype(np.float64), ids
+
+    def _gram(self, Q, k, exclude_self=False, offset=0):
         train, query = self.train64, Q.astype(np.float64)
         D = self.X.shape[1]
-        qnorm = np.einsum("ij,ij->i", query, query)
         lower = query @ train.T
         lower *= -2
         factor = 64 * (D + 1) * np.finfo(np.float64).eps
         lower += ((1 - factor) * self.norm64)[None, :]
-        lower += ((1 - factor) * qnorm)[:, None]
+        lower += ((1 - factor) * np.einsum("ij,ij->i", query, query))[:, None]
         np.maximum(lower, 0, out=lower)
-        take = min(k + 2, len(train))
+        if exclude_self:
+            lower[np.arange(len(Q)), np.arange(offset, offset + len(Q))] = np.inf
+        take = min(k + 2, len(train) - int(exclude_self))
         if take < len(train):
             part = np.argpartition(lower, take, axis=1)[:, :take + 1]
             cand = part[:, :take]
@@ -309,13 +395,16 @@ class GramPort:
         error = 16 * (D + 1) * np.finfo(np.float64).eps
         upper = values[:, k - 1] * (1 + error)
         uncertain = ~(barrier > upper)
-        if take == len(train):
+        if take == len(train) - int(exclude_self):
             uncertain[:] = False
         ambiguous = np.any(values[:, 1:] * (1 - error) <= values[:, :-1] * (1 + error), axis=1)
-        fix = np.flatnonzero(ambiguous | uncertain)
-        if len(fix):
-            d, i = knn(self.X, Q[fix], k)
-            values[fix, :k], cand[fix, :k] = d ** 2, i
+        for row in np.flatnonzero(ambiguous & ~uncertain):
+            chosen = self._exact_order(Q[row], cand[row], k)
+            cand[row, :k] = chosen
+            values[row, :k] = np.sum((train[chosen] - query[row]) ** 2, axis=1)
+        for row in np.flatnonzero(uncertain):
+            sq, ids = self._direct(Q[row:row + 1], k, exclude_self, offset + row)
+            cand[row, :k], values[row, :k] = ids[0], sq[0]
         return values[:, :k], cand[:, :k]
 
 

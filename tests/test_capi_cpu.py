@@ -349,3 +349,20 @@ def test_bad_environment_tuning_fails_every_load(monkeypatch):
         monkeypatch.delenv("MLXLEARN_SCREEN_STAGES")
         _config.reset_config()
         _lib._LIB = saved
+
+
+def test_reference_binding_loads_without_device():
+    """The committed reference-side binding imports and binds every entry point it calls."""
+    import sys
+    from mlxlearn_amd import _lib
+    os.environ["MLXLEARN_AMD_LIB"] = _lib.LIB_PATH
+    sys.path.insert(0, os.path.join(ROOT, "mlx-learn_amd", "integration"))
+    import _mi355x
+    lib = _mi355x.library()
+    for name in ("mlk_index_create", "mlk_knn_query", "mlk_radius_query", "mlk_radius_fill", "mlk_kmeans_assign"):
+        assert hasattr(lib, name)
+    assert _mi355x.SQUARED_DIST == _lib.SQUARED_DIST and _mi355x.PRECISION == _lib.PRECISIONS
+    with pytest.raises(ValueError):
+        _mi355x._check(1)
+    with pytest.raises(MemoryError):
+        _mi355x._check(-1)

@@ -116,8 +116,63 @@ def test_radius_band_and_exact_membership():
     assert oracle.exact_le(q, x, 9.0)
     assert not oracle.exact_le(q, x, math.nextafter(9.0, 0.0))
     lo, hi = oracle.radius_band(4, 2.0)
-    assert (lo, hi) == oracle.radius_thresholds(4, 2.0)
     assert lo < 2.0 < hi
+
+
+TINY = 2.0 ** -1074
+BAND_R2 = [TINY, 3 * TINY, 2.0 ** -1022, 1e-300, 1e-30, 0.5, 1.0, 2.0, 3.3, 1e30, 1e300, 2.0 ** 1023,
+           1.7976931348623157e308]
+
+
+@pytest.mark.parametrize("d", [1, 2, 4, 32, 128, 784, 4096])
+def test_radius_band_against_exact_rationals(d):
+    """The band of the membership test, pinned by exact rational arithmetic.
+
+    The canonical float64 sum s of d squared differences satisfies
+    |s - S| <= gamma * S with gamma = (d+2)u / (1 - (d+2)u), u = 2^-53
+    (radius_contract.py:129-153), so s <= lo certifies S <= R2 only if
+    lo <= R2 (1 - 2 gamma), and s > hi certifies S > R2 only if
+    hi >= R2 (1 + 2 gamma). Both are checked as exact fractions (soundness),
+    and each end must lie within four units in the last place of that exact
+    value (tightness), so a dropped factor, d in place of d+2 or rounding
+    towards the wrong side fails.
+    """
+    from fractions import Fraction as F
+    u = F(1, 2 ** 53)
+    g = (d + 2) * u / (1 - (d + 2) * u)
+    for R2 in BAND_R2:
+        lo, hi = oracle.radius_band(d, R2)
+        want_lo = F(R2) * (1 - 2 * g)
+        want_hi = F(R2) * (1 + 2 * g)
+        assert lo >= 0.0 and F(lo) <= want_lo, (d, R2, lo)
+        # tightness: four ulps of the exact value, or four subnormal steps near zero
+        slack = max(want_lo * 4 * u * 2, F(4) * F(TINY))
+        assert want_lo - F(lo) <= slack, (d, R2, lo)
+        if math.isinf(hi):
+            assert want_hi > F(1.7976931348623157e308), (d, R2)
+        else:
+            assert F(hi) >= want_hi, (d, R2, hi)
+            assert F(hi) - want_hi <= max(want_hi * 4 * u * 2, F(4) * F(TINY)), (d, R2, hi)
+    assert oracle.radius_band(d, 0.0) == (0.0, 0.0)
+    assert oracle.radius_band(d, math.inf) == (math.inf, math.inf)
+
+
+def test_radius_band_planted_mistakes_fail():
+    """The rational check above rejects plausible wrong bands (planted on the CPU)."""
+    from fractions import Fraction as F
+    u = F(1, 2 ** 53)
+
+    def sound(d, R2, lo, hi):
+        g = (d + 2) * u / (1 - (d + 2) * u)
+        return F(lo) <= F(R2) * (1 - 2 * g) and F(hi) >= F(R2) * (1 + 2 * g)
+
+    d, R2 = 3, 2.0
+    g_wrong = d * 2.0 ** -53 / (1 - d * 2.0 ** -53)        # d instead of d + 2
+    assert not sound(d, R2, R2 * (1 - 2 * g_wrong), R2 * (1 + 2 * g_wrong))
+    g = (d + 2) * 2.0 ** -53 / (1 - (d + 2) * 2.0 ** -53)
+    assert not sound(d, R2, R2 * (1 - g), R2 * (1 + g))     # dropped factor 2
+    lo, hi = oracle.radius_band(d, R2)
+    assert sound(d, R2, lo, hi)
 
 
 def test_kmeans_assign_lowest_index_on_ties():
@@ -220,6 +275,48 @@ def test_kmeans_assign_is_exact_nearest_centre():
     np.testing.assert_array_equal(labels, oracle.integer_knn(C, X, 1)[:, 0])
     for i in range(60):
         assert resid[i] == oracle.canonical_sq(X[i], C[labels[i]])
+
+
+def test_cpu_baseline_port_routes_are_exact(rng, monkeypatch):
+    """Every route of the CPU-path restatement (the timed CPU baseline) returns the
+    oracle's neighbours: the Gram proposal with exact repair over the k+2
+    candidates, the direct scan of uncertain rows, the direct scan with its own
+    exact repair, and the integer Gram of small integer data."""
+    calls = {"exact": 0, "direct": 0, "integer": 0}
+    for name, key in (("_exact_order", "exact"), ("_direct", "direct"), ("_integer_gram", "integer")):
+        orig = getattr(oracle.GramPort, name)
+
+        def wrapped(self, *a, _orig=orig, _key=key, **kw):
+            calls[_key] += 1
+            return _orig(self, *a, **kw)
+        monkeypatch.setattr(oracle.GramPort, name, wrapped)
+    X = rng.normal(size=(2000, 24)).astype(np.float32)
+    X[100:112] = X[5]                                    # 12 exact copies: uncertain barrier for k = 7
+    base = rng.normal(size=24).astype(np.float32)
+    for j in range(8):                                   # near-ties: one float32 step apart
+        X[300 + j] = base
+        X[300 + j, j] = np.nextafter(base[j], np.float32(np.inf))
+    Q = np.concatenate([X[[5, 300, 301]], base[None], rng.normal(size=(40, 24)).astype(np.float32)])
+    port = oracle.GramPort(X)
+    d, i = port.query(Q, 7)
+    wd, wi = oracle.knn(X, Q, 7)
+    np.testing.assert_array_equal(i, wi)
+    np.testing.assert_allclose(d, wd, rtol=1e-12)
+    assert calls["direct"] >= 1 and calls["exact"] >= 1
+    # short query block: the direct route with its own repair
+    d, i = port.query(Q[:4], 7)
+    np.testing.assert_array_equal(i, wi[:4])
+    # self query without the own row
+    d, i = port.query(X[:50], 5, exclude_self=True)
+    np.testing.assert_array_equal(i, oracle.knn(X, X[:50], 5, exclude_self=True)[1])
+    # small integer data: the float32 integer Gram
+    Xi = rng.integers(0, 8, size=(1000, 40)).astype(np.float32)
+    Qi = rng.integers(0, 8, size=(64, 40)).astype(np.float32)
+    d, i = oracle.GramPort(Xi).query(Qi, 9)
+    wd, wi = oracle.knn(Xi, Qi, 9)
+    np.testing.assert_array_equal(i, wi)
+    np.testing.assert_array_equal(d, wd)
+    assert calls["integer"] >= 1
 
 
 def test_cpu_baseline_port_is_exact(rng):
