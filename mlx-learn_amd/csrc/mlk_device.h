@@ -22,6 +22,22 @@ __device__ __forceinline__ int mfma32_row(int i, int lane) {
 }
 
 // ---------------------------------------------------------------------------
+// float64 product and sum, each rounded on its own. HIP compiles with
+// contraction on, and __dmul_rn / __dadd_rn are plain operators there, so
+// __dadd_rn(s, __dmul_rn(t, t)) still becomes one fused multiply-add; these
+// helpers carry the no-contraction pragma, so the compiler keeps the two
+// roundings.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double mul_rn(double a, double b) {
+#pragma clang fp contract(off)
+    return a * b;
+}
+__device__ __forceinline__ double add_rn(double a, double b) {
+#pragma clang fp contract(off)
+    return a + b;
+}
+
+// ---------------------------------------------------------------------------
 // canonical float64 squared distance: s = sum_k (double(q_k) - double(x_k))^2,
 // strictly left to right, no contraction (matches the reference's canonical
 // sum, _radius_common.h:349-357). |s - S| <= (d+2) 2^-53 S.
@@ -29,8 +45,8 @@ __device__ __forceinline__ int mfma32_row(int i, int lane) {
 __device__ __forceinline__ double canonical_sq(const float* __restrict__ q, const float* __restrict__ x, int d) {
     double s = 0.0;
     for (int i = 0; i < d; ++i) {
-        double t = __dsub_rn((double)q[i], (double)x[i]);
-        s = __dadd_rn(s, __dmul_rn(t, t));
+        const double t = (double)q[i] - (double)x[i];
+        s = add_rn(s, mul_rn(t, t));
     }
     return s;
 }
@@ -43,14 +59,14 @@ __device__ __forceinline__ double canonical_sq4(const float* __restrict__ q, con
     for (int i = 0; i < d; i += 4) {
         const float4 a = *reinterpret_cast<const float4*>(q + i);
         const float4 b = *reinterpret_cast<const float4*>(x + i);
-        double t = __dsub_rn((double)a.x, (double)b.x);
-        s = __dadd_rn(s, __dmul_rn(t, t));
-        t = __dsub_rn((double)a.y, (double)b.y);
-        s = __dadd_rn(s, __dmul_rn(t, t));
-        t = __dsub_rn((double)a.z, (double)b.z);
-        s = __dadd_rn(s, __dmul_rn(t, t));
-        t = __dsub_rn((double)a.w, (double)b.w);
-        s = __dadd_rn(s, __dmul_rn(t, t));
+        double t = (double)a.x - (double)b.x;
+        s = add_rn(s, mul_rn(t, t));
+        t = (double)a.y - (double)b.y;
+        s = add_rn(s, mul_rn(t, t));
+        t = (double)a.z - (double)b.z;
+        s = add_rn(s, mul_rn(t, t));
+        t = (double)a.w - (double)b.w;
+        s = add_rn(s, mul_rn(t, t));
     }
     return s;
 }

@@ -784,8 +784,8 @@ __device__ void rank_query(const RankArgs& a, int64_t item, int64_t q, int* __re
             const float* xr = stage + lane * rs;
             double sc = 0.0;
             for (int e = 0; e < d; ++e) {
-                const double t = __dsub_rn((double)sq_[e], (double)xr[e]);
-                sc = __dadd_rn(sc, __dmul_rn(t, t));
+                const double t = (double)sq_[e] - (double)xr[e];
+                sc = add_rn(sc, mul_rn(t, t));
             }
             a.out_dist[q * k + r0 + lane] = a.squared ? sc : sqrt(sc);
         }

@@ -223,10 +223,10 @@ __global__ void k_radius_mean(const int64_t* __restrict__ indptr, int64_t m, con
             // product rounded before the sum (no contraction), as the reference
             // forms values * weights before adding
             const double w = entry_weight(dist, j, coincident);
-            num = __dadd_rn(num, __dmul_rn(targets[idx[j]], w));
-            den = __dadd_rn(den, w);
+            num = add_rn(num, mul_rn(targets[idx[j]], w));
+            den = add_rn(den, w);
         } else {
-            num = __dadd_rn(num, targets[idx[j]]);
+            num = add_rn(num, targets[idx[j]]);
         }
     }
     if (!weighted) den = (double)(e - b);

@@ -203,7 +203,7 @@ class KMeans(ClusterMixin, TransformerMixin, BaseEstimator):
                 new = (sums / counts.clamp(min=1).to(torch.float64)[:, None])
                 if empty:
                     largest = int(torch.argmax(counts).item())
-                    new[counts == 0] = new[largest]
+                    new[counts == 0] = new[largest].clone()  # a row of `new` itself: copy before the write
                 new32 = new.to(torch.float32)
                 shift = float(((new32.to(torch.float64) - centres.to(torch.float64)) ** 2).sum().item())
                 unchanged = previous is not None and bool(torch.equal(labels, previous))

@@ -243,3 +243,43 @@ def test_misaligned_rows_assignment(mlk, rng):
         labels, resid = assign_device(Xm, C, precision=prec)
         np.testing.assert_array_equal(labels.cpu().numpy(), want_l)
         np.testing.assert_allclose(resid.cpu().numpy(), want_r, rtol=1e-12, atol=0)
+
+
+def _relocated_centres(X, labels, residual, k):
+    """NumPy restatement of the reference's empty-cluster relocation
+    (_k_means.py:116-142): the farthest rows (largest residual, lower row on a
+    tie) leave their clusters for the empty ones in ascending id order; a
+    cluster left empty takes the largest cluster's mean."""
+    sums, counts = oracle.kmeans_update(X, labels, k)
+    empty = np.flatnonzero(counts == 0)
+    far = np.argsort(-residual, kind="stable")[:empty.size]
+    if empty.size and residual.max() > 0:
+        for e, r in zip(empty, far):
+            sums[labels[r]] -= X[r]
+            counts[labels[r]] -= 1
+            sums[e] = X[r]
+            counts[e] = 1
+    means = sums / np.maximum(counts, 1)[:, None]
+    means[counts == 0] = means[np.argmax(counts)]
+    return means.astype(np.float32), empty, far
+
+
+def test_kmeans_relocates_empty_clusters(mlk):
+    """Initial centres far from every row leave clusters empty; one Lloyd step
+    moves the farthest rows into them (and refills a donor that empties)."""
+    r = np.random.default_rng(123)
+    X = (r.normal(size=(3000, 5)) + 10 * r.integers(0, 4, (3000, 1))).astype(np.float32)
+    X[17] = [400.0, 0, 0, 0, 0]                         # an outlier alone in its cluster
+    C0 = np.concatenate([X[[0, 1, 2, 3, 4, 5]], [[200.0, 0, 0, 0, 0]],  # centre 6 holds only the outlier
+                         np.full((3, 5), 1e4, np.float32)]).astype(np.float32)
+    labels, resid = oracle.kmeans_assign(X, C0)
+    assert np.bincount(labels, minlength=10)[6] == 1 and np.bincount(labels, minlength=10)[7:].sum() == 0
+    want, empty, far = _relocated_centres(X, labels, resid, 10)
+    assert list(empty) == [7, 8, 9] and far[0] == 17    # the outlier moves first and empties centre 6
+    km = mlk.KMeans(n_clusters=10, init=C0, max_iter=1, tol=0).fit(X)
+    got = km.cluster_centers_
+    for e, row in zip(empty, far):
+        np.testing.assert_array_equal(got[e], X[row])   # a relocated cluster is its one row, exactly
+    np.testing.assert_allclose(got, want, rtol=1e-6, atol=1e-6)
+    l1, r1 = oracle.kmeans_assign(X, got)
+    np.testing.assert_array_equal(km.labels_, l1)

@@ -380,3 +380,30 @@ def test_rank_with_more_survivors_than_one_wave_holds(mlk, rng, near):
         assert st["fallback_queries"] == (0 if near <= 3900 else 3), st
     finally:
         _lib.set_tuning()
+
+
+def _canonical(Q, X, idx):
+    """sum_j (double(q_j) - double(x_j))^2 left to right, each product and sum
+    rounded on its own (NumPy's float64 array operations round separately)."""
+    s = np.zeros(idx.shape)
+    for j in range(Q.shape[1]):
+        t = Q[:, None, j].astype(np.float64) - X[idx, j].astype(np.float64)
+        s = s + t * t
+    return s
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize("d,k", [(37, 9), (96, 1), (128, 10), (784, 3)])
+def test_squared_distances_are_canonical_sums(mlk, precision, d, k):
+    """Reported squared distances are the canonical float64 sums bit for bit
+    (no fused multiply-add): the rank kernels, the k = 1 group rank and the
+    fallback all write the same value."""
+    r = np.random.default_rng(d)
+    X = r.normal(size=(4000, d)).astype(np.float32)
+    Q = r.normal(size=(300, d)).astype(np.float32)
+    index = mlk.DeviceNeighborIndex(X)
+    dist, ind = index.query(Q, k, precision=precision, squared=True)
+    ind = ind.cpu().numpy()
+    np.testing.assert_array_equal(ind, oracle.knn(X, Q, k)[1])
+    np.testing.assert_array_equal(dist.cpu().numpy(), _canonical(Q, X, ind))
+    index.close()

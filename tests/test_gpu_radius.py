@@ -100,6 +100,22 @@ def test_candidate_buffer_regrows(mlk, rng):
         assert_csr(gpu_radius(mlk, X, Q, R2, p), want)
 
 
+def test_sort_count_limit_is_refused(mlk, rng):
+    """Counts past the one-pass sort limit (INT32_MAX, lowered here by a test hook)
+    are refused with a capacity error instead of sorting a wrapped count; the
+    next call without the hook is exact."""
+    from mlxlearn_amd import _lib
+    X = rng.normal(size=(2000, 6)).astype(np.float32)
+    Q = rng.normal(size=(40, 6)).astype(np.float32)
+    want = oracle.radius(X, Q, 4.0, sort_results=True)
+    assert want[0][-1] > 100
+    _lib.set_test_hooks(_lib.HOOK_SORT_LIMIT, 100)
+    with pytest.raises(ValueError, match="sort limit"):
+        gpu_radius(mlk, X, Q, 4.0, sort_results=True)
+    _lib.set_test_hooks(0, 0)
+    assert_csr(gpu_radius(mlk, X, Q, 4.0, sort_results=True), want)
+
+
 def test_clumped_workload_batches(mlk):
     """Workload c4 generator at reduced size: uniform cube plus 20% tight clumps, calibrated radius."""
     X, Q, sizes, radius = oracle.gen_c4(seed=7, n=200_000, n_batches=6, max_batch=256)
