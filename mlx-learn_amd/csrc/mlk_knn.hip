@@ -1357,7 +1357,7 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
         sa.self_off = q_offset;
         sa.exclude = exclude;
         sa.xcd_map = (S % 8) == 0;
-        sa.nchunk = screen_chunks(dp);
+        sa.nchunk = screen_chunks(prec, dp);
         sa.qgroup = sa.nchunk > 1 ? DEEP_QGROUP : 1;
         const unsigned blocks = (unsigned)(QB * S);
         auto merge = [&]() -> int {

@@ -495,7 +495,7 @@ int radius_query_impl(mlk_index* index, const float* Q, int64_t m, double R2, in
             sa.self_off = q_offset;
             sa.exclude = exclude;
             sa.xcd_map = (S % 8) == 0;
-            sa.nchunk = screen_chunks(dp);
+            sa.nchunk = screen_chunks(prec, dp);
             sa.qgroup = sa.nchunk > 1 ? DEEP_QGROUP : 1;
             sa.pairs = ws[9].as<int2>();
             sa.pair_count = dcount;

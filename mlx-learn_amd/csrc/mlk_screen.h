@@ -97,15 +97,16 @@ struct ScreenArgs {
     unsigned long long* pair_count;
     unsigned long long pair_cap;
     // rows wider than the widest register-held query (deep kernels): the row
-    // is DEEP_CHUNK features per chunk, nchunk chunks; 1 for every other kernel
+    // is nchunk chunks of the kernel's chunk width; 1 for every other kernel
     int nchunk;
     // with xcd_map: query blocks per group of an XCD's consecutive blocks
     // (0 or 1: every query block of a split before the next split)
     int qgroup;
 };
 
-constexpr int WIDE_MAX = 800;    // widest row whose query fragments stay in registers
-constexpr int DEEP_CHUNK = 256;  // features per chunk of the deep kernels
+constexpr int WIDE_MAX = 800;    // widest row whose query fragments stay in registers (bf16 / fp16)
+constexpr int DEEP_CHUNK = 256;  // features per chunk of the bf16 / fp16 deep kernels
+constexpr int FP32_CHUNK = 128;  // features per chunk of the float32 deep kernel (512 bytes, as DEEP_CHUNK halves)
 constexpr int DEEP_QGROUP = 8;   // query blocks per XCD block group of the deep kernels
 
 constexpr int SCREEN_LISTS = 0;  // k-NN: per-(query, split) candidate lists
@@ -123,23 +124,27 @@ struct ScreenShape {
 // Measured default shape for a precision and padded width.
 ScreenShape default_shape(int prec, int dp);
 
-// Padded width of the screen mirror: a multiple of 32 up to 128, then the
-// wide-kernel widths 256, 384, 512, 640, 800, then multiples of DEEP_CHUNK
-// (bf16 / fp16 only above 128).
+// Padded width of the screen mirror: a multiple of 32 up to 128; above that,
+// bf16 / fp16 use the wide-kernel widths 256, 384, 512, 640, 800 and then
+// multiples of DEEP_CHUNK, float32 multiples of FP32_CHUNK.
 int screen_dp(int prec, int d);
 
-// Chunks per row of a padded width: dp / DEEP_CHUNK for the deep kernels, else 1.
-inline int screen_chunks(int dp) { return dp > WIDE_MAX ? dp / DEEP_CHUNK : 1; }
+// Features per chunk of a padded width (the deep kernels walk a row in chunks
+// whose query fragments fit in registers): dp itself for the narrow and wide
+// kernels.
+inline int screen_chunk(int prec, int dp) {
+    if (prec == PREC_FP32) return dp > 128 ? FP32_CHUNK : dp;
+    return dp > WIDE_MAX ? DEEP_CHUNK : dp;
+}
+inline int screen_chunks(int prec, int dp) { return dp / screen_chunk(prec, dp); }
 
-// Screen precision actually used: float32 has no wide kernel, so rows wider
-// than 128 features are screened in bf16 (results do not depend on it).
-inline int effective_prec(int prec, int d) { return (prec == PREC_FP32 && d > 128) ? PREC_BF16 : prec; }
-
-// Radius search screens float32 requests in bf16 at every width: its band
-// around r^2 stays narrow next to the result set (2M x 32 uniform cube with
-// clumps: 1.2 % more band pairs than the float32 screen, 86 ms per call
-// against 190 ms), and the results do not depend on the screen precision.
-inline int radius_prec(int prec, int d) { return prec == PREC_FP32 ? PREC_BF16 : effective_prec(prec, d); }
+// The screen precision a request runs: every request runs the precision it
+// names, at every width, for k-NN and radius search alike.
+inline int effective_prec(int prec, int d) {
+    (void)d;
+    return prec;
+}
+inline int radius_prec(int prec, int d) { return effective_prec(prec, d); }
 
 // Launch function of the screen for a padded width dp in {32, 64, 96, 128};
 // nullptr when the shape is not built for that mode and precision. Blocks

@@ -508,7 +508,10 @@ static ScreenFn pick_wide(int dp) {
 template <int PREC, int MODE>
 static ScreenFn pick_shape(int dp, int qt, int ns) {
     if (dp > 128) {
-        if constexpr (PREC == PREC_FP32) return nullptr;
+        // float32 rows wider than 128 features: 128-feature chunks (512 bytes a
+        // row, the bf16 deep chunk's size), 64-row tiles, two stages
+        if constexpr (PREC == PREC_FP32)
+            return dp % FP32_CHUNK == 0 ? k_screen<PREC, FP32_CHUNK, 1, 2, 2, MODE, true> : nullptr;
         else return pick_wide<PREC, MODE>(dp);
     }
     if constexpr (PREC == PREC_FP32) {
@@ -530,7 +533,8 @@ static ScreenFn pick_shape(int dp, int qt, int ns) {
 
 int screen_dp(int prec, int d) {
     const int p32 = (d + 31) / 32 * 32;
-    if (p32 <= 128 || prec == PREC_FP32) return p32;
+    if (p32 <= 128) return p32;
+    if (prec == PREC_FP32) return (d + FP32_CHUNK - 1) / FP32_CHUNK * FP32_CHUNK;  // float32 deep kernel
     for (int w : {256, 384, 512, 640, WIDE_MAX})
         if (d <= w) return w;
     return (d + DEEP_CHUNK - 1) / DEEP_CHUNK * DEEP_CHUNK;  // deep kernels
@@ -539,8 +543,7 @@ int screen_dp(int prec, int d) {
 // Measured on the 1M x 128 mixture workload: with the candidate appends of a
 // real query, one query tile per wave (occupancy 4) beats two (occupancy 3).
 ScreenShape default_shape(int prec, int dp) {
-    (void)prec;
-    return (dp > 128 && dp <= WIDE_MAX) ? ScreenShape{1, 3} : ScreenShape{1, 2};
+    return (prec != PREC_FP32 && dp > 128 && dp <= WIDE_MAX) ? ScreenShape{1, 3} : ScreenShape{1, 2};
 }
 
 ScreenFn pick_screen(int mode, int prec, int dp, ScreenShape shape) {

@@ -272,19 +272,22 @@ def test_estimator_kneighbors(mlk, rng):
 @pytest.mark.parametrize("precision", PRECISIONS)
 @pytest.mark.parametrize("d", [129, 200, 384, 500, 640, 784, 800])
 def test_wide_screen_matches_oracle(mlk, precision, d):
-    """Rows wider than 128 features: the wide MFMA screen (float32 requests are screened in bf16)."""
+    """Rows wider than 128 features: the wide bf16 / fp16 MFMA screen, and the float32
+    screen in 128-feature chunks; every request runs the precision it names."""
+    from mlxlearn_amd import _lib
     r = np.random.default_rng(7000 + d)
     X = r.normal(size=(3000, d)).astype(np.float32)
     X[50:55] = X[49]
     Q = np.concatenate([X[:3], r.normal(size=(157, d)).astype(np.float32)])
     assert_parity(gpu_knn(mlk, X, Q, 7, precision, screened=True), oracle.knn(X, Q, 7))
+    assert _lib.stats()["screen_precision"] == _lib.PRECISIONS[precision]
 
 
 @pytest.mark.parametrize("precision", PRECISIONS)
 @pytest.mark.parametrize("d", [801, 1024, 1100, 1536])
 def test_deep_screen_matches_oracle(mlk, precision, d):
     """Rows wider than 800 features: the deep MFMA screen walks each row tile in 256-feature chunks
-    (float32 requests are screened in bf16). Ragged database (not a multiple of the 64-row tile),
+    (bf16 / fp16) or 128-feature chunks (float32). Ragged database (not a multiple of the 64-row tile),
     duplicated rows, queries equal to rows, and no query may need the exact fallback."""
     from mlxlearn_amd import _lib
     r = np.random.default_rng(9000 + d)
@@ -292,7 +295,7 @@ def test_deep_screen_matches_oracle(mlk, precision, d):
     X[50:55] = X[49]
     Q = np.concatenate([X[:3], X[50:51], r.normal(size=(150, d)).astype(np.float32)])
     assert_parity(gpu_knn(mlk, X, Q, 7, precision, screened=True), oracle.knn(X, Q, 7))
-    assert _lib.stats()["screen_precision"] == (0x20 if precision == "float16" else 0x10)
+    assert _lib.stats()["screen_precision"] == _lib.PRECISIONS[precision]
 
 
 @pytest.mark.parametrize("precision", PRECISIONS)
@@ -317,10 +320,13 @@ def test_deep_pixels_exact_screen(mlk):
 
 
 def test_wide_pixels_exact_screen(mlk):
-    """Workload c3 generator (784 integer pixels) through the wide fp16 screen, no fallback."""
+    """Workload c3 generator (784 integer pixels) through the wide fp16 / bf16 screens and the
+    chunked float32 screen (exact for integer pixels), each request in the precision it names."""
+    from mlxlearn_amd import _lib
     X, y, Q = oracle.gen_c3(seed=13, n=8000, m=400)
     for p in PRECISIONS:
         assert_parity(gpu_knn(mlk, X, Q, 3, p), oracle.knn(X, Q, 3))
+        assert _lib.stats()["screen_precision"] == _lib.PRECISIONS[p]
 
 
 @pytest.mark.parametrize("d", [4, 64, 100, 200, 256])

@@ -59,6 +59,10 @@ def test_adversarial_cases(mlk, case, precision):
     R2 = float(case.radius) * float(case.radius)
     want = oracle.radius(X, Q, R2)
     assert_csr(gpu_radius(mlk, X, Q, R2, precision), want, case.name)
+    from mlxlearn_amd import _lib
+    st = _lib.stats()
+    if st["rounds"] or st["candidates"]:  # a screened call (not the everything-in-range shortcut)
+        assert st["screen_precision"] == _lib.PRECISIONS[precision]
     want_sorted = oracle.radius(X, Q, R2, sort_results=True)
     assert_csr(gpu_radius(mlk, X, Q, R2, precision, sort_results=True), want_sorted, case.name + " sorted")
 
@@ -159,8 +163,10 @@ def test_wide_rows(mlk, d):
     X = r.normal(size=(2500, d)).astype(np.float32)
     Q = np.concatenate([X[:5], r.normal(size=(60, d)).astype(np.float32)])
     R2 = float(radius_cases.typical_radius(X, Q, 12)) ** 2
+    from mlxlearn_amd import _lib
     for p in PRECISIONS:
         assert_csr(gpu_radius(mlk, X, Q, R2, p, sort_results=True), oracle.radius(X, Q, R2, sort_results=True))
+        assert _lib.stats()["screen_precision"] == _lib.PRECISIONS[p]   # each request runs its own precision
 
 
 def test_misaligned_queries_take_the_scalar_path(mlk, rng):
