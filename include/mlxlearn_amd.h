@@ -151,6 +151,23 @@ int mlk_knn_query(mlk_index* index, const float* Q_dev, int64_t m, int32_t k, in
                   uint32_t flags, double* dist_dev, int64_t* idx_dev, void* stream);
 
 /*
+ * mlk_knn_query without any device allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate (serving loops with a
+ * fixed memory budget, harnesses that forbid hipMalloc). Temporaries live in
+ * a fixed 128 MiB file-scope device array of the library (held on each GPU
+ * the library runs on until the process ends) instead of the index's
+ * workspace; queries are processed in chunks that fit it. Same arguments,
+ * results and errors as mlk_knn_query, and: the index must already hold the
+ * screen mirror of the requested precision (one mlk_knn_query call of that
+ * precision prepares it) -> else MLK_EINVAL; an index of more than about 3.3
+ * million rows (its exact fallback would not fit the array) -> MLK_ECAPACITY.
+ * Calls that overlap on one GPU race on the array; no call records events, so
+ * mlk_get_stats reports no device times for it.
+ */
+int mlk_knn_query_noalloc(mlk_index* index, const float* Q_dev, int64_t m, int32_t k, int64_t q_offset,
+                          uint32_t flags, double* dist_dev, int64_t* idx_dev, void* stream);
+
+/*
  * Several query batches (concatenated rows; batch b is rows
  * [offsets[b], offsets[b+1])) answered by one screen launch per round.
  * Results are identical to one mlk_knn_query per batch.

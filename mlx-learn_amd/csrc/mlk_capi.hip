@@ -60,7 +60,7 @@ mlk_stats& stats() { return g_stats; }
 
 int create_index(const float* X, int64_t n, int d, hipStream_t stream, mlk_index** out);
 
-static int check_fail_hook() {
+int check_fail_hook() {
     std::lock_guard<std::mutex> g(g_config_lock);
     if (g_hooks.hooks & MLK_HOOK_FAIL) {
         g_hooks.hooks &= ~MLK_HOOK_FAIL;
@@ -73,6 +73,22 @@ static int check_fail_hook() {
 static int invalid(const std::string& msg) {
     set_error(msg);
     return MLK_EINVAL;
+}
+
+int validate_knn(mlk_index* index, const float* Q, int64_t m, int32_t k, int64_t q_offset, uint32_t flags,
+                        const int64_t* idx) {
+    if (!index) return invalid("index is NULL");
+    if (m < 0) return invalid("query row count must be non-negative");
+    if (m && (!Q || !idx)) return invalid("queries and index output must be device pointers");
+    if (k < 1) return invalid("Expected n_neighbors > 0. Got " + std::to_string(k));
+    const bool exclude = (flags & MLK_EXCLUDE_SELF) != 0;
+    const int64_t limit = index->n - (exclude ? 1 : 0);
+    if (k > limit)
+        return invalid("n_neighbors must be <= " + std::to_string(limit) + ", got " + std::to_string(k));
+    if ((flags & MLK_PREC_MASK) == MLK_PREC_MASK) return invalid("unknown screen precision");
+    if (exclude && (q_offset < 0 || q_offset + m > index->n))
+        return invalid("exclude_self requires query rows to be training rows");
+    return MLK_OK;
 }
 
 }  // namespace mlk
@@ -159,28 +175,12 @@ int mlk_index_info(const mlk_index* index, int64_t* n, int32_t* d, int32_t* inte
     return MLK_OK;
 }
 
-static int validate_knn(mlk_index* index, const float* Q, int64_t m, int32_t k, int64_t q_offset, uint32_t flags,
-                        const int64_t* idx) {
-    if (!index) return invalid("index is NULL");
-    if (m < 0) return invalid("query row count must be non-negative");
-    if (m && (!Q || !idx)) return invalid("queries and index output must be device pointers");
-    if (k < 1) return invalid("Expected n_neighbors > 0. Got " + std::to_string(k));
-    const bool exclude = (flags & MLK_EXCLUDE_SELF) != 0;
-    const int64_t limit = index->n - (exclude ? 1 : 0);
-    if (k > limit)
-        return invalid("n_neighbors must be <= " + std::to_string(limit) + ", got " + std::to_string(k));
-    if ((flags & MLK_PREC_MASK) == MLK_PREC_MASK) return invalid("unknown screen precision");
-    if (exclude && (q_offset < 0 || q_offset + m > index->n))
-        return invalid("exclude_self requires query rows to be training rows");
-    return MLK_OK;
-}
-
 int mlk_knn_query(mlk_index* index, const float* Q_dev, int64_t m, int32_t k, int64_t q_offset, uint32_t flags,
                   double* dist_dev, int64_t* idx_dev, void* stream) {
     MLK_TRY(check_fail_hook());
     MLK_TRY(validate_knn(index, Q_dev, m, k, q_offset, flags, idx_dev));
     std::lock_guard<std::mutex> g(index->lock);
-    return knn_query_impl(index, Q_dev, m, k, q_offset, flags, dist_dev, idx_dev, (hipStream_t)stream, nullptr, 0);
+    return knn_query_impl(index, Q_dev, m, k, q_offset, flags, dist_dev, idx_dev, (hipStream_t)stream);
 }
 
 int mlk_knn_query_batched(mlk_index* index, const float* Q_dev, const int64_t* offsets, int32_t n_batches, int32_t k,
@@ -199,7 +199,7 @@ int mlk_knn_query_batched(mlk_index* index, const float* Q_dev, const int64_t* o
     std::lock_guard<std::mutex> g(index->lock);
     // Every query is screened independently, so the batches share one
     // screen launch per round over their concatenated rows.
-    return knn_query_impl(index, Q_dev, m, k, 0, flags, dist_dev, idx_dev, (hipStream_t)stream, offsets, n_batches);
+    return knn_query_impl(index, Q_dev, m, k, 0, flags, dist_dev, idx_dev, (hipStream_t)stream);
 }
 
 int mlk_radius_query(mlk_index* index, const float* Q_dev, int64_t m, double R2, int64_t q_offset, uint32_t flags,

@@ -274,18 +274,44 @@ int prepare_mirror(mlk_index* index, int prec, hipStream_t stream) {
     if (mr.n_unsafe) {
         // Deterministic order of the unsafe list (atomics appended in any order).
         std::vector<int> rows(mr.n_unsafe);
-        MLK_CHECK_HIP(hipMemcpy(rows.data(), mr.unsafe.ptr, rows.size() * sizeof(int), hipMemcpyDeviceToHost));
+        MLK_CHECK_HIP(hipMemcpyAsync(rows.data(), mr.unsafe.ptr, rows.size() * sizeof(int), hipMemcpyDeviceToHost,
+                                     stream));
+        MLK_CHECK_HIP(hipStreamSynchronize(stream));
         std::sort(rows.begin(), rows.end());
-        MLK_CHECK_HIP(hipMemcpy(mr.unsafe.ptr, rows.data(), rows.size() * sizeof(int), hipMemcpyHostToDevice));
+        MLK_CHECK_HIP(hipMemcpyAsync(mr.unsafe.ptr, rows.data(), rows.size() * sizeof(int), hipMemcpyHostToDevice,
+                                     stream));
+        MLK_CHECK_HIP(hipStreamSynchronize(stream));  // `rows` leaves scope
     }
     mr.bias_h = -1.0;
     mr.ready = true;
     return MLK_OK;
 }
 
-int prepare_bias(mlk_index* index, int prec, double G, hipStream_t stream) {
+int CallMem::get(int slot, size_t bytes, void** out) {
+    if (!arena) {
+        MLK_TRY(ws[slot].reserve(bytes));
+        *out = ws[slot].ptr;
+        return MLK_OK;
+    }
+    const size_t at = (used + 255) / 256 * 256;
+    if (at > cap || bytes > cap - at) {
+        set_error("the call needs " + std::to_string(at + bytes) + " bytes of its fixed scratch arena of " +
+                  std::to_string(cap) + " bytes");
+        return MLK_ECAPACITY;
+    }
+    *out = arena + at;
+    used = at + bytes;
+    return MLK_OK;
+}
+
+int prepare_bias(mlk_index* index, int prec, double G, hipStream_t stream, bool may_allocate) {
     Mirror& mr = index->mirror[prec];
     if (mr.bias_h == G) return MLK_OK;
+    if (!may_allocate && mr.bias.bytes < (size_t)index->n * sizeof(float)) {
+        set_error("the index has no screen bias for this precision yet: one mlk_knn_query call of the same "
+                  "precision prepares it");
+        return MLK_EINVAL;
+    }
     MLK_TRY(mr.bias.reserve((size_t)index->n * sizeof(float)));
     hipLaunchKernelGGL(k_bias, dim3((unsigned)((index->n + 255) / 256)), dim3(256), 0, stream,
                        mr.nx.as<double>(), mr.flags.as<uint8_t>(), index->n, G, mr.bias.as<float>());

@@ -75,6 +75,29 @@ struct DevBuf {
     template <class T> T* as() const { return static_cast<T*>(ptr); }
 };
 
+// Device memory of one call's temporaries. Allocating mode (arena == nullptr):
+// slot i of the index's grow-only workspace, reserved with hipMalloc when it
+// is too small. Fixed mode: consecutive 256-byte aligned pieces of a
+// preallocated arena, never grown; a piece that does not fit is an
+// MLK_ECAPACITY error. get() returns the piece in *out.
+struct CallMem {
+    DevBuf* ws = nullptr;
+    char* arena = nullptr;
+    size_t cap = 0, used = 0;
+    // > 0: k-NN calls shrink their plan's database splits until the candidate
+    // lists take at most this many bytes (fixed arenas)
+    size_t list_budget = 0;
+    bool fixed() const { return arena != nullptr; }
+    size_t left() const { return cap > used ? cap - used : 0; }
+    int get(int slot, size_t bytes, void** out);
+    template <class T> int get(int slot, size_t bytes, T** out) {
+        void* p = nullptr;
+        const int rc = get(slot, bytes, &p);
+        *out = static_cast<T*>(p);
+        return rc;
+    }
+};
+
 // Screen precisions.
 enum Prec { PREC_FP32 = 0, PREC_BF16 = 1, PREC_FP16 = 2 };
 
@@ -137,13 +160,27 @@ struct mlk_radius_result {
 };
 
 namespace mlk {
+// implemented in mlk_capi.hip: the injected-failure hook and the k-NN argument checks
+int check_fail_hook();
+int validate_knn(mlk_index* index, const float* Q, int64_t m, int32_t k, int64_t q_offset, uint32_t flags,
+                 const int64_t* idx);
 // implemented in mlk_index.hip
 int prepare_mirror(mlk_index* index, int prec, hipStream_t stream);
-int prepare_bias(mlk_index* index, int prec, double H, hipStream_t stream);
+// may_allocate false: the bias buffer must exist already (a bias for another
+// constant is rebuilt in place); otherwise MLK_EINVAL.
+int prepare_bias(mlk_index* index, int prec, double H, hipStream_t stream, bool may_allocate = true);
 // implemented in mlk_knn.hip
+// mem: where the call's temporaries live (nullptr: the index's workspace).
+// With a fixed arena the call allocates nothing, records no events and needs
+// the screen mirror and bias of its precision to exist already.
 int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q_offset, uint32_t flags,
-                   double* dist, int64_t* idx, hipStream_t stream, const int64_t* batch_offsets,
-                   int n_batches);
+                   double* dist, int64_t* idx, hipStream_t stream, CallMem* mem = nullptr);
+// Device bytes a fixed arena needs for one knn_query_impl call of m queries
+// with the plan such a call would use (0 if the index is not prepared for the
+// precision), not counting the exact fallback's per-query buffers.
+size_t knn_arena_bytes(const mlk_index* index, int prec, int64_t m, int k, uint32_t flags, size_t list_budget);
+// Device bytes of the exact fallback's buffers for one query against n rows.
+size_t knn_fallback_bytes_per_query(int64_t n);
 // implemented in mlk_tune.hip
 int autotune_impl(mlk_index* index, const float* Q, int64_t m, int k, double R2, uint32_t flags, int max_trials,
                   mlk_plan* out, hipStream_t stream);

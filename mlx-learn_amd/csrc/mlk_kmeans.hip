@@ -261,7 +261,7 @@ extern "C" int mlk_kmeans_assign(const float* X_dev, int64_t n, int32_t d, const
     } g{centres};
     const uint32_t qflags = (flags & MLK_PREC_MASK) | MLK_SQUARED_DIST;
     std::lock_guard<std::mutex> lk(centres->lock);
-    return knn_query_impl(centres, X_dev, n, 1, 0, qflags, residual_dev, labels_dev, (hipStream_t)stream, nullptr, 0);
+    return knn_query_impl(centres, X_dev, n, 1, 0, qflags, residual_dev, labels_dev, (hipStream_t)stream);
 }
 
 extern "C" int mlk_kmeans_assign_reuse(const float* X_dev, int64_t n, int32_t d, const float* C_dev, int64_t kc,
@@ -297,7 +297,7 @@ extern "C" int mlk_kmeans_assign_reuse(const float* X_dev, int64_t n, int32_t d,
         MLK_TRY(sep2.reserve((size_t)kc * sizeof(double)));
         MLK_TRY(near.reserve((size_t)kc * sizeof(int64_t)));
         MLK_TRY(knn_query_impl(centres, C_dev, kc, 1, 0, qflags | MLK_EXCLUDE_SELF, sep2.as<double>(),
-                               near.as<int64_t>(), stream, nullptr, 0));
+                               near.as<int64_t>(), stream));
     }
     hipLaunchKernelGGL(k_sep_lower, dim3(nblk(kc)), dim3(256), 0, stream, kc > 1 ? sep2.as<double>() : nullptr, kc, d,
                        sep_lo.as<double>());
@@ -326,7 +326,7 @@ extern "C" int mlk_kmeans_assign_reuse(const float* X_dev, int64_t n, int32_t d,
                 set_error("kmeans_assign_reuse: " + std::to_string(w[1]) + " previous labels outside [0, n_clusters)");
                 return MLK_EINVAL;
             }
-            MLK_TRY(knn_query_impl(centres, X_dev, n, 1, 0, qflags, residual_dev, labels_dev, stream, nullptr, 0));
+            MLK_TRY(knn_query_impl(centres, X_dev, n, 1, 0, qflags, residual_dev, labels_dev, stream));
             MLK_CHECK_HIP(hipStreamSynchronize(stream));
             return MLK_OK;
         }
@@ -355,7 +355,7 @@ extern "C" int mlk_kmeans_assign_reuse(const float* X_dev, int64_t n, int32_t d,
                            unc.as<int>(), nu, Xu.as<float>());
         MLK_CHECK_HIP(hipGetLastError());
         MLK_TRY(knn_query_impl(centres, Xu.as<float>(), nu, 1, 0, qflags, res_u.as<double>(), lab_u.as<int64_t>(),
-                               stream, nullptr, 0));
+                               stream));
         hipLaunchKernelGGL(k_scatter_labels, dim3(nblk(nu)), dim3(256), 0, stream, unc.as<int>(), nu,
                            lab_u.as<int64_t>(), res_u.as<double>(), labels_dev, residual_dev);
         MLK_CHECK_HIP(hipGetLastError());

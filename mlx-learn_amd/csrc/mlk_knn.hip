@@ -1166,6 +1166,11 @@ __global__ void k_finish_status(int64_t m, const int* __restrict__ status, int* 
     if (threadIdx.x < 3 && sh[threadIdx.x][0]) atomicAdd(&out[threadIdx.x], sh[threadIdx.x][0]);
 }
 
+__global__ void k_fill_int(int* __restrict__ out, int64_t count, int value) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) out[i] = value;
+}
+
 __global__ void k_force(int64_t m, const int64_t* __restrict__ rows, int64_t count, int* __restrict__ status,
                         float* __restrict__ thr) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1196,61 +1201,139 @@ Consts make_consts(int prec, int dp, int d, bool exact_inputs, double scale) {
     return c;
 }
 
+// The buffers of one k-NN call: the plan and every temporary's size. Shared by
+// the call itself and by knn_arena_bytes, so a fixed arena is sized exactly.
+struct KnnLayout {
+    Plan plan;
+    bool all_fallback = false;
+    bool lane_merge = false;
+    bool kbest_rows = false;
+    int64_t QB = 0, m_pad = 0;
+    int S = 0, S2 = 0, cap = 0;
+    size_t bytes_main = 0;  // everything but the fallback's per-query buffers
+};
+
+static KnnLayout knn_layout(const mlk_index* index, int prec, int64_t m, int k, const TestHooks& hk,
+                            size_t list_budget) {
+    KnnLayout L;
+    const Mirror& mr = index->mirror[prec];
+    const int dp = mr.dp;
+    L.plan = resolve_plan(PLAN_KNN, prec, dp, index->n, m, k);
+    const ScreenFn screen = pick_screen(SCREEN_LISTS, prec, dp, L.plan.shape);
+    L.all_fallback = !screen || k > 64 || mr.n_unsafe > 64 || (hk.hooks & MLK_HOOK_FORCE_FALLBACK);
+    if (hk.hooks & MLK_HOOK_CAPACITY) L.plan.cap = (int)std::max<int64_t>(1, hk.param);
+    const int BQ = 4 * L.plan.shape.qt * 32;
+    L.QB = (m + BQ - 1) / BQ;
+    L.m_pad = L.QB * BQ;
+    // a bounded list budget: fewer database splits (whole XCD groups while
+    // there are at least eight), the capacity following the split count
+    while (list_budget && L.plan.splits > 1 &&
+           (size_t)L.m_pad * 2 * L.plan.splits * (size_t)L.plan.cap * sizeof(int2) > list_budget) {
+        const int s = L.plan.splits / 2;
+        L.plan.splits = s >= 8 ? s / 8 * 8 : std::max(1, s);
+        if (!(hk.hooks & MLK_HOOK_CAPACITY)) L.plan.cap = knn_capacity(prec, L.plan.splits, k);
+    }
+    L.S = L.plan.splits;
+    L.S2 = 2 * L.S;
+    L.cap = L.plan.cap;
+    // one lane per query merges when a query has few sub-lists and k is small
+    L.lane_merge = L.S2 <= 16 && k <= 8;
+    // rows of the kbest entries (sized whenever the call may use them; the call
+    // decides once the screen's error constant is known)
+    L.kbest_rows = !L.lane_merge && k <= 32;
+    const size_t mp = (size_t)L.m_pad, al = 256;
+    auto piece = [&](size_t b) { return (b + al - 1) / al * al; };
+    size_t t = 0;
+    t += piece(mp * dp * elem_bytes(prec)) + piece(mp * 8) + piece(mp + 64) + 3 * piece(mp * 4) + piece(64);
+    if (!L.all_fallback) {
+        t += 2 * piece(mp * L.S2 * 4) + piece(mp * L.S2 * (size_t)L.cap * 8) + piece(mp * k * 4);
+        if (L.kbest_rows) t += piece(mp * k * 4);
+        t += piece((size_t)m * 12);
+    }
+    if (!hk.forced_rows.empty()) t += piece(hk.forced_rows.size() * 8);
+    t += piece((size_t)m * 4) + piece(64 * 5);
+    L.bytes_main = t;
+    return L;
+}
+
+size_t knn_fallback_bytes_per_query(int64_t n) {
+    // float64 distances, candidate rows, their float64 values, and the radix state
+    return (size_t)n * (sizeof(double) * 2 + sizeof(int)) + sizeof(int) + sizeof(double) +
+           sizeof(unsigned long long) + sizeof(unsigned) + 1024;
+}
+
+size_t knn_arena_bytes(const mlk_index* index, int prec, int64_t m, int k, uint32_t flags, size_t list_budget) {
+    (void)flags;
+    if (!index->mirror[prec].ready) return 0;
+    return knn_layout(index, prec, m, k, hooks_snapshot(), list_budget).bytes_main;
+}
+
 int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q_offset, uint32_t flags,
-                   double* dist, int64_t* idx, hipStream_t stream, const int64_t* /*batch_offsets*/,
-                   int /*n_batches*/) {
+                   double* dist, int64_t* idx, hipStream_t stream, CallMem* mem_in) {
     mlk_stats& st = stats();
     st = mlk_stats{};
     st.queries = m;
     if (m == 0) return MLK_OK;
+    CallMem own;
+    own.ws = index->ws;
+    CallMem& mem = mem_in ? *mem_in : own;
+    const bool fixed = mem.fixed();  // no allocation, no events
     const int64_t n = index->n;
     const int d = index->d;
     const int prec = effective_prec((int)((flags & MLK_PREC_MASK) >> 4), d);
     const bool exclude = (flags & MLK_EXCLUDE_SELF) != 0;
     st.screen_precision = (uint32_t)prec << 4;  // the precision actually screened
 
-    Events events;  // every event of this call
-    hipEvent_t ev0, ev1, ev2;
-    MLK_CHECK_HIP(events.make(&ev0));
-    MLK_CHECK_HIP(events.make(&ev1));
-    MLK_CHECK_HIP(events.make(&ev2));
-    MLK_CHECK_HIP(hipEventRecord(ev0, stream));
+    Events events;  // every event of this call (none in fixed mode)
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+    if (!fixed) {
+        MLK_CHECK_HIP(events.make(&ev0));
+        MLK_CHECK_HIP(events.make(&ev1));
+        MLK_CHECK_HIP(events.make(&ev2));
+        MLK_CHECK_HIP(hipEventRecord(ev0, stream));
+    }
 
+    if (fixed && !index->mirror[prec].ready) {
+        set_error("the index has no screen mirror for this precision yet: one mlk_knn_query call of the same "
+                  "precision prepares it");
+        return MLK_EINVAL;
+    }
     MLK_TRY(prepare_mirror(index, prec, stream));
     Mirror& mr = index->mirror[prec];
     const int dp = mr.dp;
-    ScreenFn screen = nullptr;
-    Plan plan = resolve_plan(PLAN_KNN, prec, dp, n, m, k);
-    screen = pick_screen(SCREEN_LISTS, prec, dp, plan.shape);
     const TestHooks hk = hooks_snapshot();
-    const bool all_fallback = !screen || k > 64 || mr.n_unsafe > 64 || (hk.hooks & MLK_HOOK_FORCE_FALLBACK);
-    if (hk.hooks & MLK_HOOK_CAPACITY) plan.cap = (int)std::max<int64_t>(1, hk.param);
-
-    const int BQ = 4 * plan.shape.qt * 32;
-    const int64_t QB = (m + BQ - 1) / BQ;
-    const int64_t m_pad = QB * BQ;
-    const int S = plan.splits;
-    const int cap = plan.cap;
+    const KnnLayout lay = knn_layout(index, prec, m, k, hk, mem.list_budget);
+    const Plan& plan = lay.plan;
+    const ScreenFn screen = pick_screen(SCREEN_LISTS, prec, dp, plan.shape);
+    const bool all_fallback = lay.all_fallback;
+    const int64_t QB = lay.QB;
+    const int64_t m_pad = lay.m_pad;
+    const int S = lay.S;
+    const int cap = lay.cap;
     st.splits = S;
     st.capacity = cap;
 
-    // workspace
-    DevBuf* ws = index->ws;
+    // temporaries
     const size_t eb = elem_bytes(prec);
-    MLK_TRY(ws[0].reserve((size_t)m_pad * dp * eb));          // query mirror
-    MLK_TRY(ws[1].reserve((size_t)m_pad * sizeof(double)));   // nq
-    MLK_TRY(ws[2].reserve((size_t)m_pad + 64));               // qflags (+ flag words)
-    MLK_TRY(ws[3].reserve((size_t)m_pad * sizeof(float)));    // tau
-    MLK_TRY(ws[4].reserve((size_t)m_pad * sizeof(float)));    // thr
-    MLK_TRY(ws[5].reserve((size_t)m_pad * sizeof(int)));      // status
+    void* qmirror;
+    double* nq;
+    uint8_t* qflags;
+    MLK_TRY(mem.get(0, (size_t)m_pad * dp * eb, &qmirror));     // query mirror
+    MLK_TRY(mem.get(1, (size_t)m_pad * sizeof(double), &nq));   // nq
+    MLK_TRY(mem.get(2, (size_t)m_pad + 64, &qflags));           // qflags
+    float* tau;
+    float* thr;
+    int* status;
+    MLK_TRY(mem.get(3, (size_t)m_pad * sizeof(float), &tau));
+    MLK_TRY(mem.get(4, (size_t)m_pad * sizeof(float), &thr));
+    MLK_TRY(mem.get(5, (size_t)m_pad * sizeof(int), &status));
     unsigned* dflag;
-    MLK_TRY(ws[6].reserve(64));
-    dflag = ws[6].as<unsigned>();
+    MLK_TRY(mem.get(6, 64, &dflag));
     // flag words: [0] query rounding inexact, [2] fallback count, [4..10) three
     // 64-bit statistics sums
     MLK_CHECK_HIP(hipMemsetAsync(dflag, 0, 40, stream));
-    MLK_TRY(mirror_rows(prec, Q, m, m_pad, d, dp, index->centre.as<float>(), mr.scale, ws[0].ptr,
-                        ws[1].as<double>(), ws[2].as<uint8_t>(), dflag, stream));
+    MLK_TRY(mirror_rows(prec, Q, m, m_pad, d, dp, index->centre.as<float>(), mr.scale, qmirror, nq, qflags, dflag,
+                        stream));
     // the queries' rounding only matters when every database value is exact
     // in the screen type (integer data); otherwise no wait here
     unsigned q_inexact = 1;
@@ -1259,31 +1342,28 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
         MLK_CHECK_HIP(hipStreamSynchronize(stream));
     }
     const Consts c = make_consts(prec, dp, d, mr.exact && q_inexact == 0, mr.scale);
-    MLK_TRY(prepare_bias(index, prec, c.G, stream));
+    MLK_TRY(prepare_bias(index, prec, c.G, stream, !fixed));
 
     const float* X = index->X.as<float>();
-    float* tau = ws[3].as<float>();
-    float* thr = ws[4].as<float>();
-    int* status = ws[5].as<int>();
     // sampled start threshold: 32 rows per neighbour asked for, 64 to 256 rows
     const int tau_samples = std::min(TAU0_SAMPLES, std::max(64, 32 * k));
-    const int S2 = 2 * S;  // candidate sub-lists per query (split x lane half)
-    // one lane per query merges when a query has few sub-lists and k is small
-    const bool lane_merge = S2 <= 16 && k <= 8;
-    // rows of the kbest entries: the wave merge tightens tau with their float64
-    // distances, which pays where the screen's band is wide (relative error
-    // constant >= 2^-11: bf16, and fp16 on inexact inputs); float32 screens and
-    // inputs that are exact in the screen type have a narrow band already
+    const int S2 = lay.S2;  // candidate sub-lists per query (split x lane half)
+    const bool lane_merge = lay.lane_merge;
     int* kbest_row = nullptr;
+    int* cnt = nullptr;
+    int2* cand = nullptr;
+    int* seen = nullptr;
+    float* kbest = nullptr;
     if (!all_fallback) {
-        MLK_TRY(ws[8].reserve((size_t)m_pad * S2 * sizeof(int)));          // counts
-        MLK_TRY(ws[9].reserve((size_t)m_pad * S2 * cap * sizeof(int2)));   // (row, dot-product estimate)
-        MLK_TRY(ws[17].reserve((size_t)m_pad * S2 * sizeof(int)));         // merged counts
-        MLK_TRY(ws[18].reserve((size_t)m_pad * k * sizeof(float)));        // k smallest upper bounds
-        if (!lane_merge && k <= 32 && c.H >= 0x1p-11) {
-            MLK_TRY(ws[21].reserve((size_t)m_pad * k * sizeof(int)));     // their rows (cleared by tau0)
-            kbest_row = ws[21].as<int>();
-        }
+        MLK_TRY(mem.get(8, (size_t)m_pad * S2 * sizeof(int), &cnt));                 // counts
+        MLK_TRY(mem.get(9, (size_t)m_pad * S2 * cap * sizeof(int2), &cand));         // (row, dot-product estimate)
+        MLK_TRY(mem.get(17, (size_t)m_pad * S2 * sizeof(int), &seen));               // merged counts
+        MLK_TRY(mem.get(18, (size_t)m_pad * k * sizeof(float), &kbest));             // k smallest upper bounds
+        // rows of the kbest entries: the wave merge tightens tau with their float64
+        // distances, which pays where the screen's band is wide (relative error
+        // constant >= 2^-11: bf16, and fp16 on inexact inputs); float32 screens and
+        // inputs that are exact in the screen type have a narrow band already
+        if (lay.kbest_rows && c.H >= 0x1p-11) MLK_TRY(mem.get(21, (size_t)m_pad * k * sizeof(int), &kbest_row));
     }
     {
         // U sample rows per lane (64 U samples), QW queries per wave: U * QW = 32
@@ -1292,9 +1372,8 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
         auto launch = [&](auto kern, int qw) {
             const int64_t waves = (m_pad + qw - 1) / qw;
             hipLaunchKernelGGL(kern, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, stream, X, n, d, Q, m, m_pad,
-                               k, tau_samples, q_offset, (int)exclude, ws[1].as<double>(), ws[2].as<uint8_t>(), c,
-                               tau, thr, status, S2, all_fallback ? nullptr : ws[8].as<int>(), ws[17].as<int>(),
-                               ws[18].as<float>(), kbest_row);
+                               k, tau_samples, q_offset, (int)exclude, nq, qflags, c, tau, thr, status, S2,
+                               all_fallback ? nullptr : cnt, seen, kbest, kbest_row);
         };
         // the most queries per wave (register reuse of the samples) that still
         // leaves about 4096 waves (4 per SIMD) to fill the device
@@ -1327,15 +1406,17 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
 
     if ((hk.hooks & MLK_HOOK_FORCE_UNCERTIFIED) && !hk.forced_rows.empty()) {
         const auto& rows = hk.forced_rows;
-        MLK_TRY(ws[7].reserve(rows.size() * sizeof(int64_t)));
-        MLK_CHECK_HIP(hipMemcpyAsync(ws[7].ptr, rows.data(), rows.size() * sizeof(int64_t), hipMemcpyHostToDevice,
+        int64_t* drows;
+        MLK_TRY(mem.get(7, rows.size() * sizeof(int64_t), &drows));
+        MLK_CHECK_HIP(hipMemcpyAsync(drows, rows.data(), rows.size() * sizeof(int64_t), hipMemcpyHostToDevice,
                                      stream));
-        hipLaunchKernelGGL(k_force, dim3((unsigned)((rows.size() + 255) / 256)), dim3(256), 0, stream, m,
-                           ws[7].as<int64_t>(), (int64_t)rows.size(), status, thr);
+        hipLaunchKernelGGL(k_force, dim3((unsigned)((rows.size() + 255) / 256)), dim3(256), 0, stream, m, drows,
+                           (int64_t)rows.size(), status, thr);
+        MLK_CHECK_HIP(hipStreamSynchronize(stream));  // the host rows are a snapshot of this call
         MLK_CHECK_HIP(hipGetLastError());
     }
 
-    MLK_CHECK_HIP(hipEventRecord(ev1, stream));
+    if (!fixed) MLK_CHECK_HIP(hipEventRecord(ev1, stream));
     std::vector<hipEvent_t> round_events;
     const int* rank_stats = nullptr;  // per-query counts written by the rank (screened calls)
     bool big_rank = false;            // k_rank ran: queries marked ST_BIG are ranked again
@@ -1345,10 +1426,10 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
         sa.xs = mr.xs.ptr;
         sa.bias = mr.bias.as<float>();
         sa.nx32 = mr.nx32.as<float>();
-        sa.qs = ws[0].ptr;
+        sa.qs = qmirror;
         sa.thr = thr;
-        sa.cnt = ws[8].as<int>();
-        sa.cand = ws[9].as<int2>();
+        sa.cnt = cnt;
+        sa.cand = cand;
         sa.n = n;
         sa.split_rows = (n + S - 1) / S;
         sa.S = S;
@@ -1363,12 +1444,12 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
         auto merge = [&]() -> int {
             if (lane_merge)
                 hipLaunchKernelGGL(k == 1 ? k_merge_lane<1> : k_merge_lane<8>, dim3((unsigned)((m + 255) / 256)),
-                                   dim3(256), 0, stream, m, k, sa.cnt, ws[17].as<int>(), sa.cand, sa.nx32, S2, cap,
-                                   ws[18].as<float>(), ws[1].as<double>(), c, tau, thr, status);
+                                   dim3(256), 0, stream, m, k, sa.cnt, seen, sa.cand, sa.nx32, S2, cap, kbest, nq, c,
+                                   tau, thr, status);
             else
                 hipLaunchKernelGGL(k_merge, dim3((unsigned)((m + 3) / 4)), dim3(256), 0, stream, m, k, sa.cnt,
-                                   ws[17].as<int>(), sa.cand, sa.nx32, S2, cap, ws[18].as<float>(),
-                                   ws[1].as<double>(), c, tau, thr, status, kbest_row, X, Q, d);
+                                   seen, sa.cand, sa.nx32, S2, cap, kbest, nq, c, tau, thr, status, kbest_row, X, Q,
+                                   d);
             MLK_CHECK_HIP(hipGetLastError());
             return MLK_OK;
         };
@@ -1380,32 +1461,35 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
             sa.off_hi = r + 1 == plan.rounds.size() ? sa.split_rows : (int64_t)std::llround(hi * sa.split_rows);
             prev = hi;
             if (sa.off_hi <= sa.off_lo) continue;
-            hipEvent_t ea, eb;
-            MLK_CHECK_HIP(events.make(&ea));
-            MLK_CHECK_HIP(events.make(&eb));
-            round_ev.push_back(ea);
-            round_ev.push_back(eb);
-            MLK_CHECK_HIP(hipEventRecord(ea, stream));
+            hipEvent_t ea = nullptr, eb = nullptr;
+            if (!fixed) {
+                MLK_CHECK_HIP(events.make(&ea));
+                MLK_CHECK_HIP(events.make(&eb));
+                round_ev.push_back(ea);
+                round_ev.push_back(eb);
+                MLK_CHECK_HIP(hipEventRecord(ea, stream));
+            }
             hipLaunchKernelGGL(screen, dim3(blocks), dim3(256), 0, stream, sa);
             MLK_CHECK_HIP(hipGetLastError());
-            MLK_CHECK_HIP(hipEventRecord(eb, stream));
+            if (!fixed) MLK_CHECK_HIP(hipEventRecord(eb, stream));
             ++st.rounds;
             if (r + 1 < plan.rounds.size()) MLK_TRY(merge());
         }
         round_events = std::move(round_ev);
         // final overflow check (the last round has no merge)
         MLK_TRY(merge());
-        MLK_CHECK_HIP(hipEventRecord(ev2, stream));
+        if (!fixed) MLK_CHECK_HIP(hipEventRecord(ev2, stream));
 
         ra.X = X; ra.n = n; ra.d = d; ra.Q = Q; ra.m = m; ra.k = k;
-        ra.cnt = sa.cnt; ra.cand = sa.cand; ra.nx32 = sa.nx32; ra.nq = ws[1].as<double>();
+        ra.cnt = sa.cnt; ra.cand = sa.cand; ra.nx32 = sa.nx32; ra.nq = nq;
         ra.S = S2; ra.cap = cap;
         ra.tau = tau; ra.c = c;
         ra.unsafe = mr.unsafe.as<int>(); ra.n_unsafe = (int)mr.n_unsafe;
         ra.mode = 0; ra.self_off = q_offset; ra.exclude = exclude; ra.status = status;
         ra.out_idx = idx; ra.out_dist = dist; ra.squared = (flags & MLK_SQUARED_DIST) != 0;
-        MLK_TRY(ws[19].reserve((size_t)m * 3 * sizeof(int)));
-        ra.qstats = ws[19].as<int>();  // written for every query by k_rank
+        int* qstats;
+        MLK_TRY(mem.get(19, (size_t)m * 3 * sizeof(int), &qstats));
+        ra.qstats = qstats;  // written for every query by k_rank
         const bool vec = d % 4 == 0 && ((uintptr_t)Q % 16) == 0 && ((uintptr_t)X % 16) == 0;
         if (k == 1 && mr.n_unsafe == 0 && vec && d <= 256) {
             hipLaunchKernelGGL(d <= 64 ? k_rank1_group<2> : d <= 128 ? k_rank1_group<4> : k_rank1_group<8>,
@@ -1421,15 +1505,15 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
         MLK_CHECK_HIP(hipGetLastError());
         rank_stats = ra.qstats;
     } else {
-        MLK_CHECK_HIP(hipEventRecord(ev2, stream));
-        std::vector<int> all(m, ST_FALLBACK);
-        MLK_CHECK_HIP(hipMemcpyAsync(status, all.data(), m * sizeof(int), hipMemcpyHostToDevice, stream));
-        MLK_CHECK_HIP(hipStreamSynchronize(stream));
+        if (!fixed) MLK_CHECK_HIP(hipEventRecord(ev2, stream));
+        hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, stream, status, m,
+                           ST_FALLBACK);
+        MLK_CHECK_HIP(hipGetLastError());
     }
 
     // queries without a certificate -> exact fallback
-    MLK_TRY(ws[11].reserve((size_t)m * sizeof(int)));
-    int* fq = ws[11].as<int>();
+    int* fq;
+    MLK_TRY(mem.get(11, (size_t)m * sizeof(int), &fq));
     hipLaunchKernelGGL(k_finish_status, dim3((unsigned)std::min<int64_t>(1024, (m + 255) / 256)), dim3(256), 0,
                        stream, m, status, fq, dflag + 2, rank_stats, (unsigned long long*)(dflag + 4));
     MLK_CHECK_HIP(hipGetLastError());
@@ -1467,17 +1551,30 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
     st.reranked = (int64_t)words.cnts[2];
     if (nf) {
         std::sort(fq_host.begin(), fq_host.end());
-        MLK_CHECK_HIP(hipMemcpy(fq, fq_host.data(), nf * sizeof(int), hipMemcpyHostToDevice));
-        // batches bounded by ~1 GiB of float64 distances + candidate lists
-        const int64_t per_q = n * (int64_t)(sizeof(double) * 2 + sizeof(int));
-        int64_t fb = std::max<int64_t>(1, std::min<int64_t>(nf, (int64_t(1) << 30) / std::max<int64_t>(per_q, 1)));
-        MLK_TRY(ws[12].reserve((size_t)fb * n * sizeof(double)));  // distances
-        MLK_TRY(ws[13].reserve((size_t)fb * n * sizeof(int)));     // plist
-        MLK_TRY(ws[14].reserve((size_t)fb * n * sizeof(double)));  // ps
+        MLK_CHECK_HIP(hipMemcpyAsync(fq, fq_host.data(), nf * sizeof(int), hipMemcpyHostToDevice, stream));
+        // batches bounded by ~1 GiB of float64 distances + candidate lists (in
+        // fixed mode: by what is left of the arena)
+        const int64_t per_q = (int64_t)knn_fallback_bytes_per_query(n) + 8 * 256;
+        const int64_t budget = fixed ? (int64_t)mem.left() - 16 * 256 : (int64_t(1) << 30);
+        int64_t fb = std::min<int64_t>(nf, budget / std::max<int64_t>(per_q, 1));
+        if (fb < 1) {
+            if (fixed) {
+                set_error("the exact fallback needs " + std::to_string(per_q) + " bytes per query; the fixed "
+                          "scratch arena has " + std::to_string(mem.left()) + " left");
+                return MLK_ECAPACITY;
+            }
+            fb = 1;
+        }
+        double* fb_dist;
+        int* fb_plist;
+        double* fb_ps;
+        MLK_TRY(mem.get(12, (size_t)fb * n * sizeof(double), &fb_dist));  // distances
+        MLK_TRY(mem.get(13, (size_t)fb * n * sizeof(int), &fb_plist));    // plist
+        MLK_TRY(mem.get(14, (size_t)fb * n * sizeof(double), &fb_ps));    // ps
         // per fallback query: count, cut, radix prefix and rank, 256-bin histogram
         const size_t per = sizeof(int) + sizeof(double) + sizeof(unsigned long long) + sizeof(unsigned) + 1024;
-        MLK_TRY(ws[15].reserve((size_t)fb * per + 5 * 64));  // five pieces, each 64-byte aligned
-        char* wp = (char*)ws[15].ptr;
+        char* wp;
+        MLK_TRY(mem.get(15, (size_t)fb * per + 5 * 64, &wp));  // five pieces, each 64-byte aligned
         auto carve = [&](size_t bytes) {
             char* at = wp;
             wp += (bytes + 63) / 64 * 64;
@@ -1493,25 +1590,31 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
             dim3 g2((unsigned)((n + 255) / 256), (unsigned)nb);
             dim3 g16((unsigned)((n + 15) / 16), (unsigned)nb);
             hipLaunchKernelGGL(k_fb_dist, g16, dim3(256), 0, stream, X, n, d, Q, fq + f0, q_offset, (int)exclude,
-                               ws[12].as<double>(), k, hist, rprefix, rrank);
+                               fb_dist, k, hist, rprefix, rrank);
             // about 4096 values per histogram block
             const dim3 gh((unsigned)std::max<int64_t>(1, std::min<int64_t>(256, (n + 4095) / 4096)), (unsigned)nb);
             for (int shift = 56; shift >= 0; shift -= 8) {
-                hipLaunchKernelGGL(k_fb_hist, gh, dim3(256), 0, stream, ws[12].as<double>(), n, shift, rprefix, hist);
+                hipLaunchKernelGGL(k_fb_hist, gh, dim3(256), 0, stream, fb_dist, n, shift, rprefix, hist);
                 hipLaunchKernelGGL(k_fb_pick, dim3((unsigned)nb), dim3(256), 0, stream, shift, d, hist, rprefix,
                                    rrank, cut);
             }
             MLK_CHECK_HIP(hipMemsetAsync(pcount, 0, nb * sizeof(int), stream));
-            hipLaunchKernelGGL(k_fb_collect, g2, dim3(256), 0, stream, ws[12].as<double>(), n, cut, pcount,
-                               ws[13].as<int>(), ws[14].as<double>(), n);
+            hipLaunchKernelGGL(k_fb_collect, g2, dim3(256), 0, stream, fb_dist, n, cut, pcount,
+                               fb_plist, fb_ps, n);
             RankArgs fa{};
             fa.X = X; fa.n = n; fa.d = d; fa.Q = Q; fa.m = m; fa.k = k;
-            fa.fq = fq + f0; fa.plist = ws[13].as<int>(); fa.ps = ws[14].as<double>(); fa.pcount = pcount;
+            fa.fq = fq + f0; fa.plist = fb_plist; fa.ps = fb_ps; fa.pcount = pcount;
             fa.pstride = n; fa.mode = 1; fa.self_off = q_offset; fa.exclude = exclude; fa.status = status;
             fa.out_idx = idx; fa.out_dist = dist; fa.squared = (flags & MLK_SQUARED_DIST) != 0;
             hipLaunchKernelGGL(k_rank, dim3((unsigned)((nb + 3) / 4)), dim3(256), 0, stream, fa, nb);
             MLK_CHECK_HIP(hipGetLastError());
         }
+    }
+    if (fixed) {
+        // no events: the outputs are ordered on the stream; the fallback's host
+        // list must outlive its copy
+        if (nf) MLK_CHECK_HIP(hipStreamSynchronize(stream));
+        return MLK_OK;
     }
     hipEvent_t ev3;
     MLK_CHECK_HIP(events.make(&ev3));

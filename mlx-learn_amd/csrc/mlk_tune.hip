@@ -294,7 +294,7 @@ int autotune_impl(mlk_index* index, const float* Q, int64_t m, int k, double R2,
     auto call = [&](const Plan& p) -> int {
         TrialGuard tg(&p);
         if (kind == PLAN_KNN)
-            return knn_query_impl(index, Q, m, k, 0, qflags, dist.as<double>(), idx.as<int64_t>(), stream, nullptr, 0);
+            return knn_query_impl(index, Q, m, k, 0, qflags, dist.as<double>(), idx.as<int64_t>(), stream);
         int64_t nnz = 0;
         mlk_radius_result* res = nullptr;
         const int rc = radius_query_impl(index, Q, m, R2, 0, qflags, indptr.as<int64_t>(), &nnz, &res, stream);

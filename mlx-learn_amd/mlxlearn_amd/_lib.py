@@ -67,6 +67,7 @@ SIGNATURES = {
     "mlk_index_destroy": ([_vp], ctypes.c_int),
     "mlk_index_info": ([_vp, _i64p, ctypes.POINTER(_i32), ctypes.POINTER(_i32)], ctypes.c_int),
     "mlk_knn_query": ([_vp, _vp, _i64, _i32, _i64, _u32, _vp, _vp, _vp], ctypes.c_int),
+    "mlk_knn_query_noalloc": ([_vp, _vp, _i64, _i32, _i64, _u32, _vp, _vp, _vp], ctypes.c_int),
     "mlk_knn_query_batched": ([_vp, _vp, _i64p, _i32, _i32, _u32, _vp, _vp, _vp], ctypes.c_int),
     "mlk_radius_query": ([_vp, _vp, _i64, _dbl, _i64, _u32, _vp, _i64p, ctypes.POINTER(_vp), _vp], ctypes.c_int),
     "mlk_radius_fill": ([_vp, _vp, _vp, _vp], ctypes.c_int),
