@@ -125,9 +125,9 @@ int mlk_set_forced_rows(const int64_t* rows, int64_t count) {
 int mlk_set_tuning(int32_t splits, int32_t capacity, int32_t query_tiles, int32_t stages, int32_t n_rounds,
                    const double* round_fractions) {
     std::lock_guard<std::mutex> g(g_config_lock);
-    if (splits < 0 || capacity < 0 || query_tiles < 0 || query_tiles > 2 || stages < 0 || stages == 1 || stages > 4 ||
-        n_rounds < 0)
-        return invalid("tuning values must be non-negative (query_tiles <= 2, stages 2-4)");
+    if (splits < 0 || capacity < 0 || query_tiles < 0 || query_tiles == 3 || query_tiles > 4 || stages < 0 ||
+        stages == 1 || stages > 4 || n_rounds < 0)
+        return invalid("tuning values must be non-negative (query_tiles 1, 2 or 4, stages 2-4)");
     Tuning t;
     t.splits = splits;
     t.capacity = capacity;

@@ -105,7 +105,8 @@ __global__ __launch_bounds__(256) void k_mirror(const float* __restrict__ X, int
                                                 const float* __restrict__ c, double scale,
                                                 typename ScreenType<PREC>::T* __restrict__ xs,
                                                 double* __restrict__ norms, uint8_t* __restrict__ flags,
-                                                unsigned* __restrict__ inexact) {
+                                                unsigned* __restrict__ inexact, double* __restrict__ nt_out,
+                                                double* __restrict__ d2_out) {
     // R lanes per row (64 / R rows per wave); lane r holds features r + R t (t < T,
     // T R >= dp) as float64 between the two passes, so every row is read once
     typedef typename ScreenType<PREC>::T T_;
@@ -113,9 +114,11 @@ __global__ __launch_bounds__(256) void k_mirror(const float* __restrict__ X, int
     const int64_t row = ((int64_t)blockIdx.x * 256 + threadIdx.x) / R;
     if (row >= n_pad) return;  // whole row groups leave together
     const bool real = row < n;
-    // pass 1: norm (float64, any order), safe range, exactness; pass 2: the mirror
-    // values (zero for rows outside the safe range, whose screen products must stay finite)
-    double s = 0.0;
+    // pass 1: norm (float64, any order), safe range, exactness, and the squared
+    // norms of the rounded row (st) and of its rounding residual (sd; each
+    // difference x - x~ is exact); pass 2: the mirror values (zero for rows
+    // outside the safe range, whose screen products must stay finite)
+    double s = 0.0, st = 0.0, sd = 0.0;
     bool safe = true, exact = true;
     if constexpr (T == 0) {  // any width: two reads of the row, 64 lanes
         static_assert(R == 64, "looping mirror uses whole waves");
@@ -123,10 +126,15 @@ __global__ __launch_bounds__(256) void k_mirror(const float* __restrict__ X, int
             const double x = ((double)X[row * d + j] - (double)c[j]) * scale;
             s += x * x;
             safe &= safe_value<PREC>(x);
-            exact &= (double)(float)to_screen<PREC>(x) == x;
+            const double xt = (double)(float)to_screen<PREC>(x);
+            exact &= xt == x;
+            st += xt * xt;
+            sd += (x - xt) * (x - xt);
         }
         for (int o = R / 2; o > 0; o >>= 1) {
             s += __shfl_xor(s, o, R);
+            st += __shfl_xor(st, o, R);
+            sd += __shfl_xor(sd, o, R);
             safe = __shfl_xor((int)safe, o, R) & safe;
             exact = __shfl_xor((int)exact, o, R) & exact;
         }
@@ -145,12 +153,17 @@ __global__ __launch_bounds__(256) void k_mirror(const float* __restrict__ X, int
             v[t] = ((double)X[row * d + j] - (double)c[j]) * scale;
             s += v[t] * v[t];
             safe &= safe_value<PREC>(v[t]);
-            exact &= (double)(float)to_screen<PREC>(v[t]) == v[t];
+            const double xt = (double)(float)to_screen<PREC>(v[t]);
+            exact &= xt == v[t];
+            st += xt * xt;
+            sd += (v[t] - xt) * (v[t] - xt);
         }
     }
 #pragma unroll
     for (int o = R / 2; o > 0; o >>= 1) {
         s += __shfl_xor(s, o, R);
+        st += __shfl_xor(st, o, R);
+        sd += __shfl_xor(sd, o, R);
         safe = __shfl_xor((int)safe, o, R) & safe;
         exact = __shfl_xor((int)exact, o, R) & exact;
     }
@@ -162,6 +175,8 @@ __global__ __launch_bounds__(256) void k_mirror(const float* __restrict__ X, int
     }
     if (r == 0) {
         norms[row] = real ? s : 0.0;
+        if (nt_out) nt_out[row] = real ? st : 0.0;
+        if (d2_out) d2_out[row] = real ? sd : 0.0;
         flags[row] = (real && safe) ? 1 : 0;
         // one flag for the whole mirror: skip the (same-address) atomic once it is set
         if (real && safe && !exact && __hip_atomic_load(inexact, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u)
@@ -169,13 +184,13 @@ __global__ __launch_bounds__(256) void k_mirror(const float* __restrict__ X, int
     }
 }
 
-// Negated bias -a_x with a_x = round_down((1 - G) * nx / 2) (the screen's
+// Negated bias -a_x, a_x = screen_row_bound(...) rounded down (the screen's
 // accumulator start value); unsafe rows get a start value no threshold reaches.
-__global__ void k_bias(const double* __restrict__ nx, const uint8_t* __restrict__ flags, int64_t n, double G,
-                       float* __restrict__ bias) {
+__global__ void k_bias(const double* __restrict__ nx, const double* __restrict__ nt, const double* __restrict__ d2,
+                       const uint8_t* __restrict__ flags, int64_t n, Consts c, float* __restrict__ bias) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    bias[i] = flags[i] ? -__double2float_rd((1.0 - G) * nx[i] * 0.5) : -3.0e38f;
+    bias[i] = flags[i] ? -__double2float_rd(screen_row_bound(nx[i], nt[i], d2[i], c)) : -3.0e38f;
 }
 
 __global__ void k_to_f32(const double* __restrict__ v, int64_t n, float* __restrict__ out) {
@@ -194,14 +209,14 @@ __global__ void k_unsafe_list(const uint8_t* __restrict__ flags, int64_t n, int*
 
 template <int PREC>
 static void launch_mirror(const float* X, int64_t n, int64_t n_pad, int d, int dp, const float* c,
-                          double scale, void* xs, double* norms, uint8_t* flags, unsigned* inexact,
-                          hipStream_t stream) {
+                          double scale, void* xs, double* norms, uint8_t* flags, unsigned* inexact, double* nt,
+                          double* d2, hipStream_t stream) {
     typedef typename ScreenType<PREC>::T T_;
     auto go = [&](auto kern, int lanes) {
         const int64_t rows_per_block = 256 / lanes;
         dim3 grid((unsigned)((n_pad + rows_per_block - 1) / rows_per_block));
         hipLaunchKernelGGL(kern, grid, dim3(256), 0, stream, X, n, n_pad, d, dp, c, scale, (T_*)xs, norms, flags,
-                           inexact);
+                           inexact, nt, d2);
     };
     // lanes per row: about four features per lane, at least 16 lanes
     if (dp <= 64) go(k_mirror<PREC, 16, 4>, 16);
@@ -212,13 +227,17 @@ static void launch_mirror(const float* X, int64_t n, int64_t n_pad, int d, int d
 }
 
 int mirror_rows(int prec, const float* X, int64_t n, int64_t n_pad, int d, int dp, const float* c,
-                double scale, void* xs, double* norms, uint8_t* flags, unsigned* inexact,
+                double scale, void* xs, double* norms, uint8_t* flags, unsigned* inexact, double* nt, double* d2,
                 hipStream_t stream) {
     if (n_pad == 0) return MLK_OK;
     switch (prec) {
-        case PREC_FP32: launch_mirror<PREC_FP32>(X, n, n_pad, d, dp, c, scale, xs, norms, flags, inexact, stream); break;
-        case PREC_BF16: launch_mirror<PREC_BF16>(X, n, n_pad, d, dp, c, scale, xs, norms, flags, inexact, stream); break;
-        default: launch_mirror<PREC_FP16>(X, n, n_pad, d, dp, c, scale, xs, norms, flags, inexact, stream); break;
+        case PREC_FP32:
+            launch_mirror<PREC_FP32>(X, n, n_pad, d, dp, c, scale, xs, norms, flags, inexact, nt, d2, stream);
+            break;
+        case PREC_BF16:
+            launch_mirror<PREC_BF16>(X, n, n_pad, d, dp, c, scale, xs, norms, flags, inexact, nt, d2, stream);
+            break;
+        default: launch_mirror<PREC_FP16>(X, n, n_pad, d, dp, c, scale, xs, norms, flags, inexact, nt, d2, stream); break;
     }
     MLK_CHECK_HIP(hipGetLastError());
     return MLK_OK;
@@ -255,10 +274,13 @@ int prepare_mirror(mlk_index* index, int prec, hipStream_t stream) {
     mr.scale = scale;
     MLK_TRY(mr.xs.reserve((size_t)n * mr.dp * elem_bytes(prec)));
     MLK_TRY(mr.nx.reserve((size_t)n * sizeof(double)));
+    MLK_TRY(mr.nt.reserve((size_t)n * sizeof(double)));
+    MLK_TRY(mr.d2.reserve((size_t)n * sizeof(double)));
     MLK_TRY(mr.flags.reserve((size_t)n));
     MLK_CHECK_HIP(hipMemsetAsync(dflag, 0, 8, stream));
     MLK_TRY(mirror_rows(prec, index->X.as<float>(), n, n, d, mr.dp, index->centre.as<float>(), scale,
-                        mr.xs.ptr, mr.nx.as<double>(), mr.flags.as<uint8_t>(), dflag, stream));
+                        mr.xs.ptr, mr.nx.as<double>(), mr.flags.as<uint8_t>(), dflag, mr.nt.as<double>(),
+                        mr.d2.as<double>(), stream));
     MLK_TRY(mr.nx32.reserve((size_t)n * sizeof(float)));
     hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, mr.nx.as<double>(), n,
                        mr.nx32.as<float>());
@@ -282,7 +304,7 @@ int prepare_mirror(mlk_index* index, int prec, hipStream_t stream) {
                                      stream));
         MLK_CHECK_HIP(hipStreamSynchronize(stream));  // `rows` leaves scope
     }
-    mr.bias_h = -1.0;
+    mr.bias_t1 = mr.bias_t2 = -1.0;
     mr.ready = true;
     return MLK_OK;
 }
@@ -304,9 +326,9 @@ int CallMem::get(int slot, size_t bytes, void** out) {
     return MLK_OK;
 }
 
-int prepare_bias(mlk_index* index, int prec, double G, hipStream_t stream, bool may_allocate) {
+int prepare_bias(mlk_index* index, int prec, const Consts& c, hipStream_t stream, bool may_allocate) {
     Mirror& mr = index->mirror[prec];
-    if (mr.bias_h == G) return MLK_OK;
+    if (mr.bias_t1 == c.t1 && mr.bias_t2 == c.t2 && mr.bias_gamma == c.gamma && mr.bias_eps == c.eps) return MLK_OK;
     if (!may_allocate && mr.bias.bytes < (size_t)index->n * sizeof(float)) {
         set_error("the index has no screen bias for this precision yet: one mlk_knn_query call of the same "
                   "precision prepares it");
@@ -314,9 +336,13 @@ int prepare_bias(mlk_index* index, int prec, double G, hipStream_t stream, bool 
     }
     MLK_TRY(mr.bias.reserve((size_t)index->n * sizeof(float)));
     hipLaunchKernelGGL(k_bias, dim3((unsigned)((index->n + 255) / 256)), dim3(256), 0, stream,
-                       mr.nx.as<double>(), mr.flags.as<uint8_t>(), index->n, G, mr.bias.as<float>());
+                       mr.nx.as<double>(), mr.nt.as<double>(), mr.d2.as<double>(), mr.flags.as<uint8_t>(), index->n, c,
+                       mr.bias.as<float>());
     MLK_CHECK_HIP(hipGetLastError());
-    mr.bias_h = G;
+    mr.bias_t1 = c.t1;
+    mr.bias_t2 = c.t2;
+    mr.bias_gamma = c.gamma;
+    mr.bias_eps = c.eps;
     return MLK_OK;
 }
 

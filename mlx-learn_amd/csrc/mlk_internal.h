@@ -115,8 +115,10 @@ struct Mirror {
     DevBuf nx32;            // the same norms rounded to float32 (screen hit path)
     DevBuf flags;           // n uint8: bit0 = safe
     DevBuf unsafe;          // n_unsafe int32 row ids
-    DevBuf bias;            // n float, for `bias_h`
-    double bias_h = -1.0;   // error constant the bias was built for
+    DevBuf nt;              // n float64 squared norms of the rounded rows (screen values)
+    DevBuf d2;              // n float64 squared norms of the rounding residuals
+    DevBuf bias;            // n float: -a_x for the constants below
+    double bias_t1 = -1.0, bias_t2 = -1.0, bias_gamma = -1.0, bias_eps = -1.0;
 };
 
 // Tuning knobs (0 = automatic).
@@ -148,7 +150,7 @@ struct mlk_index {
     mlk::Mirror mirror[3];
     std::mutex lock;
     // per-call workspace
-    mlk::DevBuf ws[24];
+    mlk::DevBuf ws[32];
 };
 
 struct mlk_radius_result {
@@ -166,9 +168,10 @@ int validate_knn(mlk_index* index, const float* Q, int64_t m, int32_t k, int64_t
                  const int64_t* idx);
 // implemented in mlk_index.hip
 int prepare_mirror(mlk_index* index, int prec, hipStream_t stream);
-// may_allocate false: the bias buffer must exist already (a bias for another
-// constant is rebuilt in place); otherwise MLK_EINVAL.
-int prepare_bias(mlk_index* index, int prec, double H, hipStream_t stream, bool may_allocate = true);
+// may_allocate false: the bias buffer must exist already (a bias for other
+// constants is rebuilt in place); otherwise MLK_EINVAL.
+struct Consts;
+int prepare_bias(mlk_index* index, int prec, const Consts& c, hipStream_t stream, bool may_allocate = true);
 // implemented in mlk_knn.hip
 // mem: where the call's temporaries live (nullptr: the index's workspace).
 // With a fixed arena the call allocates nothing, records no events and needs

@@ -43,7 +43,8 @@
 namespace mlk {
 
 int mirror_rows(int prec, const float* X, int64_t n, int64_t n_pad, int d, int dp, const float* c,
-                double scale, void* xs, double* norms, uint8_t* flags, unsigned* inexact, hipStream_t stream);
+                double scale, void* xs, double* norms, uint8_t* flags, unsigned* inexact, double* nt, double* d2,
+                hipStream_t stream);
 
 // status bits per query
 constexpr int ST_UNSAFE = 1;     // query coordinates outside the screen's safe range
@@ -96,7 +97,7 @@ template <int U, int QW>
 __global__ __launch_bounds__(256) void k_tau0(const float* __restrict__ X, int64_t n, int d,
                                               const float* __restrict__ Q, int64_t m, int64_t m_pad, int k,
                                               int max_samples, int64_t self_off, int exclude,
-                                              const double* __restrict__ nq, const uint8_t* __restrict__ qflags,
+                                              const double* __restrict__ qa, const uint8_t* __restrict__ qflags,
                                               Consts c, float* __restrict__ tau, float* __restrict__ thr,
                                               int* __restrict__ status, int S2, int* __restrict__ cnt,
                                               int* __restrict__ seen, float* __restrict__ kbest,
@@ -242,7 +243,7 @@ __global__ __launch_bounds__(256) void k_tau0(const float* __restrict__ X, int64
             const int st = qflags[q] ? 0 : ST_UNSAFE;
             status[q] = st;
             tau[q] = t0;
-            thr[q] = st ? INFINITY : thr_from_tau((double)t0, nq[q], c);
+            thr[q] = st ? INFINITY : thr_from_tau((double)t0, qa[q], c);
         }
     }
 }
@@ -264,8 +265,9 @@ __global__ __launch_bounds__(256) void k_tau0(const float* __restrict__ X, int64
 // next rounds' candidate sets.
 __global__ void k_merge(int64_t m, int k, const int* __restrict__ cnt, int* __restrict__ seen,
                         const int2* __restrict__ cand, const float* __restrict__ nx32, int S2, int cap,
-                        float* __restrict__ kbest, const double* __restrict__ nq, Consts c, float* __restrict__ tau,
-                        float* __restrict__ thr, int* __restrict__ status, int* __restrict__ kbest_row,
+                        float* __restrict__ kbest, const double* __restrict__ nq, const double* __restrict__ qa,
+                        Consts c, float* __restrict__ tau, float* __restrict__ thr, int* __restrict__ status,
+                        int* __restrict__ kbest_row,
                         const float* __restrict__ X, const float* __restrict__ Q, int d) {
     __shared__ uint32_t sU[4][MCAP];
     __shared__ int sR[4][MCAP];
@@ -355,7 +357,7 @@ __global__ void k_merge(int64_t m, int k, const int* __restrict__ cnt, int* __re
         if (lane == 0) {
             const float t_new = fminf(tau[q], __uint_as_float(kb));
             tau[q] = t_new;
-            thr[q] = thr_from_tau((double)t_new, nq[q], c);
+            thr[q] = thr_from_tau((double)t_new, qa[q], c);
         }
         return;
     }
@@ -428,7 +430,7 @@ __global__ void k_merge(int64_t m, int k, const int* __restrict__ cnt, int* __re
     if (lane == 0) {
         const float t_new = fminf(tau[q], kth);
         tau[q] = t_new;
-        thr[q] = thr_from_tau((double)t_new, nq[q], c);
+        thr[q] = thr_from_tau((double)t_new, qa[q], c);
     }
 }
 
@@ -440,8 +442,8 @@ __global__ void k_merge(int64_t m, int k, const int* __restrict__ cnt, int* __re
 template <int KM>
 __global__ void k_merge_lane(int64_t m, int k, const int* __restrict__ cnt, int* __restrict__ seen,
                              const int2* __restrict__ cand, const float* __restrict__ nx32, int S2, int cap,
-                             float* __restrict__ kbest, const double* __restrict__ nq, Consts c,
-                             float* __restrict__ tau, float* __restrict__ thr, int* __restrict__ status) {
+                             float* __restrict__ kbest, const double* __restrict__ nq, const double* __restrict__ qa,
+                             Consts c, float* __restrict__ tau, float* __restrict__ thr, int* __restrict__ status) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= m || status[q]) return;
     float best[KM];
@@ -499,7 +501,7 @@ __global__ void k_merge_lane(int64_t m, int k, const int* __restrict__ cnt, int*
         if (i == k - 1) kth = best[i];
     const float t_new = fminf(tau[q], kth);
     tau[q] = t_new;
-    thr[q] = thr_from_tau((double)t_new, nqq, c);
+    thr[q] = thr_from_tau((double)t_new, qa[q], c);
 }
 
 // ---------------------------------------------------------------- rank
@@ -1166,6 +1168,13 @@ __global__ void k_finish_status(int64_t m, const int* __restrict__ status, int* 
     if (threadIdx.x < 3 && sh[threadIdx.x][0]) atomicAdd(&out[threadIdx.x], sh[threadIdx.x][0]);
 }
 
+// Query part of every screen threshold (screen_qa), once per call.
+__global__ void k_qa(int64_t m_pad, const double* __restrict__ nq, const double* __restrict__ nt,
+                     const double* __restrict__ d2, Consts c, double* __restrict__ qa) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < m_pad) qa[q] = screen_qa(nq[q], nt[q], d2[q], c);
+}
+
 __global__ void k_fill_int(int* __restrict__ out, int64_t count, int value) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < count) out[i] = value;
@@ -1189,15 +1198,22 @@ static double unit_roundoff(int prec) {
     }
 }
 
-Consts make_consts(int prec, int dp, int d, bool exact_inputs, double scale) {
+Consts make_consts(int prec, int dp, int d, bool rows_exact, bool queries_exact, double scale) {
     Consts c;
-    const double eta = exact_inputs ? 0.0 : unit_roundoff(prec);
+    const double eta = rows_exact && queries_exact ? 0.0 : unit_roundoff(prec);
     const double Kp = dp + 1;
     c.gamma = Kp * 0x1p-23 / (1.0 - Kp * 0x1p-23);
     c.H = 2 * eta + eta * eta + c.gamma * (1 + eta) * (1 + eta) + (d + 8) * 0x1p-50;
     c.G = c.H + c.gamma + 0x1p-40;
     c.f0 = (dp + 2) * 0x1p-126;
     c.scale2 = scale * scale;
+    // splitting parameter of the screen bound: half the unit roundoff sits near
+    // the measured ratio |v - v~| / |v~| of rounded rows (about 0.42 of it for
+    // bf16 and fp16 on Gaussian data); 0 where that side's residuals vanish
+    const double t = 0.5 * unit_roundoff(prec);
+    c.t1 = rows_exact ? 0.0 : t;
+    c.t2 = queries_exact ? 0.0 : t;
+    c.eps = (d + 8) * 0x1p-50;
     return c;
 }
 
@@ -1244,7 +1260,7 @@ static KnnLayout knn_layout(const mlk_index* index, int prec, int64_t m, int k, 
     const size_t mp = (size_t)L.m_pad, al = 256;
     auto piece = [&](size_t b) { return (b + al - 1) / al * al; };
     size_t t = 0;
-    t += piece(mp * dp * elem_bytes(prec)) + piece(mp * 8) + piece(mp + 64) + 3 * piece(mp * 4) + piece(64);
+    t += piece(mp * dp * elem_bytes(prec)) + 4 * piece(mp * 8) + piece(mp + 64) + 3 * piece(mp * 4) + piece(64);
     if (!L.all_fallback) {
         t += 2 * piece(mp * L.S2 * 4) + piece(mp * L.S2 * (size_t)L.cap * 8) + piece(mp * k * 4);
         if (L.kbest_rows) t += piece(mp * k * 4);
@@ -1321,6 +1337,12 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
     MLK_TRY(mem.get(0, (size_t)m_pad * dp * eb, &qmirror));     // query mirror
     MLK_TRY(mem.get(1, (size_t)m_pad * sizeof(double), &nq));   // nq
     MLK_TRY(mem.get(2, (size_t)m_pad + 64, &qflags));           // qflags
+    double* qnt;
+    double* qd2;
+    double* qa;
+    MLK_TRY(mem.get(22, (size_t)m_pad * sizeof(double), &qnt));  // |q~|^2
+    MLK_TRY(mem.get(23, (size_t)m_pad * sizeof(double), &qd2));  // |q - q~|^2
+    MLK_TRY(mem.get(24, (size_t)m_pad * sizeof(double), &qa));   // threshold part
     float* tau;
     float* thr;
     int* status;
@@ -1333,7 +1355,7 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
     // 64-bit statistics sums
     MLK_CHECK_HIP(hipMemsetAsync(dflag, 0, 40, stream));
     MLK_TRY(mirror_rows(prec, Q, m, m_pad, d, dp, index->centre.as<float>(), mr.scale, qmirror, nq, qflags, dflag,
-                        stream));
+                        qnt, qd2, stream));
     // the queries' rounding only matters when every database value is exact
     // in the screen type (integer data); otherwise no wait here
     unsigned q_inexact = 1;
@@ -1341,8 +1363,9 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
         MLK_CHECK_HIP(hipMemcpyAsync(&q_inexact, dflag, 4, hipMemcpyDeviceToHost, stream));
         MLK_CHECK_HIP(hipStreamSynchronize(stream));
     }
-    const Consts c = make_consts(prec, dp, d, mr.exact && q_inexact == 0, mr.scale);
-    MLK_TRY(prepare_bias(index, prec, c.G, stream, !fixed));
+    const Consts c = make_consts(prec, dp, d, mr.exact, mr.exact && q_inexact == 0, mr.scale);
+    MLK_TRY(prepare_bias(index, prec, c, stream, !fixed));
+    hipLaunchKernelGGL(k_qa, dim3((unsigned)((m_pad + 255) / 256)), dim3(256), 0, stream, m_pad, nq, qnt, qd2, c, qa);
 
     const float* X = index->X.as<float>();
     // sampled start threshold: 32 rows per neighbour asked for, 64 to 256 rows
@@ -1372,7 +1395,7 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
         auto launch = [&](auto kern, int qw) {
             const int64_t waves = (m_pad + qw - 1) / qw;
             hipLaunchKernelGGL(kern, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, stream, X, n, d, Q, m, m_pad,
-                               k, tau_samples, q_offset, (int)exclude, nq, qflags, c, tau, thr, status, S2,
+                               k, tau_samples, q_offset, (int)exclude, qa, qflags, c, tau, thr, status, S2,
                                all_fallback ? nullptr : cnt, seen, kbest, kbest_row);
         };
         // the most queries per wave (register reuse of the samples) that still
@@ -1444,12 +1467,12 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
         auto merge = [&]() -> int {
             if (lane_merge)
                 hipLaunchKernelGGL(k == 1 ? k_merge_lane<1> : k_merge_lane<8>, dim3((unsigned)((m + 255) / 256)),
-                                   dim3(256), 0, stream, m, k, sa.cnt, seen, sa.cand, sa.nx32, S2, cap, kbest, nq, c,
-                                   tau, thr, status);
+                                   dim3(256), 0, stream, m, k, sa.cnt, seen, sa.cand, sa.nx32, S2, cap, kbest, nq, qa,
+                                   c, tau, thr, status);
             else
                 hipLaunchKernelGGL(k_merge, dim3((unsigned)((m + 3) / 4)), dim3(256), 0, stream, m, k, sa.cnt,
-                                   seen, sa.cand, sa.nx32, S2, cap, kbest, nq, c, tau, thr, status, kbest_row, X, Q,
-                                   d);
+                                   seen, sa.cand, sa.nx32, S2, cap, kbest, nq, qa, c, tau, thr, status, kbest_row, X,
+                                   Q, d);
             MLK_CHECK_HIP(hipGetLastError());
             return MLK_OK;
         };

@@ -28,8 +28,9 @@
 namespace mlk {
 
 int mirror_rows(int prec, const float* X, int64_t n, int64_t n_pad, int d, int dp, const float* c,
-                double scale, void* xs, double* norms, uint8_t* flags, unsigned* inexact, hipStream_t stream);
-Consts make_consts(int prec, int dp, int d, bool exact_inputs, double scale);
+                double scale, void* xs, double* norms, uint8_t* flags, unsigned* inexact, double* nt, double* d2,
+                hipStream_t stream);
+Consts make_consts(int prec, int dp, int d, bool rows_exact, bool queries_exact, double scale);
 
 static void band(int d, double R2, double* f_in, double* f_out) {
     // restates thresholds() of radius_contract.py:129-153
@@ -72,15 +73,16 @@ struct RadiusConsts {
     uint32_t key[SACC_LIMBS];
 };
 
-__global__ void k_radius_thr(int64_t m, int64_t m_pad, const double* __restrict__ nq, const uint8_t* __restrict__ qflags,
-                             double tau, Consts c, float* __restrict__ thr) {
+__global__ void k_radius_thr(int64_t m, int64_t m_pad, const double* __restrict__ nq, const double* __restrict__ nt,
+                             const double* __restrict__ d2, const uint8_t* __restrict__ qflags, double tau, Consts c,
+                             float* __restrict__ thr) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= m_pad) return;
     if (q >= m) { thr[q] = INFINITY; return; }
     // Unsafe query (screen values zero): every safe row is a candidate. A safe
     // row's accumulator is -a_x > -2^120, an unsafe row's is -3e38, so -1e38
     // passes exactly the safe rows; k_unsafe_pairs adds the unsafe ones once.
-    thr[q] = qflags[q] ? thr_from_tau(tau, nq[q], c) : -1e38f;
+    thr[q] = qflags[q] ? thr_from_tau(tau, screen_qa(nq[q], nt[q], d2[q], c), c) : -1e38f;
 }
 
 // Unsafe database rows never hit the screen; they are candidates of every query.
@@ -450,19 +452,21 @@ int radius_query_impl(mlk_index* index, const float* Q, int64_t m, double R2, in
     MLK_TRY(ws[2].reserve((size_t)m_pad + 64));
     MLK_TRY(ws[4].reserve((size_t)m_pad * sizeof(float)));
     MLK_TRY(ws[6].reserve(64));
+    MLK_TRY(ws[25].reserve((size_t)m_pad * sizeof(double)));  // |q~|^2
+    MLK_TRY(ws[26].reserve((size_t)m_pad * sizeof(double)));  // |q - q~|^2
     unsigned* dflag = ws[6].as<unsigned>();
     MLK_CHECK_HIP(hipMemsetAsync(dflag, 0, 16, stream));
     MLK_TRY(mirror_rows(prec, Q, m, m_pad, d, dp, index->centre.as<float>(), mr.scale, ws[0].ptr, ws[1].as<double>(),
-                        ws[2].as<uint8_t>(), dflag, stream));
+                        ws[2].as<uint8_t>(), dflag, ws[25].as<double>(), ws[26].as<double>(), stream));
     unsigned q_inexact = 1;
     MLK_CHECK_HIP(hipMemcpyAsync(&q_inexact, dflag, 4, hipMemcpyDeviceToHost, stream));
     MLK_CHECK_HIP(hipStreamSynchronize(stream));
-    const Consts c = make_consts(prec, dp, d, mr.exact && q_inexact == 0, mr.scale);
-    MLK_TRY(prepare_bias(index, prec, c.G, stream));
+    const Consts c = make_consts(prec, dp, d, mr.exact, q_inexact == 0, mr.scale);
+    MLK_TRY(prepare_bias(index, prec, c, stream));
     const double tau = R2 * c.scale2 * (1.0 + 0x1p-50);
     float* thr = ws[4].as<float>();
     hipLaunchKernelGGL(k_radius_thr, dim3(blocks_for(m_pad)), dim3(256), 0, stream, m, m_pad, ws[1].as<double>(),
-                       ws[2].as<uint8_t>(), tau, c, thr);
+                       ws[25].as<double>(), ws[26].as<double>(), ws[2].as<uint8_t>(), tau, c, thr);
     MLK_CHECK_HIP(hipGetLastError());
 
     // candidate pairs; the buffer grows and the screen reruns on overflow

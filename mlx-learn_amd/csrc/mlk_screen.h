@@ -13,17 +13,67 @@
 namespace mlk {
 
 struct Consts {
-    double H;       // relative error constant of the screen bound
-    double G;       // bias constant (>= H + gamma)
+    double H;       // relative error constant of the stored-hit bounds (hit_bounds)
+    double G;       // H + gamma + 2^-40
     double gamma;   // accumulation constant
     double f0;      // absolute floor (flushed subnormal partial sums)
     double scale2;  // sigma^2 of the screen mirror
+    // screen test (separable rounding bound): splitting parameters of the
+    // query-norm x row-residual and query-residual x row-norm products (0 when
+    // the side whose residual they weigh is exact), and the relative error of
+    // the float64 norms
+    double t1, t2, eps;
 };
 
-__device__ __forceinline__ float thr_from_tau(double tau, double nq, const Consts& c) {
+// The screen's separable bound. For a query q and a row x with screen values
+// q~, x~ (exact), residuals dq = q - q~, dx = x - x~ and accumulator
+// acc = fl32(-a_x + sum q~_i x~_i) (any order, any rounding: gamma, plus f0
+// for flushed subnormals),
+//   |q.x - q~.x~| <= |q~||dx| + |dq||x~| + |dq||dx|   (Cauchy-Schwarz)
+//   |acc - (q~.x~ - a_x)| <= gamma (|a_x| + |q~||x~|) + f0,
+// and 2ab <= t a^2 + b^2 / t splits every product into a query part alpha and a
+// row part beta:
+//   gamma |q~||x~| + |q~||dx| + |dq||x~| + |dq||dx| <= alpha(q) + beta(x),
+//   alpha = (gamma + t1) |q~|^2 / 2 + (1 / t2 + 1) |dq|^2 / 2,
+//   beta  = (gamma + t2) |x~|^2 / 2 + (1 / t1 + 1) |dx|^2 / 2,
+// with the |dq|^2 term dropped when every query residual is zero (t2 = 0) and
+// the |dx|^2 term when every row residual is zero (t1 = 0). The squared norms
+// nt = |v~|^2 and d2 = |v - v~|^2 are float64 sums (relative error <= eps).
+__host__ __device__ __forceinline__ double screen_alpha(double nt, double d2, const Consts& c) {
+    const double T2 = nt * (1.0 + c.eps), D2 = d2 * (1.0 + c.eps);
+    double a = (c.gamma + c.t1) * T2 * 0.5;
+    if (D2 > 0.0) a += c.t2 > 0.0 ? (1.0 / c.t2 + 1.0) * D2 * 0.5 : INFINITY;
+    return a * (1.0 + 0x1p-50);
+}
+__host__ __device__ __forceinline__ double screen_beta(double nt, double d2, const Consts& c) {
+    const double T2 = nt * (1.0 + c.eps), D2 = d2 * (1.0 + c.eps);
+    double b = (c.gamma + c.t2) * T2 * 0.5;
+    if (D2 > 0.0) b += c.t1 > 0.0 ? (1.0 / c.t1 + 1.0) * D2 * 0.5 : INFINITY;
+    return b * (1.0 + 0x1p-50);
+}
+// Query part of the screen threshold: qa <= nq* (1 - eps) - 2 alpha, rounded
+// down (nq = float64 squared norm of the scaled centred query).
+__host__ __device__ __forceinline__ double screen_qa(double nq, double nt, double d2, const Consts& c) {
+    const double al = screen_alpha(nt, d2, c);
+    const double v = nq * (1.0 - c.eps) - 2.0 * al;
+    return v - 0x1p-50 * (fabs(nq) + 2.0 * al);
+}
+// The row's accumulator start -a_x: a_x(1 + gamma) <= (nx* (1 - eps) - 2 beta) / 2
+// (a_x(1 - gamma) when negative), rounded down to float32, so that
+//   sigma^2 S >= 2 (qa / 2 - f0 - acc)   for every row,
+// and acc >= thr = rd((qa - tau) / 2 - f0) holds for every row with sigma^2 S <= tau.
+__host__ __device__ __forceinline__ double screen_row_bound(double nx, double nt, double d2, const Consts& c) {
+    const double be = screen_beta(nt, d2, c);
+    const double v = nx * (1.0 - c.eps) - 2.0 * be;
+    const double lo = v - 0x1p-50 * (fabs(nx) + 2.0 * be);
+    return lo >= 0.0 ? lo * 0.5 / (1.0 + c.gamma) * (1.0 - 0x1p-50) : lo * 0.5 / (1.0 - c.gamma) * (1.0 + 0x1p-50);
+}
+
+// Screen threshold of a query from its tau (an upper bound of its k-th smallest
+// sigma^2 S) and its qa: acc >= thr <=> the row may lie within tau.
+__device__ __forceinline__ float thr_from_tau(double tau, double qa, const Consts& c) {
     if (!(tau < INFINITY)) return -INFINITY;
-    const double H = c.H + 0x1p-46;
-    return __double2float_rd(((1.0 - H) * nq - tau * (1.0 + 0x1p-46)) * 0.5 - c.f0);
+    return __double2float_rd((qa - tau * (1.0 + 0x1p-46)) * 0.5 - c.f0);
 }
 
 // Certified bounds of S (scaled units) from a stored hit: p = acc + a_x and the

@@ -118,7 +118,7 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
     constexpr int LPW = (NLOAD + NW - 1) / NW;
     constexpr int BQ = NW * QT * 32;
     static_assert(NS >= 2 && NS <= 4, "stages");
-    static_assert(QT == 1 || QT == 2, "query tiles");
+    static_assert(QT == 1 || QT == 2 || QT == 4, "query tiles");
     static_assert(RTW == 1 || RTW == 2, "row tiles per step");
     static_assert(!CH || NS == 2, "deep rows: two stages (each step waits for its query reload)");
     static_assert(NS * (ABYTES + 512) + (MODE == SCREEN_PAIRS ? NW * PAIR_STAGE * 8 : 0) <= 160 * 1024,
@@ -178,19 +178,22 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
     // k-NN mode: each lane owns the candidate sub-list (query col, split, lane
     // half h), so appends need no atomics; counts live in registers.
     int cnt_r[QT];
-    size_t list_base[QT];
     if constexpr (MODE == SCREEN_LISTS) {
 #pragma unroll
         for (int qt = 0; qt < QT; ++qt) {
             const size_t q = (size_t)(qrow0 + qt * 32 + col);
             const size_t sub = (q * a.S + split) * 2 + h;
             cnt_r[qt] = a.cnt[sub];
-            // entry j of sub-list s of query q lives at (q * cap + j) * 2S + s: the j-th
-            // entries of a query's sub-lists are adjacent, so the list walks that visit
-            // every sub-list of a query read them with coalesced loads
-            list_base[qt] = q * a.cap * (size_t)(2 * a.S) + (size_t)split * 2 + h;
         }
     }
+    // entry j of sub-list s of query q lives at (q * cap + j) * 2S + s: the j-th
+    // entries of a query's sub-lists are adjacent, so the list walks that visit
+    // every sub-list of a query read them with coalesced loads (formed on the
+    // hit path only, so no registers hold it across the walk)
+    auto list_base = [&](int qt) -> size_t {
+        const size_t q = (size_t)(qrow0 + qt * 32 + col);
+        return q * a.cap * (size_t)(2 * a.S) + (size_t)split * 2 + h;
+    };
     bool live = false;
 #pragma unroll
     for (int qt = 0; qt < QT; ++qt) live |= thr[qt] < INFINITY;
@@ -406,7 +409,7 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
                             if (v >= thr[qt] && row < r_end && row != self) {
                                 const int slot = cnt_r[qt]++;
                                 if (slot < a.cap)
-                                    a.cand[list_base[qt] + (size_t)slot * (2 * a.S)] =
+                                    a.cand[list_base(qt) + (size_t)slot * (2 * a.S)] =
                                         int2{(int)row, __float_as_int(v - cB[rl])};
                             }
                         }
@@ -471,6 +474,227 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
 #endif
 }
 
+// Software-pipelined screen for k-NN lists on rows of at most 128 features
+// (bf16 / fp16): 32-row tiles, QT query tiles per wave (4 * QT * 32 queries
+// per block), a three-stage LDS tile ring. Step u issues the MFMAs of tile u
+// into one accumulator set and, in the same instruction stream, the
+// threshold test (max-tree and compare) of tile u - 1 held in the other set,
+// so the test's VALU work issues in the shadow of the running MFMAs instead
+// of after them; the A fragments of each tile are read from LDS four ahead of
+// their MFMAs. Only a wave with a hit leaves the straight-line path (the
+// append, as in k_screen, with the row's bias read from memory because its
+// LDS slot may already hold a later tile).
+template <int PREC, int DP, int QT>
+__global__ __launch_bounds__(256) void k_screen_pp(const ScreenArgs a) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    static_assert(PREC != PREC_FP32, "bf16 / fp16 operands");
+    static_assert(DP % 32 == 0 && DP <= 128, "narrow rows");
+    constexpr int NW = 4, NS = 3, BN = 32;
+    constexpr int RB = DP * 2;                // bytes per database row
+    constexpr int CPR = RB / 16;              // 16-byte chunks per row
+    constexpr int RS = RB + 16;               // LDS row stride: one pad chunk per row
+    constexpr int NCH = BN * (CPR + 1);
+    constexpr int NLOAD = (NCH + 63) / 64;    // 1 KiB load instructions per tile
+    constexpr int ABYTES = NLOAD * 1024;
+    constexpr int LPW = (NLOAD + NW - 1) / NW;
+    constexpr int BQ = NW * QT * 32;
+    constexpr int NB = DP / 16;               // k-steps (A fragments) per tile
+    constexpr int PF = NB < 4 ? NB : 4;       // A fragments read ahead
+    static_assert(NS * (ABYTES + 256) <= 160 * 1024, "LDS of one block");
+    // distinct buffers with compile-time selection, so the compiler can tell the
+    // tile being read from the tiles in flight and leaves those loads running
+    __shared__ __attribute__((aligned(1024))) char sA0[ABYTES];
+    __shared__ __attribute__((aligned(1024))) char sA1[ABYTES];
+    __shared__ __attribute__((aligned(1024))) char sA2[ABYTES];
+    __shared__ __attribute__((aligned(256))) float sB0[64];  // [0, 32): -a_x of the tile rows
+    __shared__ __attribute__((aligned(256))) float sB1[64];
+    __shared__ __attribute__((aligned(256))) float sB2[64];
+    auto bufA = [&](auto I) -> char* {
+        constexpr int i = decltype(I)::value;
+        if constexpr (i == 0) return sA0; else if constexpr (i == 1) return sA1; else return sA2;
+    };
+    auto bufB = [&](auto I) -> float* {
+        constexpr int i = decltype(I)::value;
+        if constexpr (i == 0) return sB0; else if constexpr (i == 1) return sB1; else return sB2;
+    };
+
+    int qb, split;
+    block_item(a, qb, split);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, h = lane >> 5, col = lane & 31;
+    const int64_t s_lo = (int64_t)split * a.split_rows;
+    const int64_t r_begin = s_lo + a.off_lo;
+    int64_t r_end = s_lo + (a.off_hi < a.split_rows ? a.off_hi : a.split_rows);
+    if (r_end > a.n) r_end = a.n;
+    if (r_begin >= r_end) return;
+
+    const int64_t qrow0 = (int64_t)qb * BQ + wave * QT * 32;
+    float thr[QT];
+    int cnt_r[QT];
+#pragma unroll
+    for (int qt = 0; qt < QT; ++qt) {
+        const size_t q = (size_t)(qrow0 + qt * 32 + col);
+        thr[qt] = a.thr[q];
+        cnt_r[qt] = a.cnt[(q * a.S + split) * 2 + h];
+    }
+    bool live = false;
+#pragma unroll
+    for (int qt = 0; qt < QT; ++qt) live |= thr[qt] < INFINITY;
+    const bool wave_live = ballot(live) != 0;
+
+    // query fragments: lane (col, h) holds features h * DP/2 + [8j, 8j + 8) of query col
+    i32x4 bq[QT][NB];
+#pragma unroll
+    for (int qt = 0; qt < QT; ++qt) {
+        const __bf16* qp = (const __bf16*)a.qs + (size_t)(qrow0 + qt * 32 + col) * DP + h * (DP / 2);
+#pragma unroll
+        for (int j = 0; j < NB; ++j) bq[qt][j] = *(const i32x4*)(qp + 8 * j);
+    }
+
+    const int64_t ntiles = (r_end - r_begin + BN - 1) / BN;
+    auto chunk_off = [&](int i) -> uint32_t {
+        const int L = (wave + i * NW) * 64 + lane;
+        const int row = L / (CPR + 1), pos = L % (CPR + 1);
+        return (pos < CPR && row < BN) ? (uint32_t)row * RB + (uint32_t)(pos * 16) : 0x7ffffff0u;
+    };
+    uint32_t voff[LPW];
+#pragma unroll
+    for (int i = 0; i < LPW; ++i) voff[i] = chunk_off(i);
+    const int my_loads = (NLOAD > wave ? (NLOAD - wave + NW - 1) / NW : 0) + (wave == 1);
+    auto load_tile = [&](int64_t t, char* dstA, float* dstB) {
+        const int64_t r0 = r_begin + t * BN;
+        const int64_t rows = r_end - r0 < BN ? r_end - r0 : BN;
+        const __amdgpu_buffer_rsrc_t ra = buffer_rsrc((const char*)a.xs + (size_t)r0 * RB, (uint32_t)rows * RB);
+#pragma unroll
+        for (int i = 0; i < LPW; ++i)
+            if (wave + i * NW < NLOAD)
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(ra, (lds_char*)(dstA + (wave + i * NW) * 1024), 16, voff[i],
+                                                         0, 0, 0);
+        if (wave == 1) {
+            const __amdgpu_buffer_rsrc_t rb = buffer_rsrc(a.bias + r0, (uint32_t)(rows * 4));
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rb, (lds_char*)dstB, 4, lane * 4, 0, 0, 0);
+        }
+    };
+    auto sync_tiles = [&](int stages) {
+        switch (stages * my_loads) {
+            case 0: wait_vm_lgkm0<0>(); break;
+#define MLK_WAIT_CASE(n) case n: wait_vm_lgkm0<n>(); break;
+            MLK_WAIT_CASE(1) MLK_WAIT_CASE(2) MLK_WAIT_CASE(3) MLK_WAIT_CASE(4) MLK_WAIT_CASE(5)
+            MLK_WAIT_CASE(6) MLK_WAIT_CASE(7) MLK_WAIT_CASE(8)
+#undef MLK_WAIT_CASE
+            default: wait_vm_lgkm0<0>(); break;
+        }
+        tile_barrier();
+    };
+
+    f32x16 accA[QT], accB[QT];
+    // MFMAs of tile t (LDS buffer buf) into acc
+    auto mfma_tile = [&](f32x16(&acc)[QT], const char* tA, const float* tB) {
+        const char* arow = tA + col * RS + h * (CPR / 2) * 16;
+        const float* nb = tB + (h << 2);
+        const f32x4 b0 = *(const f32x4*)(nb), b1 = *(const f32x4*)(nb + 8);
+        const f32x4 b2 = *(const f32x4*)(nb + 16), b3 = *(const f32x4*)(nb + 24);
+        i32x4 av[NB];
+#pragma unroll
+        for (int j = 0; j < PF; ++j) av[j] = *(const i32x4*)(arow + j * 16);
+        const f32x16 init = {b0[0], b0[1], b0[2], b0[3], b1[0], b1[1], b1[2], b1[3],
+                             b2[0], b2[1], b2[2], b2[3], b3[0], b3[1], b3[2], b3[3]};
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            if (j + PF < NB) av[j + PF] = *(const i32x4*)(arow + (j + PF) * 16);
+#pragma unroll
+            for (int qt = 0; qt < QT; ++qt) acc[qt] = mfma16<PREC>(av[j], bq[qt][j], j == 0 ? init : acc[qt]);
+        }
+    };
+    // hit flags of a finished tile (the branch-free part of the test)
+    auto test_tile = [&](const f32x16(&acc)[QT], bool(&hit)[QT]) {
+        bool any = false;
+#pragma unroll
+        for (int qt = 0; qt < QT; ++qt) {
+            hit[qt] = max16(acc[qt]) >= thr[qt];
+            any |= hit[qt];
+        }
+        return any;
+    };
+    // appends of a tile whose test found a hit in some lane
+    auto append_tile = [&](const f32x16(&acc)[QT], const bool(&hit)[QT], int64_t r0) {
+#pragma unroll
+        for (int qt = 0; qt < QT; ++qt) {
+            if (!ballot(hit[qt])) continue;
+            const int64_t qg = qrow0 + qt * 32 + col;
+            const int64_t self = a.exclude ? a.self_off + qg : -1;
+            const size_t base = (size_t)qg * a.cap * (size_t)(2 * a.S) + (size_t)split * 2 + h;
+            uint32_t regs = 0;
+#pragma unroll
+            for (int i = 0; i < 16; ++i) regs |= (ballot(acc[qt][i] >= thr[qt]) != 0 ? 1u : 0u) << i;
+#pragma unroll 1
+            while (regs) {
+                const int i = __builtin_ctz(regs);
+                regs &= regs - 1;
+                const float v = acc[qt][i];
+                const int64_t row = r0 + mfma32_row(i, lane);
+                if (v >= thr[qt] && row < r_end && row != self) {
+                    const int slot = cnt_r[qt]++;
+                    if (slot < a.cap) a.cand[base + (size_t)slot * (2 * a.S)] = int2{(int)row, __float_as_int(v - a.bias[row])};
+                }
+            }
+        }
+    };
+
+    // prologue: tiles 0 and 1 in flight, wait for tile 0; the "previous tile"
+    // of step 0 is an all -inf accumulator set, which no threshold reaches, so
+    // every step runs the same straight-line code
+    load_tile(0, sA0, sB0);
+    if (ntiles > 1) load_tile(1, sA1, sB1);
+    sync_tiles(ntiles > 1 ? 1 : 0);
+#pragma unroll
+    for (int qt = 0; qt < QT; ++qt)
+#pragma unroll
+        for (int i = 0; i < 16; ++i) accB[qt][i] = -INFINITY;
+    // one step: prefetch tile u + 2 into the buffer of tile u - 1, MFMAs of tile
+    // u into CUR, test of tile u - 1 in PREV
+    auto step = [&](auto BUF, f32x16(&cur)[QT], f32x16(&prev)[QT], int64_t u) {
+        constexpr int b = decltype(BUF)::value;
+        const bool full = u + 2 < ntiles;
+        if (full) load_tile(u + 2, bufA(std::integral_constant<int, (b + 2) % NS>{}),
+                            bufB(std::integral_constant<int, (b + 2) % NS>{}));
+        bool stored = false;
+        if (wave_live) {
+            mfma_tile(cur, bufA(BUF), bufB(BUF));
+            bool hit[QT];
+            if (ballot(test_tile(prev, hit))) {
+                append_tile(prev, hit, r_begin + (u - 1) * BN);
+                stored = true;
+            }
+        }
+        // stores also count in vmcnt: after one, wait for everything
+        sync_tiles(full && !stored ? 1 : 0);
+    };
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
+    using I2 = std::integral_constant<int, 2>;
+    // buffer u % 3, accumulator set u % 2: six steps per pass of the loop
+    for (int64_t u = 0; u < ntiles; u += 6) {
+        step(I0{}, accA, accB, u);
+        if (u + 1 < ntiles) step(I1{}, accB, accA, u + 1);
+        if (u + 2 < ntiles) step(I2{}, accA, accB, u + 2);
+        if (u + 3 < ntiles) step(I0{}, accB, accA, u + 3);
+        if (u + 4 < ntiles) step(I1{}, accA, accB, u + 4);
+        if (u + 5 < ntiles) step(I2{}, accB, accA, u + 5);
+    }
+    // the last tile's test
+    if (wave_live) {
+        const f32x16(&last)[QT] = ((ntiles - 1) & 1) ? accB : accA;
+        bool hit[QT];
+        if (ballot(test_tile(last, hit))) append_tile(last, hit, r_begin + (ntiles - 1) * BN);
+    }
+#pragma unroll
+    for (int qt = 0; qt < QT; ++qt)
+        a.cnt[(((size_t)(qrow0 + qt * 32 + col)) * a.S + split) * 2 + h] = cnt_r[qt];
+#else
+    (void)a;
+#endif
+}
+
 template <int PREC, int QT, int NS, int MODE>
 static ScreenFn pick_dp(int dp) {
     constexpr int RTW = QT == 1 ? 2 : 1;
@@ -479,6 +703,17 @@ static ScreenFn pick_dp(int dp) {
         case 64: return k_screen<PREC, 64, QT, RTW, NS, MODE>;
         case 96: return k_screen<PREC, 96, QT, RTW, NS, MODE>;
         case 128: return k_screen<PREC, 128, QT, RTW, NS, MODE>;
+        default: return nullptr;
+    }
+}
+
+template <int PREC, int QT>
+static ScreenFn pick_pp(int dp) {
+    switch (dp) {
+        case 32: return k_screen_pp<PREC, 32, QT>;
+        case 64: return k_screen_pp<PREC, 64, QT>;
+        case 96: return k_screen_pp<PREC, 96, QT>;
+        case 128: return k_screen_pp<PREC, 128, QT>;
         default: return nullptr;
     }
 }
@@ -523,8 +758,13 @@ static ScreenFn pick_shape(int dp, int qt, int ns) {
     } else {
         if (qt == 1 && ns == 2) return pick_dp<PREC, 1, 2, MODE>(dp);
         if constexpr (MODE == SCREEN_LISTS) {
+            // four query tiles per wave: every LDS fragment feeds four MFMAs and a
+            // block's 512 queries share each database tile (two waves per SIMD)
+            if (qt == 4 && ns == 3) return pick_dp<PREC, 4, 3, MODE>(dp);
+            if (qt == 4 && ns == 2) return pick_dp<PREC, 4, 2, MODE>(dp);
             if (qt == 2 && ns == 4) return pick_dp<PREC, 2, 4, MODE>(dp);
-            if (qt == 2 && ns == 3) return pick_dp<PREC, 2, 3, MODE>(dp);
+            // three stages, two query tiles: the software-pipelined kernel
+            if (qt == 2 && ns == 3) return pick_pp<PREC, 2>(dp);
             if (qt == 2 && ns == 2) return pick_dp<PREC, 2, 2, MODE>(dp);
         }
         return nullptr;
@@ -563,7 +803,7 @@ std::vector<ScreenShape> screen_shapes(int mode, int prec, int dp) {
         if (pick_screen(mode, prec, dp, default_shape(prec, dp))) out.push_back(default_shape(prec, dp));
         return out;
     }
-    for (int qt = 1; qt <= 2; ++qt)
+    for (int qt : {1, 2, 4})
         for (int ns = 2; ns <= 4; ++ns)
             if (pick_screen(mode, prec, dp, ScreenShape{qt, ns})) out.push_back(ScreenShape{qt, ns});
     return out;
