@@ -104,6 +104,19 @@ def knn_step(index, Qd, k, precision):
     return step
 
 
+def knn_graph_step(index, Qd, k, precision):
+    """The same call replayed from a captured HIP graph (fixed query and output buffers)."""
+    import torch
+    from mlxlearn_amd import _lib
+    out = (torch.empty((Qd.shape[0], k), dtype=torch.float64, device="cuda"),
+           torch.empty((Qd.shape[0], k), dtype=torch.int64, device="cuda"))
+
+    def step():
+        d, i = index.query(Qd, k, precision=precision, graph=True, out=out)
+        return d, i, _lib.stats()
+    return step
+
+
 def run_c1(args):
     import torch
     import mlxlearn_amd
@@ -120,16 +133,26 @@ def run_c1(args):
         spread[prec]["first_call_ms"] = first
         res[prec] = (t, d.cpu().numpy(), i.cpu().numpy(), st)
         log(f"c1 {prec}: {t * 1e3:.3f} ms/step, stats {st}")
+    # the eager call's screen time (device events) for the roofline
+    st_eager = res["float32"][3]
+    # serving-loop form: the fp32 call captured once in a HIP graph, then replayed
+    gstep = knn_graph_step(index, Qd, 5, "float32")
+    tg, (dg, ig, _), spread["float32_graph"] = timed(gstep, max(args.steps, 20), max(args.warmup, 2))
+    log(f"c1 float32 graph replay: {tg * 1e3:.3f} ms/step")
     wd, wi = oracle.knn(X, Q, 5)
     t, d, i, st = res["float32"]
     match = float((i == wi).all(axis=1).mean())
+    match_graph = float((ig.cpu().numpy() == wi).all(axis=1).mean())
     match_bf = float((res["bfloat16"][2] == wi).all(axis=1).mean())
     flops = 2.0 * Q.shape[0] * X.shape[0] * X.shape[1]
-    line = base_line(args, "exact kNN queries/s 10k x 64 k=5 (plain fp32 path)", Q.shape[0] / t, "queries/s",
-                     t * 1e3, "fp32", "synthetic (seeded N(0,1))",
+    line = base_line(args, "exact kNN queries/s 10k x 64 k=5 (plain fp32 path)", Q.shape[0] / tg, "queries/s",
+                     tg * 1e3, "fp32", "synthetic (seeded N(0,1))",
                      {"workload": "c1: NearestNeighbors(k=5), database 10,000x64 float32, 1,000 queries, fp32 MFMA "
-                                  "screen + fp64 exact rerank", "n": 10000, "d": 64, "queries": 1000, "k": 5})
-    line["roofline"] = roofline(flops, st["screen_ms"], "float32", "k_screen<fp32,64>", st["rounds"])
+                                  "screen + fp64 exact rerank, call replayed from a captured HIP graph",
+                      "n": 10000, "d": 64, "queries": 1000, "k": 5})
+    line["roofline"] = roofline(flops, st_eager["screen_ms"], "float32", "k_screen<fp32,64>", st_eager["rounds"])
+    line["eager_path"] = {"value": round(Q.shape[0] / t, 1), "ms_per_step": round(t * 1e3, 3),
+                          "index_match_graph_vs_oracle": match_graph}
     line["step_ms"] = spread
     line["autotune"] = tuned or None
     line["bf16_path"] = {"value": round(Q.shape[0] / res["bfloat16"][0], 1),
@@ -200,7 +223,14 @@ def run_c3(args):
                       "n": X.shape[0], "d": X.shape[1], "queries": m, "k": k})
     line["roofline"] = roofline(flops, max(st["screen_ms"], 1e-9), "float16", "k_screen<fp16,784>",
                                 st["rounds"]) if st["rounds"] else None
-    line["fp32_path"] = {"value": round(m / res["float32"][0], 1), "ms_per_step": round(res["float32"][0] * 1e3, 3)}
+    t32, i32, p32, st32 = res["float32"]
+    line["fp32_path"] = {"value": round(m / t32, 1), "ms_per_step": round(t32 * 1e3, 3),
+                         "screen": {0x00: "fp32", 0x10: "bf16", 0x20: "fp16"}[int(st32["screen_precision"])],
+                         "roofline": roofline(flops, max(st32["screen_ms"], 1e-9), "float32", "k_screen<fp32,896>",
+                                              st32["rounds"]) if st32["rounds"] else None,
+                         "index_match_vs_oracle": float((i32[:sample] == wi).all(axis=1).mean()),
+                         "pred_vs_oracle": float((p32[:sample] == want_pred).mean()),
+                         "pred_vs_fp16": float((p32 == p).mean())}
     line["step_ms"] = spread
     line["autotune"] = tuned or None
     line["index_match"] = {"fp16_vs_oracle": float((i[:sample] == wi).all(axis=1).mean()),
@@ -230,13 +260,23 @@ def run_c4(args):
     Qd = torch.from_numpy(Q).cuda()
     offsets = np.concatenate([[0], np.cumsum(sizes)])
 
-    def step():
-        # every batch in one call: the CSR row pointers split it per batch
-        indptr, indices, dist = index.radius(Qd, R2, sort_results=True, precision="float32")
-        return indptr, indices, dist, _lib.stats()
+    def step_for(prec):
+        def step():
+            # every batch in one call: the CSR row pointers split it per batch
+            indptr, indices, dist = index.radius(Qd, R2, sort_results=True, precision=prec)
+            return indptr, indices, dist, _lib.stats()
+        return step
 
+    # the float32 screen of the float32 request, timed beside the bf16 screen
+    # (identical CSR: the screen precision changes only the candidate pairs)
+    first32 = cold(step_for("float32"))
+    t32, (ip32, ind32, dist32, st32), spread32 = timed(step_for("float32"), max(2, args.steps // 2), 1)
+    spread32["first_call_ms"] = first32
+    ip32, ind32 = ip32.cpu().numpy(), ind32.cpu().numpy()
+    del dist32
+    step = step_for("bfloat16")
     first = cold(step)
-    tuned = tune(index, Qd, radius_sq=R2, sort_results=True, max_trials=10) if args.autotune else None
+    tuned = tune(index, Qd, radius_sq=R2, sort_results=True, precision="bfloat16", max_trials=10) if args.autotune else None
     t, (indptr, indices, dist, st), spread = timed(step, args.steps, args.warmup)
     spread["first_call_ms"] = first
     indptr = indptr.cpu().numpy()
@@ -252,28 +292,40 @@ def run_c4(args):
         ok += int(np.array_equal(got_i, wi) and np.allclose(got_d, wd, rtol=1e-12, atol=0))
     m = Q.shape[0]
     flops = 2.0 * m * X.shape[0] * X.shape[1]
-    # the library screens float32 radius requests in bf16 (results do not depend on it)
     sp = {0x00: ("float32", "fp32"), 0x10: ("bfloat16", "bf16"), 0x20: ("float16", "fp16")}[int(st["screen_precision"])]
     line = base_line(args, "radius_neighbors queries/s 2M x 32, 256 batches, sorted CSR", m / t, "queries/s", t * 1e3,
                      sp[1], "synthetic (seeded uniform cube + 20% Gaussian clumps, calibrated radius)",
                      {"workload": f"c4: radius_neighbors(r={radius}, sort_results=True), database 2,000,000x32 float32, "
-                                  f"{len(sizes)} batches ({m} queries) in one call, {sp[1]} MFMA screen of a float32 "
-                                  "request + exact float64 band",
+                                  f"{len(sizes)} batches ({m} queries) in one call, {sp[1]} MFMA screen of the float32 "
+                                  "queries + exact float64 band",
                       "n": X.shape[0], "d": X.shape[1], "queries": m, "batches": len(sizes)})
     line["roofline"] = roofline(flops, st["screen_ms"], sp[0], f"k_screen<{sp[1]},32,pairs>", 1)
+    line["fp32_path"] = {"value": round(m / t32, 1), "ms_per_step": round(t32 * 1e3, 3),
+                         "screen": {0x00: "fp32", 0x10: "bf16", 0x20: "fp16"}[int(st32["screen_precision"])],
+                         "roofline": roofline(flops, st32["screen_ms"], "float32", "k_screen<fp32,32,pairs>", 1),
+                         "candidates": st32["candidates"], "step_ms": spread32,
+                         "csr_equal_to_bf16": bool(np.array_equal(ip32, indptr) and np.array_equal(ind32, ind_h))}
     line["step_ms"] = spread
     line["autotune"] = tuned
     line["result"] = {"nnz": int(indptr[-1]), "median_per_query": float(np.median(np.diff(indptr))),
                       "oracle_rows_checked": len(picks), "oracle_rows_equal": ok, "candidates": st["candidates"]}
     if not args.no_cpu:
-        nq = min(64, m)
-        t0 = time.perf_counter()
-        oracle.radius(X, Q[:nq], R2, sort_results=True)
-        tc = time.perf_counter() - t0
-        line["cpu_baseline"] = {"value": round(nq / tc, 1), "unit": "queries/s",
-                                "cores": oracle.default_threads(), "kind": "port",
-                                "sample": f"{nq} queries x 2,000,000 rows (C restatement of the canonical float64 "
-                                          "brute force with the exact band, all cores)"}
+        # the reference's own radius engine (brute_query, compiled from its sources
+        # into oracle/_ref) on a bounded sample of the queries, every host core
+        nq = min(256, m)
+        cores = oracle.default_threads()
+        if oracle.reference_radius_engine() is not None:
+            t0 = time.perf_counter()
+            oracle.reference_radius(X, Q[:nq], R2, sort_results=True, threads=cores)
+            tc = time.perf_counter() - t0
+            kind, what = "reference", "the reference's brute_query (_radius_cpu_native.cpp:219-310)"
+        else:
+            t0 = time.perf_counter()
+            oracle.radius(X, Q[:nq], R2, sort_results=True)
+            tc = time.perf_counter() - t0
+            kind, what = "port", "C restatement of the canonical float64 brute force with the exact band"
+        line["cpu_baseline"] = {"value": round(nq / tc, 1), "unit": "queries/s", "cores": cores, "kind": kind,
+                                "sample": f"{nq} queries x 2,000,000 rows, sorted CSR ({what}, {cores} threads)"}
     return line
 
 
@@ -347,13 +399,17 @@ def run_c5(args):
 def finish_c5(args, line, X, C, kc):
     import oracle
     if not args.no_cpu:
-        nr = 256
+        # the reference's assignment on the CPU: float32 Gram argmin (BLAS sgemm on
+        # every host core) + the float64 check, on a bounded sample of rows
+        port = oracle.KMeansPort(C)
+        nr = 8192
         t0 = time.perf_counter()
-        oracle.kmeans_assign(X[:nr], C)
+        port.assign(X[:nr])
         tc = time.perf_counter() - t0
-        line["cpu_baseline"] = {"value": round(nr / tc, 1), "unit": "rows/s", "cores": oracle.default_threads(),
-                                "kind": "port", "sample": f"{nr} rows x {kc} centroids (C exact nearest-centre "
-                                                          "restatement, all cores; update excluded)"}
+        line["cpu_baseline"] = {"value": round(nr / tc, 1), "unit": "rows/s", "cores": len(os.sched_getaffinity(0)),
+                                "kind": "port", "sample": f"{nr} rows x {kc} centroids (numpy float32 Gram argmin "
+                                                          "+ float64 check, e_means.py:50-69 restated; update "
+                                                          "excluded)"}
     return line
 
 

@@ -46,6 +46,10 @@ typedef struct mlk_radius_result mlk_radius_result;
 #define MLK_SORT_RESULTS 0x100u   /* radius: order rows by (distance, index)              */
 #define MLK_NEED_DISTANCE 0x200u  /* radius: keep distances                               */
 #define MLK_SQUARED_DIST 0x400u   /* k-NN: dist_dev receives squared distances s           */
+#define MLK_GRAPH_REPLAY 0x800u   /* k-NN: capture the call's device chain in a HIP graph and
+                                     replay it on later calls with the same arguments and
+                                     buffers (one launch, one read-back per call; no device
+                                     times in mlk_get_stats)                               */
 
 /* weights for mlk_knn_votes */
 #define MLK_WEIGHTS_UNIFORM 0

@@ -163,6 +163,7 @@ int mlk_index_create(const float* X_dev, int64_t n, int32_t d, uint32_t flags, v
 int mlk_index_destroy(mlk_index* index) {
     if (!index) return MLK_OK;
     (void)hipDeviceSynchronize();
+    destroy_graphs(index);
     delete index;
     return MLK_OK;
 }

@@ -22,8 +22,12 @@ namespace mlk {
 
 int DevBuf::reserve(size_t need) {
     if (need <= bytes && ptr) return MLK_OK;
+    // a buffer that has to grow gets an eighth more than asked, so calls of
+    // slightly different sizes do not free and allocate it again each time
+    const bool regrow = ptr != nullptr;
     release();
     size_t want = need < 256 ? 256 : need;
+    if (regrow) want += want / 8;
     hipError_t e = hipMalloc(&ptr, want);
     if (e != hipSuccess) {
         ptr = nullptr;

@@ -151,6 +151,8 @@ struct mlk_index {
     std::mutex lock;
     // per-call workspace
     mlk::DevBuf ws[32];
+    // captured k-NN call graphs (MLK_GRAPH_REPLAY), owned by mlk_knn.hip
+    void* graphs = nullptr;
 };
 
 struct mlk_radius_result {
@@ -184,6 +186,7 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
 size_t knn_arena_bytes(const mlk_index* index, int prec, int64_t m, int k, uint32_t flags, size_t list_budget);
 // Device bytes of the exact fallback's buffers for one query against n rows.
 size_t knn_fallback_bytes_per_query(int64_t n);
+void destroy_graphs(mlk_index* index);
 // implemented in mlk_tune.hip
 int autotune_impl(mlk_index* index, const float* Q, int64_t m, int k, double R2, uint32_t flags, int max_trials,
                   mlk_plan* out, hipStream_t stream);

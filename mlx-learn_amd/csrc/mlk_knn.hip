@@ -1272,6 +1272,14 @@ static KnnLayout knn_layout(const mlk_index* index, int prec, int64_t m, int k, 
     return L;
 }
 
+// Bytes of the candidate lists of a call (0 if it screens nothing).
+static size_t knn_list_bytes(const mlk_index* index, int prec, int64_t m, int k) {
+    const KnnLayout L = knn_layout(index, prec, m, k, hooks_snapshot(), 0);
+    return L.all_fallback ? 0 : (size_t)L.m_pad * L.S2 * (size_t)L.cap * sizeof(int2);
+}
+constexpr size_t KNN_LIST_BUDGET = size_t(4) << 30;  // candidate-list bytes of one chunk of queries
+constexpr int64_t KNN_CHUNK_MIN = 65536;              // smallest chunk of a split call
+
 size_t knn_fallback_bytes_per_query(int64_t n) {
     // float64 distances, candidate rows, their float64 values, and the radix state
     return (size_t)n * (sizeof(double) * 2 + sizeof(int)) + sizeof(int) + sizeof(double) +
@@ -1284,29 +1292,45 @@ size_t knn_arena_bytes(const mlk_index* index, int prec, int64_t m, int k, uint3
     return knn_layout(index, prec, m, k, hooks_snapshot(), list_budget).bytes_main;
 }
 
-int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q_offset, uint32_t flags,
-                   double* dist, int64_t* idx, hipStream_t stream, CallMem* mem_in) {
+// One k-NN call between its enqueued device chain and its completion.
+struct KnnCall {
+    int64_t m = 0, n = 0, q_offset = 0;
+    int d = 0, k = 0, prec = 0;
+    uint32_t flags = 0;
+    bool exclude = false;
+    const float* Q = nullptr;
+    const float* X = nullptr;
+    double* dist = nullptr;
+    int64_t* idx = nullptr;
+    int* status = nullptr;
+    int* fq = nullptr;
+    unsigned* dflag = nullptr;
+    bool big_rank = false;  // k_rank ran: queries marked ST_BIG are ranked again
+    RankArgs ra{};
+    Consts c{};
+    int64_t rounds = 0, splits = 0, capacity = 0;
+};
+
+// Enqueue a call's device chain on `stream`, up to the per-query status
+// (k_finish_status). `capture`: the stream is being captured into a graph, so
+// nothing may wait for the device (the queries' exactness is not read back:
+// the screen then assumes inexact queries, which only loosens its bound), and
+// no event is recorded. `events` (optional) times the call and its screens.
+static int knn_enqueue(mlk_index* index, const float* Q, int64_t m, int k, int64_t q_offset, uint32_t flags,
+                       double* dist, int64_t* idx, hipStream_t stream, CallMem& mem, bool capture, KnnCall& call,
+                       Events* events, std::vector<hipEvent_t>* round_events) {
     mlk_stats& st = stats();
-    st = mlk_stats{};
-    st.queries = m;
-    if (m == 0) return MLK_OK;
-    CallMem own;
-    own.ws = index->ws;
-    CallMem& mem = mem_in ? *mem_in : own;
-    const bool fixed = mem.fixed();  // no allocation, no events
+    const bool fixed = mem.fixed();  // no allocation
+    const bool timed = events != nullptr;
     const int64_t n = index->n;
     const int d = index->d;
     const int prec = effective_prec((int)((flags & MLK_PREC_MASK) >> 4), d);
     const bool exclude = (flags & MLK_EXCLUDE_SELF) != 0;
     st.screen_precision = (uint32_t)prec << 4;  // the precision actually screened
-
-    Events events;  // every event of this call (none in fixed mode)
-    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
-    if (!fixed) {
-        MLK_CHECK_HIP(events.make(&ev0));
-        MLK_CHECK_HIP(events.make(&ev1));
-        MLK_CHECK_HIP(events.make(&ev2));
-        MLK_CHECK_HIP(hipEventRecord(ev0, stream));
+    hipEvent_t ev1 = nullptr, ev2 = nullptr;
+    if (timed) {
+        MLK_CHECK_HIP(events->make(&ev1));
+        MLK_CHECK_HIP(events->make(&ev2));
     }
 
     if (fixed && !index->mirror[prec].ready) {
@@ -1352,14 +1376,14 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
     unsigned* dflag;
     MLK_TRY(mem.get(6, 64, &dflag));
     // flag words: [0] query rounding inexact, [2] fallback count, [4..10) three
-    // 64-bit statistics sums
-    MLK_CHECK_HIP(hipMemsetAsync(dflag, 0, 40, stream));
+    // 64-bit statistics sums (cleared by a kernel: graphs replay no memset nodes)
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, stream, (int*)dflag, (int64_t)10, 0);
     MLK_TRY(mirror_rows(prec, Q, m, m_pad, d, dp, index->centre.as<float>(), mr.scale, qmirror, nq, qflags, dflag,
                         qnt, qd2, stream));
     // the queries' rounding only matters when every database value is exact
     // in the screen type (integer data); otherwise no wait here
     unsigned q_inexact = 1;
-    if (mr.exact) {
+    if (mr.exact && !capture) {
         MLK_CHECK_HIP(hipMemcpyAsync(&q_inexact, dflag, 4, hipMemcpyDeviceToHost, stream));
         MLK_CHECK_HIP(hipStreamSynchronize(stream));
     }
@@ -1439,10 +1463,9 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
         MLK_CHECK_HIP(hipGetLastError());
     }
 
-    if (!fixed) MLK_CHECK_HIP(hipEventRecord(ev1, stream));
-    std::vector<hipEvent_t> round_events;
+    if (timed) MLK_CHECK_HIP(hipEventRecord(ev1, stream));
     const int* rank_stats = nullptr;  // per-query counts written by the rank (screened calls)
-    bool big_rank = false;            // k_rank ran: queries marked ST_BIG are ranked again
+    bool big_rank = false;
     RankArgs ra{};
     if (!all_fallback) {
         ScreenArgs sa{};
@@ -1477,7 +1500,6 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
             return MLK_OK;
         };
         double prev = 0.0;
-        std::vector<hipEvent_t> round_ev;
         for (size_t r = 0; r < plan.rounds.size(); ++r) {
             const double hi = r + 1 == plan.rounds.size() ? 1.0 : plan.rounds[r];
             sa.off_lo = (int64_t)std::llround(prev * sa.split_rows);
@@ -1485,23 +1507,24 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
             prev = hi;
             if (sa.off_hi <= sa.off_lo) continue;
             hipEvent_t ea = nullptr, eb = nullptr;
-            if (!fixed) {
-                MLK_CHECK_HIP(events.make(&ea));
-                MLK_CHECK_HIP(events.make(&eb));
-                round_ev.push_back(ea);
-                round_ev.push_back(eb);
+            if (timed) {
+                MLK_CHECK_HIP(events->make(&ea));
+                MLK_CHECK_HIP(events->make(&eb));
+                if (round_events) {
+                    round_events->push_back(ea);
+                    round_events->push_back(eb);
+                }
                 MLK_CHECK_HIP(hipEventRecord(ea, stream));
             }
             hipLaunchKernelGGL(screen, dim3(blocks), dim3(256), 0, stream, sa);
             MLK_CHECK_HIP(hipGetLastError());
-            if (!fixed) MLK_CHECK_HIP(hipEventRecord(eb, stream));
+            if (timed) MLK_CHECK_HIP(hipEventRecord(eb, stream));
             ++st.rounds;
             if (r + 1 < plan.rounds.size()) MLK_TRY(merge());
         }
-        round_events = std::move(round_ev);
         // final overflow check (the last round has no merge)
         MLK_TRY(merge());
-        if (!fixed) MLK_CHECK_HIP(hipEventRecord(ev2, stream));
+        if (timed) MLK_CHECK_HIP(hipEventRecord(ev2, stream));
 
         ra.X = X; ra.n = n; ra.d = d; ra.Q = Q; ra.m = m; ra.k = k;
         ra.cnt = sa.cnt; ra.cand = sa.cand; ra.nx32 = sa.nx32; ra.nq = nq;
@@ -1528,7 +1551,7 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
         MLK_CHECK_HIP(hipGetLastError());
         rank_stats = ra.qstats;
     } else {
-        if (!fixed) MLK_CHECK_HIP(hipEventRecord(ev2, stream));
+        if (timed) MLK_CHECK_HIP(hipEventRecord(ev2, stream));
         hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, stream, status, m,
                            ST_FALLBACK);
         MLK_CHECK_HIP(hipGetLastError());
@@ -1540,6 +1563,37 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
     hipLaunchKernelGGL(k_finish_status, dim3((unsigned)std::min<int64_t>(1024, (m + 255) / 256)), dim3(256), 0,
                        stream, m, status, fq, dflag + 2, rank_stats, (unsigned long long*)(dflag + 4));
     MLK_CHECK_HIP(hipGetLastError());
+    call.m = m; call.n = n; call.q_offset = q_offset; call.d = d; call.k = k; call.prec = prec;
+    call.flags = flags; call.exclude = exclude; call.Q = Q; call.X = X; call.dist = dist; call.idx = idx;
+    call.status = status; call.fq = fq; call.dflag = dflag; call.big_rank = big_rank; call.ra = ra; call.c = c;
+    call.rounds = st.rounds; call.splits = S; call.capacity = cap;
+    return MLK_OK;
+}
+
+// Completion of an enqueued call: one read-back of the fallback count and the
+// statistics, then the exact fallback of the queries without a certificate.
+static int knn_complete(mlk_index* index, const KnnCall& call, hipStream_t stream, CallMem& mem, Events* events,
+                        const std::vector<hipEvent_t>& round_events, hipEvent_t ev0) {
+    mlk_stats& st = stats();
+    const bool fixed = mem.fixed();
+    const int64_t m = call.m, n = call.n, q_offset = call.q_offset;
+    const int d = call.d, k = call.k;
+    const uint32_t flags = call.flags;
+    const bool exclude = call.exclude;
+    const float* Q = call.Q;
+    const float* X = call.X;
+    double* dist = call.dist;
+    int64_t* idx = call.idx;
+    int* status = call.status;
+    int* fq = call.fq;
+    unsigned* dflag = call.dflag;
+    const bool big_rank = call.big_rank;
+    const RankArgs& ra = call.ra;
+    st.screen_precision = (uint32_t)call.prec << 4;
+    st.rounds = call.rounds;
+    st.splits = call.splits;
+    st.capacity = call.capacity;
+    (void)index;
     // one read-back: the fallback count and the statistics sums
     struct {
         unsigned nf, pad;
@@ -1633,14 +1687,14 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
             MLK_CHECK_HIP(hipGetLastError());
         }
     }
-    if (fixed) {
+    if (!events) {
         // no events: the outputs are ordered on the stream; the fallback's host
         // list must outlive its copy
         if (nf) MLK_CHECK_HIP(hipStreamSynchronize(stream));
         return MLK_OK;
     }
     hipEvent_t ev3;
-    MLK_CHECK_HIP(events.make(&ev3));
+    MLK_CHECK_HIP(events->make(&ev3));
     MLK_CHECK_HIP(hipEventRecord(ev3, stream));
     MLK_CHECK_HIP(hipEventSynchronize(ev3));
     float ms = 0.0f;
@@ -1650,6 +1704,190 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
         if (hipEventElapsedTime(&ms, round_events[i], round_events[i + 1]) == hipSuccess) st.screen_ms += ms;
     if (hipEventElapsedTime(&ms, ev0, ev3) == hipSuccess) st.total_ms = ms;
     return MLK_OK;
+}
+
+// ---------------------------------------------------------------- graph replay
+// With MLK_GRAPH_REPLAY a call's device chain (query mirror, start threshold,
+// screen rounds and merges, rank, status) is captured once into a HIP graph;
+// a later call with the same arguments, plan and buffers replays it with one
+// launch and no host round trip before its single read-back. A graph holds
+// device addresses, so an entry is used only while every buffer it captured
+// is still the one in place.
+struct GraphEntry {
+    // what the captured chain depends on
+    const float* Q = nullptr;
+    int64_t m = 0, q_offset = 0;
+    int k = 0, device = -1;
+    uint32_t flags = 0;
+    double* dist = nullptr;
+    int64_t* idx = nullptr;
+    Plan plan;
+    std::vector<void*> buffers;  // every workspace slot and the mirror's buffers at capture time
+    hipGraphExec_t exec = nullptr;
+    KnnCall call;
+    uint64_t last_use = 0;
+};
+struct GraphCache {
+    std::vector<GraphEntry> entries;
+    hipStream_t capture_stream = nullptr;
+    uint64_t clock = 0;
+};
+constexpr size_t GRAPH_CACHE_MAX = 8;
+
+static std::vector<void*> index_buffers(const mlk_index* index, int prec) {
+    std::vector<void*> v;
+    for (const DevBuf& b : index->ws) v.push_back(b.ptr);
+    const Mirror& mr = index->mirror[prec];
+    for (const DevBuf* b : {&mr.xs, &mr.nx, &mr.nx32, &mr.flags, &mr.unsafe, &mr.bias, &index->X, &index->centre})
+        v.push_back(b->ptr);
+    return v;
+}
+
+static bool same_plan_fields(const Plan& a, const Plan& b) {
+    return a.shape.qt == b.shape.qt && a.shape.ns == b.shape.ns && a.splits == b.splits && a.cap == b.cap &&
+           a.rounds == b.rounds;
+}
+
+void destroy_graphs(mlk_index* index) {
+    GraphCache* gc = static_cast<GraphCache*>(index->graphs);
+    if (!gc) return;
+    for (GraphEntry& e : gc->entries)
+        if (e.exec) (void)hipGraphExecDestroy(e.exec);
+    if (gc->capture_stream) (void)hipStreamDestroy(gc->capture_stream);
+    delete gc;
+    index->graphs = nullptr;
+}
+
+static int knn_query_graph(mlk_index* index, const float* Q, int64_t m, int k, int64_t q_offset, uint32_t flags,
+                           double* dist, int64_t* idx, hipStream_t stream) {
+    const int prec = effective_prec((int)((flags & MLK_PREC_MASK) >> 4), index->d);
+    Mirror& mr = index->mirror[prec];
+    int device = 0;
+    MLK_CHECK_HIP(hipGetDevice(&device));
+    if (!index->graphs) index->graphs = new GraphCache();
+    GraphCache& gc = *static_cast<GraphCache*>(index->graphs);
+    const Plan plan = resolve_plan(PLAN_KNN, prec, mr.dp, index->n, m, k);
+    CallMem mem;
+    mem.ws = index->ws;
+    GraphEntry* hit = nullptr;
+    for (GraphEntry& e : gc.entries)
+        if (e.Q == Q && e.m == m && e.k == k && e.q_offset == q_offset && e.flags == flags && e.dist == dist &&
+            e.idx == idx && e.device == device && same_plan_fields(e.plan, plan) &&
+            e.buffers == index_buffers(index, prec)) {
+            hit = &e;
+            break;
+        }
+    if (!hit) {
+        // First call of this shape: an ordinary call answers it and sizes the
+        // workspace (nothing is allocated while capturing); then the chain is
+        // captured, with the bias its thresholds assume (queries taken as
+        // inexact) built beforehand.
+        {
+            mlk_stats& st0 = stats();
+            st0 = mlk_stats{};
+            st0.queries = m;
+            KnnCall probe;
+            MLK_TRY(knn_enqueue(index, Q, m, k, q_offset, flags, dist, idx, stream, mem, false, probe, nullptr,
+                                nullptr));
+            MLK_TRY(knn_complete(index, probe, stream, mem, nullptr, {}, nullptr));
+        }
+        const mlk_stats answered = stats();
+        const Consts c = make_consts(prec, mr.dp, index->d, mr.exact, false, mr.scale);
+        MLK_TRY(prepare_bias(index, prec, c, stream));
+        if (!gc.capture_stream) MLK_CHECK_HIP(hipStreamCreateWithFlags(&gc.capture_stream, hipStreamNonBlocking));
+        MLK_CHECK_HIP(hipStreamSynchronize(stream));
+        GraphEntry e;
+        e.Q = Q; e.m = m; e.k = k; e.q_offset = q_offset; e.flags = flags; e.dist = dist; e.idx = idx;
+        e.device = device; e.plan = plan;
+        MLK_CHECK_HIP(hipStreamBeginCapture(gc.capture_stream, hipStreamCaptureModeRelaxed));
+        const int rc = knn_enqueue(index, Q, m, k, q_offset, flags, dist, idx, gc.capture_stream, mem, true, e.call,
+                                   nullptr, nullptr);
+        hipGraph_t graph = nullptr;
+        const hipError_t ec = hipStreamEndCapture(gc.capture_stream, &graph);
+        if (rc != MLK_OK || ec != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            if (rc != MLK_OK) return rc;
+            MLK_CHECK_HIP(ec);
+        }
+        const hipError_t ei = hipGraphInstantiate(&e.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        MLK_CHECK_HIP(ei);
+        e.buffers = index_buffers(index, prec);
+        if (gc.entries.size() >= GRAPH_CACHE_MAX) {
+            auto oldest = std::min_element(gc.entries.begin(), gc.entries.end(),
+                                           [](const GraphEntry& a, const GraphEntry& b) { return a.last_use < b.last_use; });
+            (void)hipGraphExecDestroy(oldest->exec);
+            gc.entries.erase(oldest);
+        }
+        e.last_use = ++gc.clock;
+        gc.entries.push_back(std::move(e));
+        stats() = answered;
+        return MLK_OK;
+    }
+    hit->last_use = ++gc.clock;
+    // the index's bias must be the one the graph's thresholds assume
+    MLK_TRY(prepare_bias(index, prec, hit->call.c, stream));
+    mlk_stats& st = stats();
+    st = mlk_stats{};
+    st.queries = m;
+    MLK_CHECK_HIP(hipGraphLaunch(hit->exec, stream));
+    return knn_complete(index, hit->call, stream, mem, nullptr, {}, nullptr);
+}
+
+int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q_offset, uint32_t flags,
+                   double* dist, int64_t* idx, hipStream_t stream, CallMem* mem_in) {
+    mlk_stats& st = stats();
+    st = mlk_stats{};
+    st.queries = m;
+    if (m == 0) return MLK_OK;
+    const int prec = effective_prec((int)((flags & MLK_PREC_MASK) >> 4), index->d);
+    // graph replay: allocating calls without test hooks on a prepared index
+    if ((flags & MLK_GRAPH_REPLAY) && !mem_in && index->mirror[prec].ready && hooks_snapshot().hooks == 0)
+        return knn_query_graph(index, Q, m, k, q_offset, flags, dist, idx, stream);
+    CallMem own;
+    own.ws = index->ws;
+    CallMem& mem = mem_in ? *mem_in : own;
+    KnnCall call;
+    if (!mem.fixed() && index->mirror[prec].ready && m > KNN_CHUNK_MIN) {
+        // Candidate lists beyond KNN_LIST_BUDGET bytes: the queries go in chunks
+        // (each with its own plan), which bounds the workspace instead of growing
+        // it to the whole call (10^7 k-means rows would need 15 GB of lists).
+        const size_t lists = knn_list_bytes(index, prec, m, k);
+        if (lists > KNN_LIST_BUDGET) {
+            const int64_t parts = (int64_t)((lists + KNN_LIST_BUDGET - 1) / KNN_LIST_BUDGET);
+            const int64_t chunk = std::max<int64_t>(KNN_CHUNK_MIN, (m + parts - 1) / parts / 512 * 512);
+            mlk_stats total{};
+            for (int64_t off = 0; off < m; off += chunk) {
+                const int64_t mc = std::min<int64_t>(chunk, m - off);
+                MLK_TRY(knn_query_impl(index, Q + off * index->d, mc, k, q_offset + off, flags & ~MLK_GRAPH_REPLAY,
+                                       dist ? dist + off * k : nullptr, idx + off * k, stream, nullptr));
+                const mlk_stats& sc = stats();
+                total.queries += sc.queries;
+                total.candidates += sc.candidates;
+                total.reranked += sc.reranked;
+                total.fallback_queries += sc.fallback_queries;
+                total.rounds += sc.rounds;
+                total.splits = sc.splits;
+                total.capacity = sc.capacity;
+                total.screen_precision = sc.screen_precision;
+                total.screen_ms += sc.screen_ms;
+                total.total_ms += sc.total_ms;
+            }
+            stats() = total;
+            return MLK_OK;
+        }
+    }
+    if (mem.fixed()) {
+        MLK_TRY(knn_enqueue(index, Q, m, k, q_offset, flags, dist, idx, stream, mem, false, call, nullptr, nullptr));
+        return knn_complete(index, call, stream, mem, nullptr, {}, nullptr);
+    }
+    Events events;  // every event of this call
+    hipEvent_t ev0;
+    MLK_CHECK_HIP(events.make(&ev0));
+    MLK_CHECK_HIP(hipEventRecord(ev0, stream));
+    std::vector<hipEvent_t> round_events;
+    MLK_TRY(knn_enqueue(index, Q, m, k, q_offset, flags, dist, idx, stream, mem, false, call, &events, &round_events));
+    return knn_complete(index, call, stream, mem, &events, round_events, ev0);
 }
 
 }  // namespace mlk

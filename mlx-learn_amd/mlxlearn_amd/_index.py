@@ -117,20 +117,35 @@ class DeviceNeighborIndex:
         return Qd
 
     def query(self, Q, k, *, return_distance=True, exclude_self=False, q_offset=0, precision="float32",
-              squared=False):
-        """(float64 distances | None, int64 indices) as CUDA tensors, ordered by (S, index)."""
+              squared=False, graph=False, out=None):
+        """(float64 distances | None, int64 indices) as CUDA tensors, ordered by (S, index).
+
+        ``graph=True`` captures the call's device chain in a HIP graph and
+        replays it on later calls with the same query tensor, outputs and
+        shape (pass ``out=(dist, idx)`` to keep the outputs fixed); results
+        are identical either way.
+        """
         torch = _torch()
         Qd = self._queries(Q)
         m, k = int(Qd.shape[0]), int(k)
-        idx = torch.empty((m, k), dtype=torch.int64, device=self._device)
-        dist = torch.empty((m, k), dtype=torch.float64, device=self._device) if return_distance else None
+        if out is not None:
+            dist, idx = out
+            if idx.shape != (m, k) or idx.dtype != torch.int64 or not idx.is_cuda or not idx.is_contiguous():
+                raise ValueError(f"out indices must be a contiguous int64 CUDA tensor of shape {(m, k)}")
+            if return_distance and (dist is None or dist.shape != (m, k) or dist.dtype != torch.float64
+                                    or not dist.is_cuda or not dist.is_contiguous()):
+                raise ValueError(f"out distances must be a contiguous float64 CUDA tensor of shape {(m, k)}")
+            dist = dist if return_distance else None
+        else:
+            idx = torch.empty((m, k), dtype=torch.int64, device=self._device)
+            dist = torch.empty((m, k), dtype=torch.float64, device=self._device) if return_distance else None
         flags = precision_flag(precision) | (_lib.EXCLUDE_SELF if exclude_self else 0)
-        flags |= _lib.SQUARED_DIST if squared else 0
+        flags |= (_lib.SQUARED_DIST if squared else 0) | (_lib.GRAPH_REPLAY if graph else 0)
         _lib.check(self._lib.mlk_knn_query(self._handle, _ptr(Qd), m, k, int(q_offset), flags, _ptr(dist), _ptr(idx),
                                            _stream()), "mlk_knn_query")
         return dist, idx
 
-    def query_batched(self, batches, k, *, return_distance=True, precision="float32"):
+    def query_batched(self, batches, k, *, return_distance=True, precision="float32", graph=False):
         """Several query sets answered by one screen launch per round; list of results."""
         torch = _torch()
         for b in batches:
@@ -144,7 +159,8 @@ class DeviceNeighborIndex:
         dist = torch.empty((m, k), dtype=torch.float64, device=self._device) if return_distance else None
         _lib.check(self._lib.mlk_knn_query_batched(
             self._handle, _ptr(Qd), offsets.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), len(parts), int(k),
-            precision_flag(precision), _ptr(dist), _ptr(idx), _stream()), "mlk_knn_query_batched")
+            precision_flag(precision) | (_lib.GRAPH_REPLAY if graph else 0), _ptr(dist), _ptr(idx), _stream()),
+            "mlk_knn_query_batched")
         out = []
         for b in range(len(parts)):
             lo, hi = offsets[b], offsets[b + 1]
@@ -172,6 +188,26 @@ class DeviceNeighborIndex:
         finally:
             self._lib.mlk_radius_free(res)
         return indptr, indices, (dist if need_distance else None)
+
+    def radius_batched(self, batches, R2, *, sort_results=False, need_distance=True, precision="float32"):
+        """Radius neighbourhoods of several query sets in one call (their rows
+        concatenated): a list of (indptr, indices, distances | None) per batch,
+        each indptr starting at 0."""
+        torch = _torch()
+        for b in batches:
+            self._check_width(b)
+        parts = [self._queries(b) for b in batches]
+        sizes = [int(p.shape[0]) for p in parts]
+        Qd = torch.cat(parts) if parts else torch.empty((0, self.n_features), device=self._device)
+        indptr, indices, dist = self.radius(Qd, R2, sort_results=sort_results, need_distance=need_distance,
+                                            precision=precision)
+        out, lo = [], 0
+        for sz in sizes:
+            ip = indptr[lo:lo + sz + 1]
+            a, b = int(ip[0]), int(ip[-1])
+            out.append((ip - a, indices[a:b], None if dist is None else dist[a:b]))
+            lo += sz
+        return out
 
     def radius_reduce(self, Q, R2, *, codes=None, n_classes=None, targets=None, weighted=False,
                       exclude_self=False, q_offset=0, precision="float32"):

@@ -21,6 +21,7 @@ PREC_FP16 = 0x20
 SORT_RESULTS = 0x100
 NEED_DISTANCE = 0x200
 SQUARED_DIST = 0x400
+GRAPH_REPLAY = 0x800
 WEIGHTS_UNIFORM = 0
 WEIGHTS_DISTANCE = 1
 HOOK_FORCE_FALLBACK = 0x1

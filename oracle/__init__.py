@@ -15,6 +15,10 @@ Contents
 * ``radius_band`` -- the outward-rounded float64 band of
   ``radius_contract.thresholds`` (``radius_contract.py:129-153``), via the C
   oracle.
+* ``KMeansPort`` -- float32 Gram argmin + float64 check (the k-means CPU
+  baseline, ``e_means.py:50-69``).
+* ``reference_radius`` -- the reference's own radius engine compiled into
+  ``oracle/_ref`` (the radius CPU baseline), when it is built.
 * ``GramPort`` -- numpy restatement of the reference CPU path
   ``CPUIndex._gram`` / ``_direct`` / ``_exact_order``
   (``neighbors_cpu.py:174-181, 213-262, 314-364``), used as the timed CPU
@@ -406,6 +410,101 @@ class GramPort:
             sq, ids = self._direct(Q[row:row + 1], k, exclude_self, offset + row)
             cand[row, :k], values[row, :k] = ids[0], sq[0]
         return values[:, :k], cand[:, :k]
+
+
+class KMeansPort:
+    """The reference's k-means assignment on the CPU: float32 Gram argmin plus a
+    float64 check (the timed CPU baseline of the 10M x 96 workload).
+
+    Restates ``e_means.assign``'s scoring (``e_means.py:50-69``): per row the
+    float32 score ``|c|^2 - 2 x.c`` over every centre (one BLAS sgemm per row
+    block, all host cores), the smallest score with the lowest index on ties.
+    The float64 check recomputes the two best centres' squared distances in
+    float64; a row whose two best are within the float32 scores' error is
+    resolved by float64 distances to every centre (``direct``). The reference
+    itself stops at the float32 argmin; the check is the certification the
+    exact contract needs.
+    """
+
+    def __init__(self, C):
+        self.C = _c32(C)
+        self.C64 = self.C.astype(np.float64)
+        self.norms = np.einsum("ij,ij->i", self.C64, self.C64).astype(np.float32)
+
+    def assign(self, X, block=2048):
+        X = _c32(X)
+        labels = np.empty(len(X), np.int64)
+        resid = np.empty(len(X), np.float64)
+        d = X.shape[1]
+        for s in range(0, len(X), block):
+            xb = X[s:s + block]
+            scores = self.norms[None, :] - np.float32(2.0) * (xb @ self.C.T)
+            two = np.argpartition(scores, 1, axis=1)[:, :2]
+            first = np.argmin(scores, axis=1)                      # lowest index among equal minima
+            rows = np.arange(len(xb))
+            other = np.where(two[:, 0] == first, two[:, 1], two[:, 0])
+            x64 = xb.astype(np.float64)
+            d_first = np.sum((x64 - self.C64[first]) ** 2, axis=1)
+            d_other = np.sum((x64 - self.C64[other]) ** 2, axis=1)
+            # float32 score error: about (d + 2) 2^-24 (|x|^2 + |c|^2) per score
+            slack = (d + 2) * 2.0 ** -23 * (np.sum(x64 * x64, axis=1) + self.norms[first] + self.norms[other])
+            close = np.flatnonzero(np.abs(d_first - d_other) <= slack)
+            for r in close:
+                full = np.sum((x64[r] - self.C64) ** 2, axis=1)
+                first[r] = int(np.argmin(full))
+                d_first[r] = full[first[r]]
+            labels[s:s + block] = first
+            resid[s:s + block] = d_first
+            del rows
+        return labels, resid
+
+
+def reference_radius_engine():
+    """The reference's own radius engine (``_radius_cpu_native``), compiled from
+    its sources into ``oracle/_ref`` by ``build_oracle.py``; None when absent."""
+    ref = os.path.join(HERE, "_ref")
+    if not os.path.isdir(ref):
+        return None
+    import sys
+    if ref not in sys.path:
+        sys.path.insert(0, ref)
+    try:
+        import _radius_cpu_native
+    except ImportError:
+        return None
+    return _radius_cpu_native
+
+
+def reference_thresholds(d, R2):
+    """The threshold tuple the reference engine takes (R2, f64_in, f64_out,
+    prune_above, bulk_below), built as ``radius_contract.thresholds`` builds
+    it (radius_contract.py:129-153); the band pair is ``radius_band``'s."""
+    if R2 == 0.0:
+        return (0.0, 0.0, 0.0, 0.0, 0.0)
+    lo, hi = radius_band(d, R2)
+    u = 2.0 ** -53
+    g = (d + 2) * u / (1.0 - (d + 2) * u)
+    dn = lambda v: math.nextafter(v, -math.inf)  # noqa: E731
+    up = lambda v: math.nextafter(v, math.inf)  # noqa: E731
+    return (float(R2), lo, hi, up(up(R2 * up(1.0 + 4.0 * g))), max(dn(dn(R2 * dn(1.0 - 4.0 * g))), 0.0))
+
+
+def reference_radius(X, Q, R2, *, sort_results=False, threads=None):
+    """CSR of the reference engine's ``brute_query`` + ``result_fill_csr``
+    (``_radius_cpu_native.cpp:219-310, 692-800``)."""
+    rr = reference_radius_engine()
+    if rr is None:
+        raise RuntimeError("the reference radius engine is not built (oracle/_ref)")
+    X, Q = _c32(X), _c32(Q)
+    mode = rr.NEED_DISTANCE | (rr.SORT if sort_results else 0)
+    res = rr.brute_query(X, Q, reference_thresholds(X.shape[1], float(R2)), 0, False, mode,
+                         threads or default_threads())
+    nq, nnz = rr.result_sizes(res)
+    indptr = np.empty(nq + 1, np.int64)
+    indices = np.empty(nnz, np.int64)
+    dist = np.empty(nnz, np.float64)
+    rr.result_fill_csr(res, indptr, indices, dist)
+    return indptr, indices, dist
 
 
 # ------------------------------------------------------------------ generators

@@ -1,0 +1,136 @@
+"""Uneven query batches in one call, and graph-replayed k-NN calls, against
+per-batch and eager calls (bit-identical results)."""
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def mlk():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import mlxlearn_amd
+    from mlxlearn_amd import _lib
+    _lib.load()
+    return mlxlearn_amd
+
+
+def uneven_sizes(rng, count, top):
+    """Log-uniform sizes in 1..top (the 256-batch radius workload's rule), some batches empty."""
+    sizes = np.rint(np.exp(rng.uniform(0, np.log(top), count))).astype(int)
+    sizes[::37] = 0
+    return sizes
+
+
+@pytest.mark.parametrize("precision", ["float32", "bfloat16"])
+def test_256_uneven_knn_batches_equal_per_batch_calls(mlk, precision):
+    import torch
+    rng = np.random.default_rng(256)
+    X = rng.normal(size=(20_000, 48)).astype(np.float32)
+    X[10:30] = X[9]
+    sizes = uneven_sizes(rng, 256, 600)
+    batches = [rng.normal(size=(s, 48)).astype(np.float32) for s in sizes]
+    batches[5] = X[:max(1, sizes[5])].copy()
+    index = mlk.DeviceNeighborIndex(torch.from_numpy(X).cuda())
+    print(f"256 k-NN batches, {int(sizes.sum())} queries, {precision}", flush=True)
+    together = index.query_batched(batches, 7, precision=precision)
+    for b, Qb in enumerate(batches):
+        if len(Qb) == 0:
+            assert together[b][1].shape == (0, 7)
+            continue
+        d, i = index.query(Qb, 7, precision=precision)
+        np.testing.assert_array_equal(together[b][1].cpu().numpy(), i.cpu().numpy(), err_msg=f"batch {b}")
+        np.testing.assert_array_equal(together[b][0].cpu().numpy(), d.cpu().numpy(), err_msg=f"batch {b}")
+    allq = np.concatenate([b for b in batches if len(b)])
+    wd, wi = oracle.knn(X, allq, 7)
+    got = np.concatenate([t[1].cpu().numpy() for t in together if t[1].shape[0]])
+    np.testing.assert_array_equal(got, wi)
+    # the graph-replayed batched call (captured on the second call) gives the same bits
+    for rep in range(3):
+        again = index.query_batched(batches, 7, precision=precision, graph=True)
+        got2 = np.concatenate([t[1].cpu().numpy() for t in again if t[1].shape[0]])
+        np.testing.assert_array_equal(got2, wi, err_msg=f"graph call {rep}")
+    index.close()
+
+
+def test_256_uneven_radius_batches_equal_per_batch_calls(mlk):
+    import torch
+    rng = np.random.default_rng(257)
+    X, _, _, radius = oracle.gen_c4(seed=3, n=100_000, n_batches=1, max_batch=8)
+    sizes = uneven_sizes(rng, 256, 400)
+    batches = [oracle.gen_c4_points(rng, int(s), 32, X[:64].astype(np.float64)) if s else np.empty((0, 32), np.float32)
+               for s in sizes]
+    R2 = radius * radius
+    index = mlk.DeviceNeighborIndex(torch.from_numpy(X).cuda())
+    print(f"256 radius batches, {int(sizes.sum())} queries", flush=True)
+    together = index.radius_batched(batches, R2, sort_results=True)
+    for b, Qb in enumerate(batches):
+        ip, ind, dst = together[b]
+        assert ip.shape[0] == len(Qb) + 1
+        if len(Qb) == 0:
+            continue
+        wp, wi, wd = index.radius(Qb, R2, sort_results=True)
+        np.testing.assert_array_equal(ip.cpu().numpy(), wp.cpu().numpy(), err_msg=f"batch {b}")
+        np.testing.assert_array_equal(ind.cpu().numpy(), wi.cpu().numpy(), err_msg=f"batch {b}")
+        np.testing.assert_array_equal(dst.cpu().numpy(), wd.cpu().numpy(), err_msg=f"batch {b}")
+    for b in range(0, 256, 32):
+        if len(batches[b]) == 0:
+            continue
+        wp, wi, wd = oracle.radius(X, batches[b], R2, sort_results=True)
+        np.testing.assert_array_equal(together[b][1].cpu().numpy(), wi)
+    index.close()
+
+
+@pytest.mark.parametrize("precision", ["float32", "bfloat16", "float16"])
+def test_graph_replays_equal_eager_calls(mlk, precision):
+    """Capture on the first graph call, replay on later ones; eager work and synchronisation between
+    replays; the query contents change between replays (same buffer); every replay is checked."""
+    import torch
+    rng = np.random.default_rng(7)
+    X = rng.normal(size=(30_000, 64)).astype(np.float32)
+    X[100:140] = X[3]
+    index = mlk.DeviceNeighborIndex(torch.from_numpy(X).cuda())
+    Qd = torch.from_numpy(rng.normal(size=(1000, 64)).astype(np.float32)).cuda()
+    dist = torch.empty((1000, 5), dtype=torch.float64, device="cuda")
+    idx = torch.empty((1000, 5), dtype=torch.int64, device="cuda")
+    for rep in range(5):
+        if rep == 3:
+            Qd.copy_(torch.from_numpy(np.concatenate([X[:40], rng.normal(size=(960, 64)).astype(np.float32)])))
+        want_d, want_i = index.query(Qd, 5, precision=precision)            # eager
+        torch.cuda.synchronize()
+        idx.fill_(-1)
+        dist.fill_(-1.0)
+        index.query(Qd, 5, precision=precision, graph=True, out=(dist, idx))
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(idx.cpu().numpy(), want_i.cpu().numpy(), err_msg=f"replay {rep}")
+        np.testing.assert_array_equal(dist.cpu().numpy(), want_d.cpu().numpy(), err_msg=f"replay {rep}")
+    wd, wi = oracle.knn(X, Qd.cpu().numpy(), 5)
+    np.testing.assert_array_equal(idx.cpu().numpy(), wi)
+    index.close()
+
+
+def test_graph_survives_workspace_regrowth_and_fallback(mlk):
+    """A larger call regrows the workspace (the captured graph no longer matches and is captured
+    again); queries outside the screen's safe range take the exact fallback after the replay."""
+    import torch
+    rng = np.random.default_rng(8)
+    X = rng.normal(size=(8_000, 32)).astype(np.float32)
+    index = mlk.DeviceNeighborIndex(torch.from_numpy(X).cuda())
+    Q = rng.normal(size=(300, 32)).astype(np.float32)
+    Q[7] *= np.float32(1e30)                      # unsafe for the screen: exact fallback
+    Qd = torch.from_numpy(Q).cuda()
+    out = (torch.empty((300, 4), dtype=torch.float64, device="cuda"),
+           torch.empty((300, 4), dtype=torch.int64, device="cuda"))
+    wd, wi = oracle.knn(X, Q, 4)
+    for rep in range(3):
+        index.query(Qd, 4, precision="bfloat16", graph=True, out=out)
+        np.testing.assert_array_equal(out[1].cpu().numpy(), wi, err_msg=f"call {rep}")
+        np.testing.assert_allclose(out[0].cpu().numpy(), wd, rtol=1e-12)
+        if rep == 1:   # a bigger call regrows every workspace slot
+            big = torch.from_numpy(rng.normal(size=(20_000, 32)).astype(np.float32)).cuda()
+            index.query(big, 9, precision="bfloat16")
+    index.close()

@@ -328,3 +328,33 @@ def test_cpu_baseline_port_is_exact(rng):
     wd, wi = oracle.knn(X, Q, 7)
     np.testing.assert_array_equal(i, wi)
     np.testing.assert_allclose(d, wd, rtol=1e-12)
+
+
+def test_kmeans_cpu_port_labels_are_exact(rng):
+    """The timed k-means CPU baseline (float32 argmin + float64 check) returns the exact labels,
+    lowest index on ties (duplicate centres, rows on centres, near-equidistant rows)."""
+    X = rng.normal(size=(4000, 24)).astype(np.float32)
+    C = X[:300].copy()
+    C[7] = C[3]                                   # duplicate centres: label 3, never 7
+    mid = ((C[10].astype(np.float64) + C[11]) / 2).astype(np.float32)
+    X[-5:] = mid                                  # near-equidistant rows
+    labels, resid = oracle.KMeansPort(C).assign(X, block=512)
+    want, wres = oracle.kmeans_assign(X, C)
+    np.testing.assert_array_equal(labels, want)
+    np.testing.assert_allclose(resid, wres, rtol=1e-13, atol=0)
+    assert not np.any(labels == 7)
+
+
+def test_reference_radius_engine_matches_oracle(rng):
+    """The reference's compiled radius engine, called as the radius CPU baseline calls it,
+    gives the oracle's CSR (sorted and unsorted)."""
+    if oracle.reference_radius_engine() is None:
+        pytest.skip("reference native engines not built (oracle/_ref)")
+    X = rng.random((20000, 8), dtype=np.float32)
+    X[100:140] = X[5]
+    Q = np.concatenate([X[:3], rng.random((60, 8), dtype=np.float32)])
+    for sort in (False, True):
+        got = oracle.reference_radius(X, Q, 0.09, sort_results=sort, threads=4)
+        want = oracle.radius(X, Q, 0.09, sort_results=sort)
+        for g, w in zip(got, want):
+            np.testing.assert_array_equal(g, w)
