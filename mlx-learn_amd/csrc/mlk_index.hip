@@ -86,13 +86,24 @@ template <> struct ScreenType<PREC_FP16> { typedef _Float16 T; };
 
 // Safe range of the scaled centred value for each screen type: zero, or a
 // magnitude whose rounding stays normal and whose products with any other safe
-// value are normal float32 numbers.
+// value are normal float32 numbers, up to `hi` (safe_limit).
 template <int PREC>
-__device__ __forceinline__ bool safe_value(double v) {
+__device__ __forceinline__ bool safe_value(double v, double hi) {
     double a = fabs(v);
     if (a == 0.0) return true;
     if (PREC == PREC_FP16) return a >= 0x1p-14 && a <= 0x1p15;
-    return a >= 0x1p-60 && a <= 0x1p56;
+    return a >= 0x1p-60 && a <= hi;
+}
+
+// Largest safe magnitude for rows of d features: 2^56 (products of two safe
+// values stay below 2^112), lowered for very wide rows so that d hi^2 <= 2^124:
+// a safe row's dot products and norms then stay below 2^124 and its screen
+// accumulator below 2^125, under the screen's lowest threshold (THR_FLOOR).
+static double safe_limit(int prec, int d) {
+    if (prec == PREC_FP16) return 0x1p15;
+    int e = 0;
+    while ((int64_t(1) << e) < d) ++e;  // 2^e >= d
+    return std::ldexp(1.0, std::min(56, (124 - e) / 2));
 }
 
 template <int PREC>
@@ -106,7 +117,7 @@ __device__ __forceinline__ typename ScreenType<PREC>::T to_screen(double v) {
 // Used for database rows (flags_out per row) and for query rows.
 template <int PREC, int R, int T>
 __global__ __launch_bounds__(256) void k_mirror(const float* __restrict__ X, int64_t n, int64_t n_pad, int d, int dp,
-                                                const float* __restrict__ c, double scale,
+                                                const float* __restrict__ c, double scale, double safe_hi,
                                                 typename ScreenType<PREC>::T* __restrict__ xs,
                                                 double* __restrict__ norms, uint8_t* __restrict__ flags,
                                                 unsigned* __restrict__ inexact, double* __restrict__ nt_out,
@@ -129,7 +140,7 @@ __global__ __launch_bounds__(256) void k_mirror(const float* __restrict__ X, int
         for (int j = r; j < d && real; j += R) {
             const double x = ((double)X[row * d + j] - (double)c[j]) * scale;
             s += x * x;
-            safe &= safe_value<PREC>(x);
+            safe &= safe_value<PREC>(x, safe_hi);
             const double xt = (double)(float)to_screen<PREC>(x);
             exact &= xt == x;
             st += xt * xt;
@@ -156,7 +167,7 @@ __global__ __launch_bounds__(256) void k_mirror(const float* __restrict__ X, int
         if (real && j < d) {
             v[t] = ((double)X[row * d + j] - (double)c[j]) * scale;
             s += v[t] * v[t];
-            safe &= safe_value<PREC>(v[t]);
+            safe &= safe_value<PREC>(v[t], safe_hi);
             const double xt = (double)(float)to_screen<PREC>(v[t]);
             exact &= xt == v[t];
             st += xt * xt;
@@ -216,10 +227,11 @@ static void launch_mirror(const float* X, int64_t n, int64_t n_pad, int d, int d
                           double scale, void* xs, double* norms, uint8_t* flags, unsigned* inexact, double* nt,
                           double* d2, hipStream_t stream) {
     typedef typename ScreenType<PREC>::T T_;
+    const double hi = safe_limit(PREC, d);
     auto go = [&](auto kern, int lanes) {
         const int64_t rows_per_block = 256 / lanes;
         dim3 grid((unsigned)((n_pad + rows_per_block - 1) / rows_per_block));
-        hipLaunchKernelGGL(kern, grid, dim3(256), 0, stream, X, n, n_pad, d, dp, c, scale, (T_*)xs, norms, flags,
+        hipLaunchKernelGGL(kern, grid, dim3(256), 0, stream, X, n, n_pad, d, dp, c, scale, hi, (T_*)xs, norms, flags,
                            inexact, nt, d2);
     };
     // lanes per row: about four features per lane, at least 16 lanes

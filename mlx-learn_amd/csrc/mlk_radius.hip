@@ -79,10 +79,9 @@ __global__ void k_radius_thr(int64_t m, int64_t m_pad, const double* __restrict_
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= m_pad) return;
     if (q >= m) { thr[q] = INFINITY; return; }
-    // Unsafe query (screen values zero): every safe row is a candidate. A safe
-    // row's accumulator is -a_x > -2^120, an unsafe row's is -3e38, so -1e38
-    // passes exactly the safe rows; k_unsafe_pairs adds the unsafe ones once.
-    thr[q] = qflags[q] ? thr_from_tau(tau, screen_qa(nq[q], nt[q], d2[q], c), c) : -1e38f;
+    // Unsafe query (screen values zero): every safe row is a candidate, and
+    // THR_FLOOR passes exactly the safe rows; k_unsafe_pairs adds the unsafe ones once.
+    thr[q] = qflags[q] ? thr_from_tau(tau, screen_qa(nq[q], nt[q], d2[q], c), c) : THR_FLOOR;
 }
 
 // Unsafe database rows never hit the screen; they are candidates of every query.
@@ -105,17 +104,12 @@ __global__ void k_pair_keys(const int2* __restrict__ pairs, int64_t np, int rb, 
     if (i < np) keys[i] = ((unsigned long long)(unsigned)pairs[i].x << rb) | (unsigned)pairs[i].y;
 }
 
-// A classified pair: its (query << rb) | row key and its distance.
-struct KeyDist {
-    unsigned long long key;
-    double dist;
-};
-
+// Membership and distance of every sorted candidate pair.
 // VEC: d % 4 == 0 and 16-byte aligned rows (four features per load).
 template <bool VEC>
 __global__ void k_classify(const unsigned long long* __restrict__ keys, int64_t np, int rb, const float* __restrict__ X,
                            const float* __restrict__ Q, int d, RadiusConsts rc, uint8_t* __restrict__ member,
-                           KeyDist* __restrict__ kd) {
+                           double* __restrict__ pdist) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
     const unsigned long long key = keys[i];
@@ -129,32 +123,100 @@ __global__ void k_classify(const unsigned long long* __restrict__ keys, int64_t 
     else if (!(s <= rc.f_out)) in = false;
     else in = exact_cmp_key(qv, xv, d, rc.key) <= 0;
     member[i] = in;
-    kd[i] = KeyDist{key, sqrt(s < rc.R2 ? s : rc.R2)};
+    pdist[i] = sqrt(s < rc.R2 ? s : rc.R2);
 }
 
-// CSR row pointers from the sorted member keys: indptr[q] is the first key
-// whose query part is >= q (binary search; indptr[m] = nnz).
-__global__ void k_indptr_search(const KeyDist* __restrict__ kd, int64_t nnz, int64_t m, int rb,
-                                int64_t* __restrict__ indptr) {
+// Stable compaction of the members (pairs in (query, row) order): blocks of
+// MEMBER_TILE pairs. Pass 1 counts each block's members (16 consecutive flags
+// per thread); a scan of the counts gives each block's first output position;
+// pass 2 walks the block's pairs in rounds of 256 and writes every member's
+// row, distance and query at its position.
+constexpr int MEMBER_PER_THREAD = 16;
+constexpr int MEMBER_TILE = 256 * MEMBER_PER_THREAD;
+
+__device__ __forceinline__ int member_count16(const uint8_t* __restrict__ member, int64_t np, int64_t at) {
+    int c = 0;
+    if (at + MEMBER_PER_THREAD <= np && (at & 15) == 0) {
+        const uint4 v = *reinterpret_cast<const uint4*>(member + at);
+        // flags are 0 or 1: the byte sum of each word is its popcount
+        for (uint32_t w : {v.x, v.y, v.z, v.w}) c += __popc(w);
+    } else {
+        for (int j = 0; j < MEMBER_PER_THREAD; ++j)
+            if (at + j < np) c += member[at + j];
+    }
+    return c;
+}
+
+__global__ __launch_bounds__(256) void k_member_count(const uint8_t* __restrict__ member, int64_t np,
+                                                      unsigned long long* __restrict__ bcount) {
+    __shared__ int part[4];
+    const int64_t at = (int64_t)blockIdx.x * MEMBER_TILE + (int64_t)threadIdx.x * MEMBER_PER_THREAD;
+    int c = member_count16(member, np, at);
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) bcount[blockIdx.x] = (unsigned long long)(part[0] + part[1] + part[2] + part[3]);
+}
+
+__global__ __launch_bounds__(256) void k_member_scatter(const uint8_t* __restrict__ member,
+                                                        const unsigned long long* __restrict__ keys,
+                                                        const double* __restrict__ pdist, int64_t np, int rb,
+                                                        const unsigned long long* __restrict__ bend,
+                                                        const unsigned long long* __restrict__ bcount,
+                                                        int64_t* __restrict__ idx, double* __restrict__ dist,
+                                                        int* __restrict__ mq) {
+    // MEMBER_PER_THREAD rounds of 256 consecutive pairs: in each, a member's
+    // position is the block's running count plus the members before it in the
+    // round (its wave's earlier lanes, then the earlier waves), so consecutive
+    // members go to consecutive positions (coalesced reads and writes)
+    __shared__ int wcount[MEMBER_PER_THREAD][4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t base = (int64_t)blockIdx.x * MEMBER_TILE;
+    bool flag[MEMBER_PER_THREAD];
+    int rank[MEMBER_PER_THREAD];
+#pragma unroll
+    for (int j = 0; j < MEMBER_PER_THREAD; ++j) {
+        const int64_t i = base + (int64_t)j * 256 + threadIdx.x;
+        flag[j] = i < np && member[i];
+        const unsigned long long mask = ballot(flag[j]);
+        rank[j] = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0));
+        if (lane == 0) wcount[j][wave] = __popcll(mask);
+    }
+    __syncthreads();
+    int64_t run = (int64_t)(bend[blockIdx.x] - bcount[blockIdx.x]);
+    const uint64_t row_mask = (1ull << rb) - 1;
+#pragma unroll
+    for (int j = 0; j < MEMBER_PER_THREAD; ++j) {
+        int before = 0, total = 0;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            const int c = wcount[j][w];
+            before += w < wave ? c : 0;
+            total += c;
+        }
+        if (flag[j]) {
+            const int64_t i = base + (int64_t)j * 256 + threadIdx.x;
+            const int64_t pos = run + before + rank[j];
+            const unsigned long long key = keys[i];
+            idx[pos] = (int64_t)(key & row_mask);
+            if (dist) dist[pos] = pdist[i];
+            mq[pos] = (int)(key >> rb);
+        }
+        run += total;
+    }
+}
+
+// CSR row pointers from the members' queries (non-decreasing): indptr[q] is
+// the first member whose query is >= q (binary search; indptr[m] = nnz).
+__global__ void k_indptr_search(const int* __restrict__ mq, int64_t nnz, int64_t m, int64_t* __restrict__ indptr) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q > m) return;
-    const unsigned long long target = (unsigned long long)q << rb;
     int64_t lo = 0, hi = nnz;
     while (lo < hi) {
         const int64_t mid = lo + ((hi - lo) >> 1);
-        if (kd[mid].key < target) lo = mid + 1; else hi = mid;
+        if ((int64_t)mq[mid] < q) lo = mid + 1; else hi = mid;
     }
     indptr[q] = lo;
-}
-
-// Member records -> CSR columns (row part of the key) and, if kept, distances.
-__global__ void k_split_members(const KeyDist* __restrict__ kd, int64_t nnz, int rb, int64_t* __restrict__ idx,
-                                double* __restrict__ dist) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nnz) return;
-    const KeyDist r = kd[i];
-    idx[i] = (int64_t)(r.key & ((1ull << rb) - 1));
-    if (dist) dist[i] = r.dist;
 }
 
 // All rows (except self) are members: R2 >= 2^342 or +inf.
@@ -537,7 +599,7 @@ int radius_query_impl(mlk_index* index, const float* Q, int64_t m, double R2, in
     while ((1ll << rb) < n) ++rb;
     MLK_TRY(ws[10].reserve((size_t)std::max<int64_t>(np, 1) * 8));   // keys in
     MLK_TRY(ws[11].reserve((size_t)std::max<int64_t>(np, 1) * 8));   // keys sorted
-    MLK_TRY(ws[12].reserve((size_t)std::max<int64_t>(np, 1) * sizeof(KeyDist)));  // (key, distance) records
+    MLK_TRY(ws[12].reserve((size_t)std::max<int64_t>(np, 1) * sizeof(double)));  // distances of the pairs
     MLK_TRY(ws[13].reserve((size_t)std::max<int64_t>(np, 1) + 16));  // member flags
     unsigned long long* keys = ws[10].as<unsigned long long>();
     unsigned long long* skeys = ws[11].as<unsigned long long>();
@@ -551,36 +613,39 @@ int radius_query_impl(mlk_index* index, const float* Q, int64_t m, double R2, in
         MLK_CHECK_HIP(hipcub::DeviceRadixSort::SortKeys(ws[14].ptr, tmp_bytes, keys, skeys, (int)np, 0, rb + qbits, stream));
         const bool vec = d % 4 == 0 && ((uintptr_t)Q % 16) == 0 && ((uintptr_t)X % 16) == 0;
         hipLaunchKernelGGL(vec ? k_classify<true> : k_classify<false>, dim3(blocks_for(np)), dim3(256), 0, stream,
-                           skeys, np, rb, X, Q, d, rc, ws[13].as<uint8_t>(), ws[12].as<KeyDist>());
+                           skeys, np, rb, X, Q, d, rc, ws[13].as<uint8_t>(), ws[12].as<double>());
         MLK_CHECK_HIP(hipGetLastError());
     }
-    // compact members (order preserved)
-    MLK_TRY(ws[15].reserve(64));
-    int64_t* dnnz = ws[15].as<int64_t>();
-    MLK_CHECK_HIP(hipMemsetAsync(dnnz, 0, sizeof(int64_t), stream));
-    DevBuf& members = ws[16];  // member records (workspace, grown on demand)
-    MLK_TRY(members.reserve((size_t)std::max<int64_t>(np, 1) * sizeof(KeyDist)));
-    if (np) {
-        size_t tmp_bytes = 0;
-        MLK_CHECK_HIP(hipcub::DeviceSelect::Flagged(nullptr, tmp_bytes, ws[12].as<KeyDist>(), ws[13].as<uint8_t>(),
-                                                    members.as<KeyDist>(), dnnz, (int)np, stream));
-        MLK_TRY(ws[14].reserve(tmp_bytes + 16));
-        MLK_CHECK_HIP(hipcub::DeviceSelect::Flagged(ws[14].ptr, tmp_bytes, ws[12].as<KeyDist>(), ws[13].as<uint8_t>(),
-                                                    members.as<KeyDist>(), dnnz, (int)np, stream));
-    }
+    // compact the members (order preserved): block counts, their inclusive
+    // scan (the last entry is nnz), then each block's members at its offset
+    const int64_t nblk = (np + MEMBER_TILE - 1) / MEMBER_TILE;
+    MLK_TRY(ws[15].reserve((size_t)std::max<int64_t>(nblk, 1) * 16 + 64));
+    unsigned long long* bcount = ws[15].as<unsigned long long>();
+    unsigned long long* bend = bcount + std::max<int64_t>(nblk, 1);
     int64_t nnz = 0;
-    MLK_CHECK_HIP(hipMemcpyAsync(&nnz, dnnz, sizeof(int64_t), hipMemcpyDeviceToHost, stream));
-    MLK_CHECK_HIP(hipStreamSynchronize(stream));
-
-    // CSR row pointers
-    hipLaunchKernelGGL(k_indptr_search, dim3(blocks_for(m + 1)), dim3(256), 0, stream, members.as<KeyDist>(), nnz, m,
-                       rb, indptr);
-    MLK_CHECK_HIP(hipGetLastError());
+    if (np) {
+        hipLaunchKernelGGL(k_member_count, dim3((unsigned)nblk), dim3(256), 0, stream, ws[13].as<uint8_t>(), np,
+                           bcount);
+        size_t tmp_bytes = 0;
+        MLK_CHECK_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, tmp_bytes, bcount, bend, (int)nblk, stream));
+        MLK_TRY(ws[14].reserve(tmp_bytes + 16));
+        MLK_CHECK_HIP(hipcub::DeviceScan::InclusiveSum(ws[14].ptr, tmp_bytes, bcount, bend, (int)nblk, stream));
+        unsigned long long total = 0;
+        MLK_CHECK_HIP(hipMemcpyAsync(&total, bend + nblk - 1, sizeof(total), hipMemcpyDeviceToHost, stream));
+        MLK_CHECK_HIP(hipStreamSynchronize(stream));
+        nnz = (int64_t)total;
+    }
     MLK_TRY(res->indices.reserve((size_t)std::max<int64_t>(nnz, 1) * sizeof(int64_t)));
     if (res->has_dist) MLK_TRY(res->dist.reserve((size_t)std::max<int64_t>(nnz, 1) * sizeof(double)));
+    MLK_TRY(ws[16].reserve((size_t)std::max<int64_t>(nnz, 1) * sizeof(int)));  // each member's query
+    int* mq = ws[16].as<int>();
     if (nnz)
-        hipLaunchKernelGGL(k_split_members, dim3(blocks_for(nnz)), dim3(256), 0, stream, members.as<KeyDist>(), nnz,
-                           rb, res->indices.as<int64_t>(), res->has_dist ? res->dist.as<double>() : nullptr);
+        hipLaunchKernelGGL(k_member_scatter, dim3((unsigned)nblk), dim3(256), 0, stream, ws[13].as<uint8_t>(), skeys,
+                           ws[12].as<double>(), np, rb, bend, bcount, res->indices.as<int64_t>(),
+                           res->has_dist ? res->dist.as<double>() : nullptr, mq);
+    // CSR row pointers
+    hipLaunchKernelGGL(k_indptr_search, dim3(blocks_for(m + 1)), dim3(256), 0, stream, mq, nnz, m, indptr);
+    MLK_CHECK_HIP(hipGetLastError());
     if (sort_rows) MLK_TRY(sort_rows_by_distance(res, indptr, m, nnz, ws, stream));
     MLK_CHECK_HIP(hipGetLastError());
     MLK_CHECK_HIP(hipStreamSynchronize(stream));

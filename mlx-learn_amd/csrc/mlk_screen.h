@@ -69,11 +69,20 @@ __host__ __device__ __forceinline__ double screen_row_bound(double nx, double nt
     return lo >= 0.0 ? lo * 0.5 / (1.0 + c.gamma) * (1.0 - 0x1p-50) : lo * 0.5 / (1.0 - c.gamma) * (1.0 + 0x1p-50);
 }
 
+// Lowest screen threshold. A safe row's accumulator has magnitude below 2^125
+// (the mirror's safe range bounds d |v|^2 by 2^124), an unsafe row's is -3e38
+// (its bias; its mirror values are zero): a threshold of THR_FLOOR passes every
+// safe row and no unsafe one, so unsafe rows reach a query only through the
+// unsafe-row lists, never twice.
+constexpr float THR_FLOOR = -1.0e38f;
+
 // Screen threshold of a query from its tau (an upper bound of its k-th smallest
-// sigma^2 S) and its qa: acc >= thr <=> the row may lie within tau.
+// sigma^2 S) and its qa: acc >= thr <=> the row may lie within tau. Thresholds
+// below THR_FLOOR (an infinite tau, or one beyond float32) are raised to it.
 __device__ __forceinline__ float thr_from_tau(double tau, double qa, const Consts& c) {
-    if (!(tau < INFINITY)) return -INFINITY;
-    return __double2float_rd((qa - tau * (1.0 + 0x1p-46)) * 0.5 - c.f0);
+    if (!(tau < INFINITY)) return THR_FLOOR;
+    const float t = __double2float_rd((qa - tau * (1.0 + 0x1p-46)) * 0.5 - c.f0);
+    return t > THR_FLOOR ? t : THR_FLOOR;
 }
 
 // Certified bounds of S (scaled units) from a stored hit: p = acc + a_x and the

@@ -201,12 +201,13 @@ class DeviceNeighborIndex:
         Qd = torch.cat(parts) if parts else torch.empty((0, self.n_features), device=self._device)
         indptr, indices, dist = self.radius(Qd, R2, sort_results=sort_results, need_distance=need_distance,
                                             precision=precision)
-        out, lo = [], 0
-        for sz in sizes:
-            ip = indptr[lo:lo + sz + 1]
-            a, b = int(ip[0]), int(ip[-1])
-            out.append((ip - a, indices[a:b], None if dist is None else dist[a:b]))
-            lo += sz
+        # every batch's first and last CSR position in one device-to-host read
+        bounds = torch.tensor([0] + sizes, dtype=torch.int64).cumsum(0)
+        ends = indptr[bounds.to(indptr.device)].cpu().tolist()
+        out = []
+        for i, sz in enumerate(sizes):
+            lo, a, b = int(bounds[i]), ends[i], ends[i + 1]
+            out.append((indptr[lo:lo + sz + 1] - a, indices[a:b], None if dist is None else dist[a:b]))
         return out
 
     def radius_reduce(self, Q, R2, *, codes=None, n_classes=None, targets=None, weighted=False,

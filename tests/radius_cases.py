@@ -215,8 +215,17 @@ def r2_sweep():
     X = X[r.permutation(len(X))]
     Q = np.concatenate([X[::4], r.normal(size=(8, 3)).astype(np.float32)])
     radii = [(0.0, "zero"), (2.0 ** -63, "2^-126"), (TINY, "2^-298"), (3e-165, "subnormal-r2"), (5e-324, "underflow"),
-             (1.25, "normal"), (1e32, "huge"), (1e150, "1e300"), (1e180, "overflow"), (math.inf, "inf")]
-    return [Case(X, Q, v, f"r2-{tag}") for v, tag in radii]
+             (1.25, "normal"), (1e32, "huge"), (1e45, "1e90"), (1e150, "1e300"), (1e180, "overflow"),
+             (math.inf, "inf")]
+    out = [Case(X, Q, v, f"r2-{tag}") for v, tag in radii]
+    # safe rows and queries beside rows outside the screen's safe range (their
+    # +-1e30 values cancel in the centre): at r = 1e45 every row is a member,
+    # the unsafe ones exactly once
+    huge = (r.normal(size=(6, 3)) * 1e30).astype(np.float32)
+    X = np.concatenate([r.normal(size=(60, 3)).astype(np.float32), huge, -huge])
+    Q = r.normal(size=(20, 3)).astype(np.float32)
+    out += [Case(X, Q, 1e45, "unsafe-rows-r1e45"), Case(X, Q, 2.0, "unsafe-rows-r2")]
+    return out
 
 
 def hubs():

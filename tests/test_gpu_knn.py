@@ -222,6 +222,21 @@ def test_unsafe_rows_and_queries(mlk, rng):
         assert_parity(gpu_knn(mlk, X, Q, 5, p), oracle.knn(X, Q, 5))
 
 
+def test_unsafe_rows_with_unbounded_tau(mlk):
+    """Fewer safe rows than k: a query's tau stays infinite, so its screen passes every safe row.
+    The unsafe rows (outside the screen's safe range) must still arrive once each, through the
+    unsafe-row list, never a second time through the screen."""
+    r = np.random.default_rng(77)
+    X = r.normal(size=(12, 4)).astype(np.float32)
+    huge = (r.normal(size=(4, 4)) * 1e30).astype(np.float32)
+    X[:8] = np.concatenate([huge, -huge])  # they cancel in the column means: the other rows stay safe
+    Q = r.normal(size=(5, 4)).astype(np.float32)
+    for p in PRECISIONS:
+        got = gpu_knn(mlk, X, Q, 6, p)
+        assert all(len(set(row)) == 6 for row in got[1].tolist())
+        assert_parity(got, oracle.knn(X, Q, 6))
+
+
 def test_exact_order_across_float32_range(mlk):
     """tests/unit/test_neighbors_precision.py:145-186: random bit patterns over the whole exponent range."""
     r = np.random.default_rng(9021)
