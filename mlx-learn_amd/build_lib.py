@@ -17,7 +17,7 @@ OUT_DIR = os.path.join(HERE, "mlxlearn_amd", "_lib")
 OUT = os.path.join(OUT_DIR, "libmlxlearn_amd.so")
 OBJ_DIR = os.path.join(HERE, "_build")
 SOURCES = ["mlk_capi.hip", "mlk_index.hip", "mlk_screen.hip", "mlk_knn.hip", "mlk_radius.hip", "mlk_kmeans.hip",
-           "mlk_tune.hip", "mlk_diag.hip", "mlk_noalloc.hip"]
+           "mlk_tune.hip", "mlk_diag.hip", "mlk_noalloc.hip", "mlk_order.hip"]
 # per-source extra flags: the screen's values are finite by construction, so
 # its max-tree needs no NaN canonicalisation
 EXTRA = {"mlk_screen.hip": ["-fno-honor-nans"]}

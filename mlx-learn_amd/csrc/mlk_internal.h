@@ -153,6 +153,10 @@ struct mlk_index {
     mlk::DevBuf ws[32];
     // captured k-NN call graphs (MLK_GRAPH_REPLAY), owned by mlk_knn.hip
     void* graphs = nullptr;
+    // query order centres (mlk_order.hip): order_k rows padded to a multiple
+    // of 32 floats, then their squared norms; order_k = 0 until first use
+    mlk::DevBuf order_c;
+    int order_k = 0;
 };
 
 struct mlk_radius_result {

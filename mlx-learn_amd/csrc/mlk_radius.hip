@@ -23,6 +23,7 @@
 
 #include "mlk_device.h"
 #include "mlk_internal.h"
+#include "mlk_order.h"
 #include "mlk_screen.h"
 
 namespace mlk {
@@ -85,23 +86,30 @@ __global__ void k_radius_thr(int64_t m, int64_t m_pad, const double* __restrict_
 }
 
 // Unsafe database rows never hit the screen; they are candidates of every query.
+// Pairs name queries by screen position; qmap (nullable) maps a position to the
+// query's index.
 __global__ void k_unsafe_pairs(int64_t m, const int* __restrict__ unsafe, int n_unsafe, int64_t self_off, int exclude,
-                               int2* __restrict__ pairs, unsigned long long* __restrict__ count,
-                               unsigned long long cap) {
+                               const int* __restrict__ qmap, int2* __restrict__ pairs,
+                               unsigned long long* __restrict__ count, unsigned long long cap) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= m * n_unsafe) return;
     const int64_t q = t / n_unsafe;
     const int row = unsafe[t % n_unsafe];
-    if (exclude && row == self_off + q) return;
+    if (exclude && row == self_off + (qmap ? (int64_t)qmap[q] : q)) return;
     const unsigned long long at = atomicAdd(count, 1ull);
     if (at < cap) pairs[at] = int2{(int)q, row};
 }
 
-// Pair keys (query << rb) | row, rb = bits of the row count: sorting them
-// orders pairs by (query, row) with as few key bits as the shape needs.
-__global__ void k_pair_keys(const int2* __restrict__ pairs, int64_t np, int rb, unsigned long long* __restrict__ keys) {
+// Pair keys (query << rb) | row, rb = bits of the row count, with the query's
+// index (qmap of its screen position when the queries were regrouped): sorting
+// them orders pairs by (query, row) with as few key bits as the shape needs.
+__global__ void k_pair_keys(const int2* __restrict__ pairs, int64_t np, int rb, const int* __restrict__ qmap,
+                            unsigned long long* __restrict__ keys) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < np) keys[i] = ((unsigned long long)(unsigned)pairs[i].x << rb) | (unsigned)pairs[i].y;
+    if (i >= np) return;
+    const int2 p = pairs[i];
+    const unsigned q = qmap ? (unsigned)qmap[p.x] : (unsigned)p.x;
+    keys[i] = ((unsigned long long)q << rb) | (unsigned)p.y;
 }
 
 // Membership and distance of every sorted candidate pair.
@@ -118,6 +126,67 @@ __global__ void k_classify(const unsigned long long* __restrict__ keys, int64_t 
     const float* qv = Q + q * d;
     const float* xv = X + row * d;
     const double s = VEC ? canonical_sq4(qv, xv, d) : canonical_sq(qv, xv, d);
+    bool in;
+    if (s <= rc.f_in) in = true;
+    else if (!(s <= rc.f_out)) in = false;
+    else in = exact_cmp_key(qv, xv, d, rc.key) <= 0;
+    member[i] = in;
+    pdist[i] = sqrt(s < rc.R2 ? s : rc.R2);
+}
+
+// k_classify for d % 4 == 0 and 16-byte aligned rows, with the database rows
+// of the block's 256 pairs staged through LDS in 32-feature chunks: eight
+// lanes load one row chunk (128 contiguous bytes) instead of one lane reading
+// a whole row in eight scattered pieces, a quarter of the cache-line
+// requests. Each thread then adds its pair's chunk from LDS in feature order,
+// so the sum is bitwise canonical_sq's. The query rows (shared by consecutive
+// pairs) are read from memory.
+constexpr int CLS_CHUNK = 32;
+
+__global__ __launch_bounds__(256) void k_classify_staged(const unsigned long long* __restrict__ keys, int64_t np,
+                                                         int rb, const float* __restrict__ X,
+                                                         const float* __restrict__ Q, int d, RadiusConsts rc,
+                                                         uint8_t* __restrict__ member, double* __restrict__ pdist) {
+    // rows padded to 36 floats: eight lanes reading 16 bytes each cover 32 banks once
+    __shared__ __attribute__((aligned(16))) float sx[256][CLS_CHUNK + 4];
+    __shared__ int64_t srow[256];
+    const int t = threadIdx.x;
+    const int64_t i0 = (int64_t)blockIdx.x * 256, i = i0 + t;
+    const bool live = i < np;
+    const unsigned long long key = live ? keys[i] : 0ull;
+    const int64_t q = (int64_t)(key >> rb);
+    const int64_t row = (int64_t)(key & ((1ull << rb) - 1));
+    srow[t] = row;
+    const float* qv = Q + q * d;
+    double s = 0.0;
+    for (int c0 = 0; c0 < d; c0 += CLS_CHUNK) {
+        __syncthreads();  // srow written / the previous chunk consumed
+#pragma unroll
+        for (int k = 0; k < CLS_CHUNK / 4; ++k) {
+            const int idx = t + 256 * k;
+            const int r = idx / (CLS_CHUNK / 4), part = idx % (CLS_CHUNK / 4);
+            const int f = c0 + 4 * part;
+            float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (i0 + r < np && f < d) v = *reinterpret_cast<const float4*>(X + srow[r] * d + f);
+            *reinterpret_cast<float4*>(&sx[r][4 * part]) = v;
+        }
+        __syncthreads();
+        const int nf = d - c0 < CLS_CHUNK ? d - c0 : CLS_CHUNK;
+        for (int e = 0; e < nf; e += 4) {
+            const float4 a = *reinterpret_cast<const float4*>(qv + c0 + e);
+            const float4 b = *reinterpret_cast<const float4*>(&sx[t][e]);
+            double u = (double)a.x - (double)b.x;
+            s = add_rn(s, mul_rn(u, u));
+            u = (double)a.y - (double)b.y;
+            s = add_rn(s, mul_rn(u, u));
+            u = (double)a.z - (double)b.z;
+            s = add_rn(s, mul_rn(u, u));
+            u = (double)a.w - (double)b.w;
+            s = add_rn(s, mul_rn(u, u));
+        }
+    }
+    if (!live) return;
+    const float* xv = X + row * d;
     bool in;
     if (s <= rc.f_in) in = true;
     else if (!(s <= rc.f_out)) in = false;
@@ -518,7 +587,19 @@ int radius_query_impl(mlk_index* index, const float* Q, int64_t m, double R2, in
     MLK_TRY(ws[26].reserve((size_t)m_pad * sizeof(double)));  // |q - q~|^2
     unsigned* dflag = ws[6].as<unsigned>();
     MLK_CHECK_HIP(hipMemsetAsync(dflag, 0, 16, stream));
-    MLK_TRY(mirror_rows(prec, Q, m, m_pad, d, dp, index->centre.as<float>(), mr.scale, ws[0].ptr, ws[1].as<double>(),
+    // Query locality order: regrouped queries share the screen's tiles, and
+    // the pairs carry positions that qmap turns back into query indices.
+    const float* Qs = Q;  // the queries in screen order
+    const int* qmap = nullptr;
+    if (order_wanted(index, m)) {
+        MLK_TRY(ws[27].reserve(order_scratch_bytes(m)));
+        MLK_TRY(ws[28].reserve((size_t)m_pad * sizeof(int)));
+        MLK_TRY(ws[29].reserve((size_t)m * d * sizeof(float) + 16));
+        MLK_TRY(query_order(index, Q, m, m_pad, ws[27].ptr, ws[28].as<int>(), ws[29].as<float>(), stream));
+        Qs = ws[29].as<float>();
+        qmap = ws[28].as<int>();
+    }
+    MLK_TRY(mirror_rows(prec, Qs, m, m_pad, d, dp, index->centre.as<float>(), mr.scale, ws[0].ptr, ws[1].as<double>(),
                         ws[2].as<uint8_t>(), dflag, ws[25].as<double>(), ws[26].as<double>(), stream));
     unsigned q_inexact = 1;
     MLK_CHECK_HIP(hipMemcpyAsync(&q_inexact, dflag, 4, hipMemcpyDeviceToHost, stream));
@@ -564,6 +645,7 @@ int radius_query_impl(mlk_index* index, const float* Q, int64_t m, double R2, in
             sa.nchunk = screen_chunks(prec, dp);
             sa.qgroup = sa.nchunk > 1 ? DEEP_QGROUP : 1;
             sa.pairs = ws[9].as<int2>();
+            sa.qmap = qmap;
             sa.pair_count = dcount;
             sa.pair_cap = cap;
             hipLaunchKernelGGL(screen, dim3((unsigned)(QB * S)), dim3(256), 0, stream, sa);
@@ -575,8 +657,8 @@ int radius_query_impl(mlk_index* index, const float* Q, int64_t m, double R2, in
         MLK_CHECK_HIP(hipEventRecord(e1, stream));
         if (mr.n_unsafe) {
             hipLaunchKernelGGL(k_unsafe_pairs, dim3(blocks_for(m * mr.n_unsafe)), dim3(256), 0, stream, m,
-                               mr.unsafe.as<int>(), (int)mr.n_unsafe, q_offset, (int)exclude, ws[9].as<int2>(), dcount,
-                               cap);
+                               mr.unsafe.as<int>(), (int)mr.n_unsafe, q_offset, (int)exclude, qmap, ws[9].as<int2>(),
+                               dcount, cap);
             MLK_CHECK_HIP(hipGetLastError());
         }
         MLK_CHECK_HIP(hipMemcpyAsync(&count, dcount, sizeof(count), hipMemcpyDeviceToHost, stream));
@@ -604,7 +686,8 @@ int radius_query_impl(mlk_index* index, const float* Q, int64_t m, double R2, in
     unsigned long long* keys = ws[10].as<unsigned long long>();
     unsigned long long* skeys = ws[11].as<unsigned long long>();
     if (np) {
-        hipLaunchKernelGGL(k_pair_keys, dim3(blocks_for(np)), dim3(256), 0, stream, ws[9].as<int2>(), np, rb, keys);
+        hipLaunchKernelGGL(k_pair_keys, dim3(blocks_for(np)), dim3(256), 0, stream, ws[9].as<int2>(), np, rb, qmap,
+                           keys);
         int qbits = 1;
         while ((1ll << qbits) < m) ++qbits;
         size_t tmp_bytes = 0;
@@ -612,8 +695,12 @@ int radius_query_impl(mlk_index* index, const float* Q, int64_t m, double R2, in
         MLK_TRY(ws[14].reserve(tmp_bytes + 16));
         MLK_CHECK_HIP(hipcub::DeviceRadixSort::SortKeys(ws[14].ptr, tmp_bytes, keys, skeys, (int)np, 0, rb + qbits, stream));
         const bool vec = d % 4 == 0 && ((uintptr_t)Q % 16) == 0 && ((uintptr_t)X % 16) == 0;
-        hipLaunchKernelGGL(vec ? k_classify<true> : k_classify<false>, dim3(blocks_for(np)), dim3(256), 0, stream,
-                           skeys, np, rb, X, Q, d, rc, ws[13].as<uint8_t>(), ws[12].as<double>());
+        if (vec)
+            hipLaunchKernelGGL(k_classify_staged, dim3(blocks_for(np)), dim3(256), 0, stream, skeys, np, rb, X, Q, d,
+                               rc, ws[13].as<uint8_t>(), ws[12].as<double>());
+        else
+            hipLaunchKernelGGL(k_classify<false>, dim3(blocks_for(np)), dim3(256), 0, stream, skeys, np, rb, X, Q, d,
+                               rc, ws[13].as<uint8_t>(), ws[12].as<double>());
         MLK_CHECK_HIP(hipGetLastError());
     }
     // compact the members (order preserved): block counts, their inclusive

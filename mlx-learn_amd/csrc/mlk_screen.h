@@ -161,6 +161,9 @@ struct ScreenArgs {
     // with xcd_map: query blocks per group of an XCD's consecutive blocks
     // (0 or 1: every query block of a split before the next split)
     int qgroup;
+    // radius mode: query position -> original query index (nullptr: the
+    // identity), for the self-exclusion test of regrouped queries
+    const int* qmap;
 };
 
 constexpr int WIDE_MAX = 800;    // widest row whose query fragments stay in registers (bf16 / fp16)

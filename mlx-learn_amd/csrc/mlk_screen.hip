@@ -391,7 +391,8 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
                 for (int qt = 0; qt < QT; ++qt) {
                     if (!ballot(hit_rq[rt][qt])) continue;
                     const int64_t qg = qrow0 + qt * 32 + col;
-                    const int64_t self = a.exclude ? a.self_off + qg : -1;
+                    int64_t self = -1;
+                    if (a.exclude) self = a.self_off + (MODE == SCREEN_PAIRS && a.qmap ? (int64_t)a.qmap[qg] : qg);
                     if constexpr (MODE == SCREEN_LISTS) {
                         // wave-uniform mask of the accumulator registers with a hit
                         // in some lane; only those registers are visited (uniform

@@ -58,10 +58,11 @@ Plan heuristic_plan(int kind, int prec, int dp, int64_t n, int64_t m, int k, Scr
         p.cap = knn_capacity(prec, p.splits, k);
         p.rounds = default_rounds();
     } else {
-        // one pass; about 32 blocks per CU so that the queries with many
-        // neighbours (dense clumps) do not leave a long tail (tuned 2M x 32,
-        // 132,759 queries: 8 splits)
-        p.splits = round_splits((8192 + qb - 1) / qb, n);
+        // one pass; about 128 blocks per CU so that the query blocks with many
+        // neighbours (dense clumps, grouped together by the query order) do
+        // not leave a long tail (2M x 32, 132,759 regrouped queries: 32 splits,
+        // a 21.5 ms screen against 24 ms at 8)
+        p.splits = round_splits((32768 + qb - 1) / qb, n);
     }
     return p;
 }

@@ -29,6 +29,8 @@ HOOK_CAPACITY = 0x2
 HOOK_FAIL = 0x4
 HOOK_FORCE_UNCERTIFIED = 0x8
 HOOK_SORT_LIMIT = 0x10
+HOOK_FORCE_ORDER = 0x20
+HOOK_NO_ORDER = 0x40
 TUNE_RADIUS = 0x1000
 MAX_ROUNDS = 8
 

@@ -50,21 +50,59 @@ def assert_csr(got, want, case=""):
         np.testing.assert_allclose(gd, wd, rtol=1e-12, atol=0, err_msg=f"{case}: distances")
 
 
+class _Hooks:
+    """Test hooks for the duration of a with-block (always cleared on exit)."""
+
+    def __init__(self, hooks, param=0):
+        self.hooks, self.param = hooks, param
+
+    def __enter__(self):
+        from mlxlearn_amd import _lib
+        _lib.set_test_hooks(self.hooks, self.param)
+
+    def __exit__(self, *exc):
+        from mlxlearn_amd import _lib
+        _lib.set_test_hooks(0)
+
+
+def _order_hooks(order):
+    from mlxlearn_amd import _lib
+    return {"caller": _lib.HOOK_NO_ORDER, "regrouped": _lib.HOOK_FORCE_ORDER}[order]
+
+
+@pytest.mark.parametrize("order", ["caller", "regrouped"])
 @pytest.mark.parametrize("precision", PRECISIONS)
 @pytest.mark.parametrize("case", CASES, ids=[c.name for c in CASES])
-def test_adversarial_cases(mlk, case, precision):
-    """tests/_radius_oracle.py families, unsorted and sorted rows (radius_result.py:130-147)."""
+def test_adversarial_cases(mlk, case, precision, order):
+    """tests/_radius_oracle.py families, unsorted and sorted rows (radius_result.py:130-147), with
+    the queries screened in the caller's order and regrouped by locality."""
     X = np.ascontiguousarray(case.X, np.float32)
     Q = np.ascontiguousarray(case.Q, np.float32)
     R2 = float(case.radius) * float(case.radius)
     want = oracle.radius(X, Q, R2)
-    assert_csr(gpu_radius(mlk, X, Q, R2, precision), want, case.name)
     from mlxlearn_amd import _lib
-    st = _lib.stats()
-    if st["rounds"] or st["candidates"]:  # a screened call (not the everything-in-range shortcut)
-        assert st["screen_precision"] == _lib.PRECISIONS[precision]
-    want_sorted = oracle.radius(X, Q, R2, sort_results=True)
-    assert_csr(gpu_radius(mlk, X, Q, R2, precision, sort_results=True), want_sorted, case.name + " sorted")
+    with _Hooks(_order_hooks(order)):
+        assert_csr(gpu_radius(mlk, X, Q, R2, precision), want, case.name)
+        st = _lib.stats()
+        if st["rounds"] or st["candidates"]:  # a screened call (not the everything-in-range shortcut)
+            assert st["screen_precision"] == _lib.PRECISIONS[precision]
+        want_sorted = oracle.radius(X, Q, R2, sort_results=True)
+        assert_csr(gpu_radius(mlk, X, Q, R2, precision, sort_results=True), want_sorted, case.name + " sorted")
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_regrouped_self_queries_with_unsafe_rows(mlk, precision, rng):
+    """Regrouped self-queries: the screen and the unsafe-row pairs exclude each query's own row
+    through the position -> query map (rows outside the screen's safe range included)."""
+    from mlxlearn_amd import _lib
+    huge = (rng.normal(size=(5, 6)) * 1e30).astype(np.float32)
+    X = np.concatenate([rng.normal(size=(3000, 6)).astype(np.float32), huge, -huge])
+    X[200:204] = X[199]
+    for R2 in (2.5, 1e90):
+        want = oracle.radius(X, X, R2, exclude_self=True, sort_results=True)
+        with _Hooks(_lib.HOOK_FORCE_ORDER):
+            got = gpu_radius(mlk, X, X, R2, precision, exclude_self=True, sort_results=True)
+        assert_csr(got, want, f"R2={R2}")
 
 
 @pytest.mark.parametrize("precision", PRECISIONS)
