@@ -62,8 +62,9 @@ typedef struct mlk_radius_result mlk_radius_result;
 #define MLK_HOOK_FORCE_UNCERTIFIED 0x8u /* rows listed by mlk_set_forced_rows lose certificates */
 #define MLK_HOOK_SORT_LIMIT 0x10u      /* radius: counts sorted or selected in one pass must be <= param
                                           (default INT32_MAX, the limit of the library sorts)       */
-#define MLK_HOOK_FORCE_ORDER 0x20u     /* radius: regroup the queries by locality at any call size  */
-#define MLK_HOOK_NO_ORDER 0x40u        /* radius: keep the caller's query order at any call size    */
+#define MLK_HOOK_FORCE_ORDER 0x20u     /* regroup the queries by locality at any call size (k-NN
+                                          calls without self-exclusion, radius calls)               */
+#define MLK_HOOK_NO_ORDER 0x40u        /* keep the caller's query order at any call size            */
 
 /* Per-call statistics of the last query on this thread. */
 typedef struct {

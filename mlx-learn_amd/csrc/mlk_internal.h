@@ -153,10 +153,16 @@ struct mlk_index {
     mlk::DevBuf ws[32];
     // captured k-NN call graphs (MLK_GRAPH_REPLAY), owned by mlk_knn.hip
     void* graphs = nullptr;
-    // query order centres (mlk_order.hip): order_k rows padded to a multiple
-    // of 32 floats, then their squared norms; order_k = 0 until first use
-    mlk::DevBuf order_c;
+    // query order centres (mlk_order.hip): order_k database rows as bf16
+    // (padded to a multiple of 32 centres and of 16 features) and their
+    // float32 squared norms (+inf for the padding); order_k = 0 until first use
     int order_k = 0;
+    mlk::DevBuf order_cb, order_cn;
+    // database rows grouped by nearest order centre (ascending within a
+    // centre), each centre's first position (order_k + 1 entries) and each
+    // row's centre
+    mlk::DevBuf order_rows, order_start, order_cell;
+    bool order_rows_ready = false;
 };
 
 struct mlk_radius_result {

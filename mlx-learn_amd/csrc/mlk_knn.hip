@@ -38,6 +38,7 @@
 
 #include "mlk_device.h"
 #include "mlk_internal.h"
+#include "mlk_order.h"
 #include "mlk_screen.h"
 
 namespace mlk {
@@ -101,7 +102,7 @@ __global__ __launch_bounds__(256) void k_tau0(const float* __restrict__ X, int64
                                               Consts c, float* __restrict__ tau, float* __restrict__ thr,
                                               int* __restrict__ status, int S2, int* __restrict__ cnt,
                                               int* __restrict__ seen, float* __restrict__ kbest,
-                                              int* __restrict__ kbest_row) {
+                                              int* __restrict__ kbest_row, const int* __restrict__ srows) {
     constexpr int FL = 32;      // features staged per pass
     constexpr int FC = 8;       // features per register chunk
     constexpr int NS = 64 * U;  // sample slots of the block
@@ -129,7 +130,9 @@ __global__ __launch_bounds__(256) void k_tau0(const float* __restrict__ X, int64
         }
     }
     const int samples = n < max_samples ? (int)n : max_samples;  // <= NS
-    const int64_t stride = n / samples;  // sample j is row j * stride
+    // sample j is row j * stride, or srows[j] of this block's list (distinct rows)
+    const int64_t stride = n / samples;
+    const int* srow = srows ? srows + (int64_t)blockIdx.x * samples : nullptr;
     // staging registers: value i of this thread is sample j0 + 8 i, feature e of the
     // pass (j0 = threadIdx.x / 32, e = threadIdx.x % 32); query value i of this lane is
     // query qw0 + 2 i of the wave, feature e (qw0 = lane / 32)
@@ -145,10 +148,19 @@ __global__ __launch_bounds__(256) void k_tau0(const float* __restrict__ X, int64
         int64_t xo = x_first + f0, qo = q_first + f0;
         asm volatile("" : "+v"(xo), "+v"(qo));
         const bool fe = f0 + e_st < d;
+        if (srow) {
 #pragma unroll
-        for (int i = 0; i < PER; ++i) {
-            const bool ok = fe && j0 + (256 / FL) * i < samples;
-            stg[i] = X[ok ? xo + i * row_step : 0];  // row 0 stands in for padding (zeroed at the store)
+            for (int i = 0; i < PER; ++i) {
+                const int j = j0 + (256 / FL) * i;
+                const int64_t r = j < samples ? (int64_t)srow[j] : -1;  // -1: no sample
+                stg[i] = X[fe && r >= 0 ? r * d + f0 + e_st : 0];
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < PER; ++i) {
+                const bool ok = fe && j0 + (256 / FL) * i < samples;
+                stg[i] = X[ok ? xo + i * row_step : 0];  // row 0 stands in for padding (zeroed at the store)
+            }
         }
 #pragma unroll
         for (int i = 0; i < QS; ++i) {
@@ -221,9 +233,9 @@ __global__ __launch_bounds__(256) void k_tau0(const float* __restrict__ X, int64
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int j = lane + 64 * u;
-            const int64_t row = (int64_t)j * stride;
+            const int64_t row = j < samples ? (srow ? (int64_t)srow[j] : (int64_t)j * stride) : -1;
             bits[u] = 0x7f800000u;
-            if (j < samples && !(exclude && row == self_off + q))
+            if (row >= 0 && !(exclude && row == self_off + q))
                 bits[u] = __float_as_uint(__double2float_ru(((double)acc[qi][u] + pad) * rel * c.scale2));
         }
         uint32_t kb;
@@ -1306,6 +1318,11 @@ struct KnnCall {
     int* fq = nullptr;
     unsigned* dflag = nullptr;
     bool big_rank = false;  // k_rank ran: queries marked ST_BIG are ranked again
+    // regrouped queries: the chain's results (idx, dist) are by screen
+    // position; knn_complete moves row p to the caller's row qperm[p]
+    const int* qperm = nullptr;
+    int64_t* user_idx = nullptr;
+    double* user_dist = nullptr;
     RankArgs ra{};
     Consts c{};
     int64_t rounds = 0, splits = 0, capacity = 0;
@@ -1378,6 +1395,34 @@ static int knn_enqueue(mlk_index* index, const float* Q, int64_t m, int k, int64
     // flag words: [0] query rounding inexact, [2] fallback count, [4..10) three
     // 64-bit statistics sums (cleared by a kernel: graphs replay no memset nodes)
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, stream, (int*)dflag, (int64_t)10, 0);
+    // Query locality order (calls without self-exclusion that may allocate):
+    // the chain below runs on the regrouped queries, the start threshold of
+    // each block of them samples rows of their own neighbourhood, and the
+    // results are written by position for knn_complete to move to the caller's
+    // rows. A captured call regroups only if an eager call prepared the rows.
+    int64_t* const idx_caller = idx;
+    double* const dist_caller = dist;
+    const int* qperm = nullptr;
+    const int* qcell = nullptr;
+    if (!exclude && !fixed && order_wanted(index, m) && (!capture || index->order_rows_ready)) {
+        void* oscr;
+        int* qp;
+        float* Qp;
+        MLK_TRY(mem.get(25, order_scratch_bytes(m), &oscr));
+        MLK_TRY(mem.get(26, (size_t)m_pad * sizeof(int), &qp));
+        MLK_TRY(mem.get(27, (size_t)m * d * sizeof(float) + 16, &Qp));
+        MLK_TRY(prepare_order_rows(index, stream));
+        MLK_TRY(query_order(index, Q, m, m_pad, oscr, qp, Qp, stream));
+        int64_t* idx_pos;
+        MLK_TRY(mem.get(28, (size_t)m * k * sizeof(int64_t), &idx_pos));
+        double* dist_pos = nullptr;
+        if (dist) MLK_TRY(mem.get(29, (size_t)m * k * sizeof(double), &dist_pos));
+        Q = Qp;
+        idx = idx_pos;
+        dist = dist_pos;
+        qperm = qp;
+        qcell = static_cast<const int*>(oscr);
+    }
     MLK_TRY(mirror_rows(prec, Q, m, m_pad, d, dp, index->centre.as<float>(), mr.scale, qmirror, nq, qflags, dflag,
                         qnt, qd2, stream));
     // the queries' rounding only matters when every database value is exact
@@ -1416,11 +1461,20 @@ static int knn_enqueue(mlk_index* index, const float* Q, int64_t m, int k, int64
         // U sample rows per lane (64 U samples), QW queries per wave: U * QW = 32
         // float64 accumulators per lane
         const int samples = (int)std::min<int64_t>(n, tau_samples);
+        int launch_rc = MLK_OK;
         auto launch = [&](auto kern, int qw) {
             const int64_t waves = (m_pad + qw - 1) / qw;
-            hipLaunchKernelGGL(kern, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, stream, X, n, d, Q, m, m_pad,
+            const int64_t blocks = (waves + 3) / 4;
+            int* srows = nullptr;  // regrouped queries: each block's sample rows
+            if (qperm) {
+                launch_rc = mem.get(30, (size_t)blocks * samples * sizeof(int), &srows);
+                if (launch_rc == MLK_OK)
+                    launch_rc = tau_sample_rows(index, qcell, qperm, m, 4 * qw, blocks, samples, srows, stream);
+                if (launch_rc != MLK_OK) return;
+            }
+            hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(256), 0, stream, X, n, d, Q, m, m_pad,
                                k, tau_samples, q_offset, (int)exclude, qa, qflags, c, tau, thr, status, S2,
-                               all_fallback ? nullptr : cnt, seen, kbest, kbest_row);
+                               all_fallback ? nullptr : cnt, seen, kbest, kbest_row, srows);
         };
         // the most queries per wave (register reuse of the samples) that still
         // leaves about 4096 waves (4 per SIMD) to fill the device
@@ -1448,6 +1502,7 @@ static int knn_enqueue(mlk_index* index, const float* Q, int64_t m, int k, int64
             else if (qw >= 2) launch(k_tau0<4, 2>, 2);
             else launch(k_tau0<4, 1>, 1);
         }
+        MLK_TRY(launch_rc);
     }
     MLK_CHECK_HIP(hipGetLastError());
 
@@ -1567,7 +1622,20 @@ static int knn_enqueue(mlk_index* index, const float* Q, int64_t m, int k, int64
     call.flags = flags; call.exclude = exclude; call.Q = Q; call.X = X; call.dist = dist; call.idx = idx;
     call.status = status; call.fq = fq; call.dflag = dflag; call.big_rank = big_rank; call.ra = ra; call.c = c;
     call.rounds = st.rounds; call.splits = S; call.capacity = cap;
+    call.qperm = qperm; call.user_idx = idx_caller; call.user_dist = dist_caller;
     return MLK_OK;
+}
+
+// Regrouped queries: result row p (by screen position) to the caller's row qperm[p].
+__global__ void k_unpermute_results(const int* __restrict__ qperm, int64_t m, int k, const int64_t* __restrict__ idx,
+                                    const double* __restrict__ dist, int64_t* __restrict__ idx_out,
+                                    double* __restrict__ dist_out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m * k) return;
+    const int64_t p = t / k, j = t - p * k;
+    const int64_t o = (int64_t)qperm[p] * k + j;
+    idx_out[o] = idx[t];
+    if (dist) dist_out[o] = dist[t];
 }
 
 // Completion of an enqueued call: one read-back of the fallback count and the
@@ -1686,6 +1754,11 @@ static int knn_complete(mlk_index* index, const KnnCall& call, hipStream_t strea
             hipLaunchKernelGGL(k_rank, dim3((unsigned)((nb + 3) / 4)), dim3(256), 0, stream, fa, nb);
             MLK_CHECK_HIP(hipGetLastError());
         }
+    }
+    if (call.qperm) {
+        hipLaunchKernelGGL(k_unpermute_results, dim3((unsigned)((m * k + 255) / 256)), dim3(256), 0, stream,
+                           call.qperm, m, k, idx, dist, call.user_idx, call.user_dist);
+        MLK_CHECK_HIP(hipGetLastError());
     }
     if (!events) {
         // no events: the outputs are ordered on the stream; the fallback's host

@@ -28,4 +28,17 @@ size_t order_scratch_bytes(int64_t m);
 int query_order(mlk_index* index, const float* Q, int64_t m, int64_t m_pad, void* scratch, int* qperm, float* Qp,
                 hipStream_t stream);
 
+// The database rows grouped by nearest order centre (index->order_rows,
+// order_start), built on first use: the start-threshold samples of regrouped
+// k-NN queries come from their own neighbourhoods.
+int prepare_order_rows(mlk_index* index, hipStream_t stream);
+
+// Start-threshold sample rows of regrouped queries: block b of the start
+// threshold kernel (queries at positions b * qpb .. + qpb - 1) samples up to
+// `samples` distinct rows, srows[b * samples + j] (-1: no sample): rows of the
+// centres of its queries, and rows at a stride. `cell` is query_order's
+// per-query centre (the scratch's first m ints).
+int tau_sample_rows(const mlk_index* index, const int* cell, const int* qperm, int64_t m, int qpb, int64_t blocks,
+                    int samples, int* srows, hipStream_t stream);
+
 }  // namespace mlk

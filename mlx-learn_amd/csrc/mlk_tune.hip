@@ -21,6 +21,7 @@
 #include <string>
 
 #include "mlk_internal.h"
+#include "mlk_order.h"
 #include "mlk_screen.h"
 
 namespace mlk {
@@ -56,7 +57,11 @@ Plan heuristic_plan(int kind, int prec, int dp, int64_t n, int64_t m, int k, Scr
         // sub-lists to merge and rank (tuned 1M x 128, 10,000 queries: 32 splits)
         p.splits = round_splits((2560 + qb - 1) / qb, n);
         p.cap = knn_capacity(prec, p.splits, k);
-        p.rounds = default_rounds();
+        // calls large enough to be regrouped start from a threshold sampled in
+        // the queries' own neighbourhoods: four rounds (tuned 1M x 128, 10,000
+        // queries: 1/64, 1/8, 1/2, 1)
+        const bool regrouped = m >= ORDER_MIN_QUERIES && n >= ORDER_MIN_ROWS && dp <= ORDER_MAX_D;
+        p.rounds = regrouped ? std::vector<double>{1.0 / 64, 1.0 / 8, 1.0 / 2, 1.0} : default_rounds();
     } else {
         // one pass; about 128 blocks per CU so that the query blocks with many
         // neighbours (dense clumps, grouped together by the query order) do
@@ -404,14 +409,13 @@ int autotune_impl(mlk_index* index, const float* Q, int64_t m, int k, double R2,
             p.rounds = rs;
             MLK_TRY(check(consider(p)));
         }
-        // calls of well under a millisecond are bound by launches: fewer rounds
-        // (fewer screen and merge launches) can win despite a looser first threshold
-        if (best_ms < 1.0) {
-            for (const auto& rs : std::vector<std::vector<double>>{{1.0 / 4, 1.0}, {1.0}}) {
-                Plan p = best;
-                p.rounds = rs;
-                MLK_TRY(check(consider(p)));
-            }
+        // one or two rounds: they win where the start threshold is already
+        // close (regrouped queries sample their own neighbourhood) and where
+        // calls of well under a millisecond are bound by launches
+        for (const auto& rs : std::vector<std::vector<double>>{{1.0 / 8, 1.0}, {1.0 / 4, 1.0}, {1.0}}) {
+            Plan p = best;
+            p.rounds = rs;
+            MLK_TRY(check(consider(p)));
         }
         // 4. capacity, when candidate lists overflowed into the exact fallback
         for (int step = 0; step < 2 && best_fb > 0; ++step) {

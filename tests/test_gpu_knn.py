@@ -222,6 +222,54 @@ def test_unsafe_rows_and_queries(mlk, rng):
         assert_parity(gpu_knn(mlk, X, Q, 5, p), oracle.knn(X, Q, 5))
 
 
+def _clustered(rng, n, m, d, centres=12):
+    c = rng.normal(size=(centres, d)) * 4
+    X = (c[rng.integers(0, centres, n)] + rng.normal(size=(n, d))).astype(np.float32)
+    Q = (c[rng.integers(0, centres, m)] + rng.normal(size=(m, d))).astype(np.float32)
+    Q[:5] = X[:5]  # exact duplicates of database rows
+    return X, Q
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize("k", [1, 5, 32])
+def test_regrouped_queries(mlk, precision, k, rng):
+    """Queries regrouped by locality (forced at this size): the start thresholds sample the
+    queries' own neighbourhoods and the results return to the caller's rows."""
+    from mlxlearn_amd import _lib
+    X, Q = _clustered(rng, 6000, 700, 24)
+    _lib.set_test_hooks(_lib.HOOK_FORCE_ORDER)
+    try:
+        got = gpu_knn(mlk, X, Q, k, precision)
+        idx = mlk.DeviceNeighborIndex(X)
+        sq, si = idx.query(Q, k, precision=precision, squared=True)
+        _, ni = idx.query(Q, k, precision=precision, return_distance=False)
+        idx.close()
+    finally:
+        _lib.set_test_hooks(0)
+    want = oracle.knn(X, Q, k)
+    assert_parity(got, want)
+    np.testing.assert_array_equal(si.cpu().numpy(), want[1])
+    np.testing.assert_array_equal(ni.cpu().numpy(), want[1])
+    np.testing.assert_allclose(np.sqrt(sq.cpu().numpy()), want[0], rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("hook", ["fallback", "capacity"])
+def test_regrouped_queries_exact_fallback(mlk, hook, rng):
+    """Regrouped queries through the exact fallback (all of them, or those whose candidate
+    lists overflow a capacity of one slot)."""
+    from mlxlearn_amd import _lib
+    X, Q = _clustered(rng, 5000, 300, 16)
+    extra = _lib.HOOK_FORCE_FALLBACK if hook == "fallback" else _lib.HOOK_CAPACITY
+    _lib.set_test_hooks(_lib.HOOK_FORCE_ORDER | extra, 1)
+    try:
+        got = gpu_knn(mlk, X, Q, 7, "bfloat16")
+        st = _lib.stats()
+    finally:
+        _lib.set_test_hooks(0)
+    assert st["fallback_queries"] > 0
+    assert_parity(got, oracle.knn(X, Q, 7))
+
+
 def test_unsafe_rows_with_unbounded_tau(mlk):
     """Fewer safe rows than k: a query's tau stays infinite, so its screen passes every safe row.
     The unsafe rows (outside the screen's safe range) must still arrive once each, through the
