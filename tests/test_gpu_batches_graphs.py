@@ -134,3 +134,34 @@ def test_graph_survives_workspace_regrowth_and_fallback(mlk):
             big = torch.from_numpy(rng.normal(size=(20_000, 32)).astype(np.float32)).cuda()
             index.query(big, 9, precision="bfloat16")
     index.close()
+
+
+def test_graph_replays_of_regrouped_queries(mlk):
+    """A call large enough to be regrouped by locality (5,000 queries, 70,000 rows), replayed from
+    its captured graph: the regrouping kernels and the move back to the caller's rows are part of
+    the replay; every replay equals the eager call and the oracle, including one with a query
+    outside the screen's safe range (exact fallback after the replay)."""
+    import torch
+    rng = np.random.default_rng(9)
+    c = rng.normal(size=(40, 12)) * 3
+    X = (c[rng.integers(0, 40, 70_000)] + rng.normal(size=(70_000, 12))).astype(np.float32)
+    Q = (c[rng.integers(0, 40, 5_000)] + rng.normal(size=(5_000, 12))).astype(np.float32)
+    index = mlk.DeviceNeighborIndex(torch.from_numpy(X).cuda())
+    Qd = torch.from_numpy(Q).cuda()
+    out = (torch.empty((5_000, 6), dtype=torch.float64, device="cuda"),
+           torch.empty((5_000, 6), dtype=torch.int64, device="cuda"))
+    for rep in range(4):
+        if rep == 2:
+            Q[17] *= np.float32(1e30)
+            Qd.copy_(torch.from_numpy(Q))
+        want_d, want_i = index.query(Qd, 6, precision="bfloat16")
+        torch.cuda.synchronize()
+        out[1].fill_(-1)
+        index.query(Qd, 6, precision="bfloat16", graph=True, out=out)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(out[1].cpu().numpy(), want_i.cpu().numpy(), err_msg=f"replay {rep}")
+        np.testing.assert_array_equal(out[0].cpu().numpy(), want_d.cpu().numpy(), err_msg=f"replay {rep}")
+    wd, wi = oracle.knn(X, Q, 6)
+    np.testing.assert_array_equal(out[1].cpu().numpy(), wi)
+    np.testing.assert_allclose(out[0].cpu().numpy(), wd, rtol=1e-12, atol=0)
+    index.close()
