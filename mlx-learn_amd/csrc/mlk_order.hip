@@ -22,7 +22,7 @@ bool order_wanted(const mlk_index* index, int64_t m) {
     if (hk.hooks & MLK_HOOK_NO_ORDER) return false;
     if (index->d > ORDER_MAX_D || index->n < 2) return false;
     if (hk.hooks & MLK_HOOK_FORCE_ORDER) return true;
-    return m >= ORDER_MIN_QUERIES && index->n >= ORDER_MIN_ROWS;
+    return m >= ORDER_MIN_QUERIES && m <= ORDER_MAX_QUERIES && index->n >= ORDER_MIN_ROWS;
 }
 
 // padded sizes of the bf16 centre copy: centres to a multiple of 32 (the

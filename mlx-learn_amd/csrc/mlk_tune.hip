@@ -60,7 +60,8 @@ Plan heuristic_plan(int kind, int prec, int dp, int64_t n, int64_t m, int k, Scr
         // calls large enough to be regrouped start from a threshold sampled in
         // the queries' own neighbourhoods: four rounds (tuned 1M x 128, 10,000
         // queries: 1/64, 1/8, 1/2, 1)
-        const bool regrouped = m >= ORDER_MIN_QUERIES && n >= ORDER_MIN_ROWS && dp <= ORDER_MAX_D;
+        const bool regrouped = m >= ORDER_MIN_QUERIES && m <= ORDER_MAX_QUERIES && n >= ORDER_MIN_ROWS &&
+                               dp <= ORDER_MAX_D;
         p.rounds = regrouped ? std::vector<double>{1.0 / 64, 1.0 / 8, 1.0 / 2, 1.0} : default_rounds();
     } else {
         // one pass; about 128 blocks per CU so that the query blocks with many
