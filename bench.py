@@ -124,7 +124,11 @@ def main():
                     help="c2 (default) is the headline; c1, c3, c4, c5 are the other configurations; "
                          "deep is 1,024-feature rows (the deep screen)")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
-                    help="time the heuristic plans instead of autotuned ones")
+                    help="time the heuristic plans instead of autotuned ones (plans imported from "
+                         "MLXLEARN_PLAN_FILE are still used)")
+    ap.add_argument("--export-plans", default="", metavar="PATH",
+                    help="write the process's tuned plans to PATH at the end (for MLXLEARN_PLAN_FILE, so that "
+                         "profiled runs repeat them without autotuning under the profiler)")
     ap.add_argument("--mfma-calibration", action="store_true",
                     help="only run the library's register-only MFMA probe (MFMA counter calibration) and exit")
     ap.add_argument("--no-iter2", dest="iter2", action="store_false",
@@ -148,6 +152,7 @@ def main():
     if args.workload != "c2":
         line = bench_workloads.run(args)
         print(json.dumps(line), flush=True)
+        export_plans(args)
         return
 
     import numpy as np
@@ -245,6 +250,14 @@ def main():
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)
+    export_plans(args)
+
+
+def export_plans(args):
+    if args.export_plans:
+        from mlxlearn_amd import _lib
+        with open(args.export_plans, "w") as fh:
+            fh.write(_lib.export_plans())
 
 
 if __name__ == "__main__":

@@ -183,8 +183,9 @@ struct ScreenShape {
     int qt, ns;
 };
 
-// Measured default shape for a precision and padded width.
-ScreenShape default_shape(int prec, int dp);
+// Measured default shape for a screen mode (SCREEN_LISTS / SCREEN_PAIRS), a
+// precision and a padded width.
+ScreenShape default_shape(int mode, int prec, int dp);
 
 // Padded width of the screen mirror: a multiple of 32 up to 128; above that,
 // bf16 / fp16 use the wide-kernel widths 256, 384, 512, 640, 800 and then

@@ -99,7 +99,11 @@ __device__ __forceinline__ void tile_barrier() { asm volatile("s_barrier" ::: "m
 // for tile t+1 (vmcnt counts its outstanding tile loads in issue order).
 // Per tile a wave issues its MFMAs (accumulators start at the rows' biases
 // -a_x), then one branch-free max-tree test of every accumulator against its
-// query's threshold; only a hit enters the append path.
+// query's threshold; only a hit enters the append path. With several query
+// tiles per wave (narrow rows) a step holds two 32-row tiles that are
+// multiplied and tested one after the other, so one accumulator set is live
+// and the step's barrier and tile load serve twice the MFMA work; their A
+// fragments are read three k-steps ahead of the MFMAs that use them.
 template <int PREC, int DP, int QT, int RTW, int NS, int MODE, bool CH = false>
 __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
     // the body uses device-only types (buffer resources): compile it in the
@@ -139,7 +143,8 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
 
     int qb, split;
     block_item(a, qb, split);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, h = lane >> 5, col = lane & 31;
+    const int lane = threadIdx.x & 63, h = lane >> 5, col = lane & 31;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int64_t s_lo = (int64_t)split * a.split_rows;
     const int64_t r_begin = s_lo + a.off_lo;
     int64_t r_end = s_lo + (a.off_hi < a.split_rows ? a.off_hi : a.split_rows);
@@ -328,8 +333,8 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
             const int64_t r0 = r_begin + t * BN;
             f32x16 acc_step[RTW][QT];
             f32x16(&acc)[RTW][QT] = *(CH ? &acc_keep : &acc_step);
-#pragma unroll
-            for (int rt = 0; rt < RTW; ++rt) {
+            // MFMAs of row tile rt of the step
+            auto mfma_rt = [&](int rt) {
                 // accumulator start = -a_x of this lane's 16 rows (4 runs of 4 rows)
                 const float* nb = cB + rt * 32 + (h << 2);
                 const f32x4 b0 = *(const f32x4*)(nb), b1 = *(const f32x4*)(nb + 8);
@@ -351,6 +356,25 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
                             for (int qt = 0; qt < QT; ++qt)
                                 acc[rt][qt] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[e], bq[qt][j + e], acc[rt][qt], 0, 0, 0);
                     }
+                } else if constexpr (!CH && QT >= 2 && NB >= 4) {
+                    // A fragments read PF k-steps ahead of their MFMAs: the bias
+                    // and the first PF fragments, then one read per k-step
+                    constexpr int PF = 3;
+                    i32x4 av[NB];
+#pragma unroll
+                    for (int j = 0; j < NB; ++j) av[j] = *(const i32x4*)(arow + j * 16);
+#pragma unroll
+                    for (int qt = 0; qt < QT; ++qt) acc[rt][qt] = mfma16<PREC>(av[0], bq[qt][0], init);
+#pragma unroll
+                    for (int j = 1; j < NB; ++j)
+#pragma unroll
+                        for (int qt = 0; qt < QT; ++qt) acc[rt][qt] = mfma16<PREC>(av[j], bq[qt][j], acc[rt][qt]);
+                    __builtin_amdgcn_sched_group_barrier(0x100, 4 + PF, 0);
+#pragma unroll
+                    for (int j = 0; j < NB; ++j) {
+                        __builtin_amdgcn_sched_group_barrier(0x008, QT, 0);
+                        if (j + PF < NB) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+                    }
                 } else {
                     // the first k-step reads the bias registers as its C operand
                     // (deep rows: only in the first chunk of the tile)
@@ -365,31 +389,26 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
                         for (int qt = 0; qt < QT; ++qt) acc[rt][qt] = mfma16<PREC>(av, bq[qt][j], acc[rt][qt]);
                     }
                 }
-            }
-            // deep rows: the next step's fragments replace this step's once its
-            // MFMAs are issued (two stages: the step's final wait covers them)
-            if constexpr (CH) {
-                if (u + 1 < nsteps) load_query(bq, cc + 1 == nc ? 0 : cc + 1);
-            }
-            // Hit test: acc >= thr  <=>  L(x) <= tau. Branch-free over the tile.
-            // Deep rows: only after the tile's last chunk.
-            if (!CH || cc == nc - 1) {
-            bool hit_rq[RTW][QT];
-            bool any = false;
+            };
+            // Hit test of row tiles [rt0, rt1): acc >= thr  <=>  L(x) <= tau.
+            // Branch-free over the tiles; only a hit enters the append path.
+            auto test_rt = [&](int rt0, int rt1) {
+                bool hit_rq[RTW][QT];
+                bool any = false;
 #pragma unroll
-            for (int rt = 0; rt < RTW; ++rt)
+                for (int rt = 0; rt < RTW; ++rt)
 #pragma unroll
-                for (int qt = 0; qt < QT; ++qt) {
-                    hit_rq[rt][qt] = max16(acc[rt][qt]) >= thr[qt];
-                    any |= hit_rq[rt][qt];
-                }
-            if (ballot(any)) {
+                    for (int qt = 0; qt < QT; ++qt) {
+                        hit_rq[rt][qt] = rt >= rt0 && rt < rt1 && max16(acc[rt][qt]) >= thr[qt];
+                        any |= hit_rq[rt][qt];
+                    }
+                if (!ballot(any)) return;
                 stored = true;
 #pragma unroll
                 for (int rt = 0; rt < RTW; ++rt)
 #pragma unroll
                 for (int qt = 0; qt < QT; ++qt) {
-                    if (!ballot(hit_rq[rt][qt])) continue;
+                    if (rt < rt0 || rt >= rt1 || !ballot(hit_rq[rt][qt])) continue;
                     const int64_t qg = qrow0 + qt * 32 + col;
                     int64_t self = -1;
                     if (a.exclude) self = a.self_off + (MODE == SCREEN_PAIRS && a.qmap ? (int64_t)a.qmap[qg] : qg);
@@ -444,8 +463,27 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
                         }
                     }
                 }
+            };
+            // several query tiles per wave: each row tile is tested right after
+            // its MFMAs, so the accumulators of one row tile are live at a time
+            constexpr bool SEQ = !CH && QT >= 2 && RTW > 1;
+            if constexpr (SEQ) {
+#pragma unroll
+                for (int rt = 0; rt < RTW; ++rt) {
+                    mfma_rt(rt);
+                    test_rt(rt, rt + 1);
+                }
+            } else {
+#pragma unroll
+                for (int rt = 0; rt < RTW; ++rt) mfma_rt(rt);
+                // deep rows: the next step's fragments replace this step's once its
+                // MFMAs are issued (two stages: the step's final wait covers them)
+                if constexpr (CH) {
+                    if (u + 1 < nsteps) load_query(bq, cc + 1 == nc ? 0 : cc + 1);
+                }
+                // deep rows: the test only after the tile's last chunk
+                if (!CH || cc == nc - 1) test_rt(0, RTW);
             }
-            }  // hit test
         }
         // Stores of the hit path also count in vmcnt: after one, wait for
         // everything rather than rely on the order of loads and stores.
@@ -521,7 +559,8 @@ __global__ __launch_bounds__(256) void k_screen_pp(const ScreenArgs a) {
 
     int qb, split;
     block_item(a, qb, split);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, h = lane >> 5, col = lane & 31;
+    const int lane = threadIdx.x & 63, h = lane >> 5, col = lane & 31;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int64_t s_lo = (int64_t)split * a.split_rows;
     const int64_t r_begin = s_lo + a.off_lo;
     int64_t r_end = s_lo + (a.off_hi < a.split_rows ? a.off_hi : a.split_rows);
@@ -696,9 +735,12 @@ __global__ __launch_bounds__(256) void k_screen_pp(const ScreenArgs a) {
 #endif
 }
 
+// Narrow rows: 64-row steps for one and two query tiles per wave (measured on
+// the 1M x 128 mixture: two query tiles with 64-row steps, 2.07 ms per call,
+// against 2.2 ms with 32-row steps), 32-row steps for four.
 template <int PREC, int QT, int NS, int MODE>
 static ScreenFn pick_dp(int dp) {
-    constexpr int RTW = QT == 1 ? 2 : 1;
+    constexpr int RTW = QT <= 2 ? 2 : 1;
     switch (dp) {
         case 32: return k_screen<PREC, 32, QT, RTW, NS, MODE>;
         case 64: return k_screen<PREC, 64, QT, RTW, NS, MODE>;
@@ -781,10 +823,14 @@ int screen_dp(int prec, int d) {
     return (d + DEEP_CHUNK - 1) / DEEP_CHUNK * DEEP_CHUNK;  // deep kernels
 }
 
-// Measured on the 1M x 128 mixture workload: with the candidate appends of a
-// real query, one query tile per wave (occupancy 4) beats two (occupancy 3).
-ScreenShape default_shape(int prec, int dp) {
-    return (prec != PREC_FP32 && dp > 128 && dp <= WIDE_MAX) ? ScreenShape{1, 3} : ScreenShape{1, 2};
+// Measured on the 1M x 128 mixture workload: for k-NN lists on bf16 / fp16
+// rows of at most 128 features, two query tiles per wave with 64-row steps
+// (2.07 ms per call) beat one (2.28 ms); float32 operands keep one (two would
+// need 128 registers of query fragments), and so do radius pairs.
+ScreenShape default_shape(int mode, int prec, int dp) {
+    if (prec != PREC_FP32 && dp > 128 && dp <= WIDE_MAX) return ScreenShape{1, 3};
+    if (mode == SCREEN_LISTS && prec != PREC_FP32 && dp <= 128) return ScreenShape{2, 2};
+    return ScreenShape{1, 2};
 }
 
 ScreenFn pick_screen(int mode, int prec, int dp, ScreenShape shape) {
@@ -801,7 +847,8 @@ ScreenFn pick_screen(int mode, int prec, int dp, ScreenShape shape) {
 std::vector<ScreenShape> screen_shapes(int mode, int prec, int dp) {
     std::vector<ScreenShape> out;
     if (dp > 128) {  // wide rows: one shape, whatever is asked for
-        if (pick_screen(mode, prec, dp, default_shape(prec, dp))) out.push_back(default_shape(prec, dp));
+        const ScreenShape only = default_shape(mode, prec, dp);
+        if (pick_screen(mode, prec, dp, only)) out.push_back(only);
         return out;
     }
     for (int qt : {1, 2, 4})

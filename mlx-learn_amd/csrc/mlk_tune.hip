@@ -49,7 +49,7 @@ static int round_splits(int64_t want, int64_t n) {
 
 Plan heuristic_plan(int kind, int prec, int dp, int64_t n, int64_t m, int k, ScreenShape shape) {
     Plan p;
-    p.shape = shape.qt > 0 ? shape : default_shape(prec, dp);
+    p.shape = shape.qt > 0 ? shape : default_shape(kind == PLAN_KNN ? SCREEN_LISTS : SCREEN_PAIRS, prec, dp);
     const int64_t BQ = 4 * p.shape.qt * 32;
     const int64_t qb = std::max<int64_t>(1, (m + BQ - 1) / BQ);
     if (kind == PLAN_KNN) {
@@ -106,7 +106,7 @@ Plan resolve_plan(int kind, int prec, int dp, int64_t n, int64_t m, int k, bool*
     const Tuning t = tuning_snapshot();
     ScreenShape knob{0, 0};
     if (kind == PLAN_KNN && dp <= 128 && (t.query_tiles || t.stages)) {
-        knob = default_shape(prec, dp);  // wide rows have one shape
+        knob = default_shape(SCREEN_LISTS, prec, dp);  // wide rows have one shape
         if (t.query_tiles) knob.qt = t.query_tiles;
         if (t.stages) knob.ns = t.stages;
     }
