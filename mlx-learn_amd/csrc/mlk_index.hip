@@ -29,6 +29,11 @@ int DevBuf::reserve(size_t need) {
     size_t want = need < 256 ? 256 : need;
     if (regrow) want += want / 8;
     hipError_t e = hipMalloc(&ptr, want);
+    if (e == hipErrorOutOfMemory && workspace_release_pool()) {
+        // the pooled workspace of transient indexes held the memory
+        (void)hipGetLastError();
+        e = hipMalloc(&ptr, want);
+    }
     if (e != hipSuccess) {
         ptr = nullptr;
         bytes = 0;
@@ -45,6 +50,49 @@ void DevBuf::release() {
     if (ptr) (void)hipFree(ptr);
     ptr = nullptr;
     bytes = 0;
+}
+
+// ------------------------------------------------ pooled workspace
+constexpr int POOL_DEVICES = 64;
+constexpr int WS_SLOTS = sizeof(mlk_index::ws) / sizeof(DevBuf);
+static std::mutex g_pool_lock;
+// never destroyed: freeing device memory from a static destructor could run
+// after the HIP runtime has shut down
+static DevBuf (&g_pool)[POOL_DEVICES][WS_SLOTS] = *new DevBuf[1][POOL_DEVICES][WS_SLOTS];
+
+static void swap_buf(DevBuf& a, DevBuf& b) {
+    std::swap(a.ptr, b.ptr);
+    std::swap(a.bytes, b.bytes);
+}
+
+void workspace_adopt(mlk_index* index) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= POOL_DEVICES) return;
+    std::lock_guard<std::mutex> g(g_pool_lock);
+    for (int i = 0; i < WS_SLOTS; ++i)
+        if (!index->ws[i].ptr) swap_buf(index->ws[i], g_pool[dev][i]);
+}
+
+// The caller has synchronised the device: no queued work uses the buffers.
+void workspace_return(mlk_index* index) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= POOL_DEVICES) return;
+    std::lock_guard<std::mutex> g(g_pool_lock);
+    for (int i = 0; i < WS_SLOTS; ++i) {
+        // keep the larger of the two buffers of each slot
+        if (index->ws[i].bytes > g_pool[dev][i].bytes) swap_buf(index->ws[i], g_pool[dev][i]);
+    }
+}
+
+bool workspace_release_pool() {
+    std::lock_guard<std::mutex> g(g_pool_lock);
+    bool any = false;
+    for (auto& dev : g_pool)
+        for (DevBuf& b : dev) {
+            any |= b.ptr != nullptr;
+            b.release();
+        }
+    return any;
 }
 
 // Per-block float64 column sums (rows [b*rows_per_block, ...)) and an

@@ -75,6 +75,17 @@ struct DevBuf {
     template <class T> T* as() const { return static_cast<T*>(ptr); }
 };
 
+// Workspace pool of transient indexes (the centres index of one k-means
+// assignment): such an index borrows the device's pooled per-call workspace
+// when it is made and hands its own back when it is destroyed, so repeated
+// assignments of one shape reuse their candidate lists instead of allocating
+// and freeing gigabytes per call. One workspace is kept per device, until the
+// process ends or a device allocation fails (it is then freed and the
+// allocation tried again).
+void workspace_adopt(mlk_index* index);
+void workspace_return(mlk_index* index);
+bool workspace_release_pool();
+
 // Device memory of one call's temporaries. Allocating mode (arena == nullptr):
 // slot i of the index's grow-only workspace, reserved with hipMalloc when it
 // is too small. Fixed mode: consecutive 256-byte aligned pieces of a

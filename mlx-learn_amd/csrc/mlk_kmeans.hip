@@ -22,6 +22,18 @@ int create_index(const float* X, int64_t n, int d, hipStream_t stream, mlk_index
 
 static inline unsigned nblk(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
 
+// The centres index of one assignment: it borrows the device's pooled
+// workspace and hands it back (after the device has finished with it).
+struct TransientIndex {
+    mlk_index* p;
+    explicit TransientIndex(mlk_index* index) : p(index) { workspace_adopt(p); }
+    ~TransientIndex() {
+        (void)hipDeviceSynchronize();
+        workspace_return(p);
+        delete p;
+    }
+};
+
 // Labels outside [0, kc) are counted in counts[kc] and reported as an input
 // error by the host; their rows go to the sentinel key kc and are not summed.
 __global__ void k_labels_u32(const int64_t* __restrict__ labels, int64_t n, int64_t kc, unsigned* __restrict__ keys,
@@ -255,10 +267,7 @@ extern "C" int mlk_kmeans_assign(const float* X_dev, int64_t n, int32_t d, const
     if (n == 0) return MLK_OK;
     mlk_index* centres = nullptr;
     MLK_TRY(create_index(C_dev, kc, d, (hipStream_t)stream, &centres));
-    struct G {
-        mlk_index* p;
-        ~G() { (void)hipDeviceSynchronize(); delete p; }
-    } g{centres};
+    TransientIndex g{centres};
     const uint32_t qflags = (flags & MLK_PREC_MASK) | MLK_SQUARED_DIST;
     std::lock_guard<std::mutex> lk(centres->lock);
     return knn_query_impl(centres, X_dev, n, 1, 0, qflags, residual_dev, labels_dev, (hipStream_t)stream);
@@ -284,10 +293,7 @@ extern "C" int mlk_kmeans_assign_reuse(const float* X_dev, int64_t n, int32_t d,
     if (n == 0) return MLK_OK;
     mlk_index* centres = nullptr;
     MLK_TRY(create_index(C_dev, kc, d, stream, &centres));
-    struct G {
-        mlk_index* p;
-        ~G() { (void)hipDeviceSynchronize(); delete p; }
-    } g{centres};
+    TransientIndex g{centres};
     std::lock_guard<std::mutex> lk(centres->lock);
     const uint32_t qflags = (flags & MLK_PREC_MASK) | MLK_SQUARED_DIST;
     DevBuf sep2, sep_lo, near, unc, words, Xu, lab_u, res_u;

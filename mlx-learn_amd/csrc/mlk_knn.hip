@@ -1538,10 +1538,9 @@ static int knn_enqueue(mlk_index* index, const float* Q, int64_t m, int k, int64
         sa.QB = (int)QB;
         sa.self_off = q_offset;
         sa.exclude = exclude;
-        sa.xcd_map = (S % 8) == 0;
         sa.nchunk = screen_chunks(prec, dp);
         sa.qgroup = sa.nchunk > 1 ? DEEP_QGROUP : 1;
-        const unsigned blocks = (unsigned)(QB * S);
+        const unsigned blocks = (unsigned)screen_grid(sa);
         auto merge = [&]() -> int {
             if (lane_merge)
                 hipLaunchKernelGGL(k == 1 ? k_merge_lane<1> : k_merge_lane<8>, dim3((unsigned)((m + 255) / 256)),

@@ -641,14 +641,13 @@ int radius_query_impl(mlk_index* index, const float* Q, int64_t m, double R2, in
             sa.QB = (int)QB;
             sa.self_off = q_offset;
             sa.exclude = exclude;
-            sa.xcd_map = (S % 8) == 0;
             sa.nchunk = screen_chunks(prec, dp);
             sa.qgroup = sa.nchunk > 1 ? DEEP_QGROUP : 1;
             sa.pairs = ws[9].as<int2>();
             sa.qmap = qmap;
             sa.pair_count = dcount;
             sa.pair_cap = cap;
-            hipLaunchKernelGGL(screen, dim3((unsigned)(QB * S)), dim3(256), 0, stream, sa);
+            hipLaunchKernelGGL(screen, dim3((unsigned)screen_grid(sa)), dim3(256), 0, stream, sa);
             MLK_CHECK_HIP(hipGetLastError());
         } else {
             set_error("no radius screen kernel for this row width");

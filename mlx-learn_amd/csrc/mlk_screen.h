@@ -166,6 +166,18 @@ struct ScreenArgs {
     const int* qmap;
 };
 
+// Block order of a screen launch (sets xcd_map; S, QB and qgroup must be set):
+// with at least eight splits, the launch's work items (query block, split),
+// split-major, are dealt to the eight XCDs in contiguous shares, so the
+// blocks an XCD runs together stream the same database rows through its L2
+// (deep rows group query blocks too, and need a multiple of eight splits).
+// Returns the grid size; blocks past the last item exit at once.
+inline int64_t screen_grid(ScreenArgs& a) {
+    a.xcd_map = a.S >= 8 && (a.qgroup <= 1 || a.S % 8 == 0);
+    const int64_t items = (int64_t)a.S * a.QB;
+    return a.xcd_map ? (items + 7) / 8 * 8 : items;
+}
+
 constexpr int WIDE_MAX = 800;    // widest row whose query fragments stay in registers (bf16 / fp16)
 constexpr int DEEP_CHUNK = 256;  // features per chunk of the bf16 / fp16 deep kernels
 constexpr int FP32_CHUNK = 128;  // features per chunk of the float32 deep kernel (512 bytes, as DEEP_CHUNK halves)

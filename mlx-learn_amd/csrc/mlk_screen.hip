@@ -34,7 +34,8 @@ __device__ __forceinline__ f32x16 mfma16(const i32x4& a, const i32x4& b, const f
 
 // Work item (query block, split) of a block; the QB blocks of one split are
 // consecutive slots of one XCD (blocks are dispatched round-robin over the 8
-// XCDs), so they stream the same rows through that XCD's L2 together.
+// XCDs), so they stream the same rows through that XCD's L2 together. A block
+// past the last item gets split >= S (no rows: it exits).
 __device__ __forceinline__ void block_item(const ScreenArgs& a, int& qb, int& split) {
     const int b = blockIdx.x;
     if (a.xcd_map && a.qgroup > 1) {
@@ -48,9 +49,12 @@ __device__ __forceinline__ void block_item(const ScreenArgs& a, int& qb, int& sp
         qb = g * a.qgroup + local % gsize;
         split = (local / gsize) * 8 + xcd;
     } else if (a.xcd_map) {
+        // XCD x takes items [x * per, (x + 1) * per) of the split-major order
         const int xcd = b & 7, slot = b >> 3;
-        qb = slot % a.QB;
-        split = (slot / a.QB) * 8 + xcd;
+        const int64_t per = ((int64_t)a.S * a.QB + 7) / 8;
+        const int64_t item = xcd * per + slot;
+        qb = (int)(item % a.QB);
+        split = (int)(item / a.QB);
     } else {
         qb = b % a.QB;
         split = b / a.QB;
