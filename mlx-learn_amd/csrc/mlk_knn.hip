@@ -142,6 +142,12 @@ __global__ __launch_bounds__(256) void k_tau0(const float* __restrict__ X, int64
     const int64_t row_step = (int64_t)(256 / FL) * stride * d;  // eight sample rows
     const int64_t x_first = (int64_t)j0 * stride * d + e_st;
     const int64_t q_first = (q0 + qw0) * d + e_st;
+    // listed samples: the block's rows, read from memory once and kept in LDS
+    // for every pass (-1: no sample)
+    __shared__ int srr[NS];
+    if (srow)
+        for (int j = threadIdx.x; j < NS; j += 256) srr[j] = j < samples ? srow[j] : -1;
+    if (srow) __syncthreads();
     auto fetch = [&](int f0) {
         // one running offset per thread, opaque to the compiler at each pass, so the
         // per-value addresses are formed at their loads instead of being kept live
@@ -151,8 +157,7 @@ __global__ __launch_bounds__(256) void k_tau0(const float* __restrict__ X, int64
         if (srow) {
 #pragma unroll
             for (int i = 0; i < PER; ++i) {
-                const int j = j0 + (256 / FL) * i;
-                const int64_t r = j < samples ? (int64_t)srow[j] : -1;  // -1: no sample
+                const int64_t r = srr[j0 + (256 / FL) * i];
                 stg[i] = X[fe && r >= 0 ? r * d + f0 + e_st : 0];
             }
         } else {
@@ -375,7 +380,9 @@ __global__ void k_merge(int64_t m, int k, const int* __restrict__ cnt, int* __re
     }
     // the K2 = min(2k, 32) smallest bounds with their rows (every value below the
     // K2-th, then entries equal to it, then empty slots); their float64 distances
-    // replace the bounds where smaller, and the k smallest results become kbest
+    // replace the bounds where smaller, and the k smallest results become kbest.
+    // Entries carried over from kbest hold such a distance already (an earlier
+    // merge computed it): they are kept as -2 - row and not computed again.
     const int K2 = 2 * k < 32 ? 2 * k : 32;
     const uint32_t kb2 = wave_kth_bits(total, K2, [&](int j) { return buf[j]; });
     float* kv = sKV[w];
@@ -389,7 +396,7 @@ __global__ void k_merge(int64_t m, int k, const int* __restrict__ cnt, int* __re
             const int pos = put + __popcll(mask & ((1ull << lane) - 1));
             if (sel && pos < K2) {
                 kv[pos] = __uint_as_float(buf[j]);
-                kr[pos] = rbuf[j];
+                kr[pos] = (j < k && rbuf[j] >= 0) ? -2 - rbuf[j] : rbuf[j];
             }
             put += __popcll(mask);
         }
@@ -400,25 +407,40 @@ __global__ void k_merge(int64_t m, int k, const int* __restrict__ cnt, int* __re
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    // float64 distances of those rows, 16 lanes per row (any summation order)
+    // float64 distances of the new rows, 16 lanes per row (any summation
+    // order); eight rows per pass, the loads of both groups of four issued
+    // before either sum
     const double g = f64_interval(d);
     const int gl = lane & 15, grp = lane >> 4;
     const float* qv = Q + q * d;
-    for (int j0 = 0; j0 < K2; j0 += 4) {
-        const int j = j0 + grp;
-        const int r = j < K2 ? kr[j] : -1;
-        double part = 0.0;
-        if (r >= 0) {
-            const float* xv = X + (size_t)r * d;
-            for (int e = gl; e < d; e += 16) {
-                const double t = (double)qv[e] - (double)xv[e];
-                part += t * t;
-            }
+    for (int j0 = 0; j0 < K2; j0 += 8) {
+        const int ja = j0 + grp, jb = j0 + 4 + grp;
+        const int ra = ja < K2 ? kr[ja] : -1, rb = jb < K2 ? kr[jb] : -1;
+        if (!ballot(ra >= 0 || rb >= 0)) continue;
+        const float* xa = X + (size_t)(ra >= 0 ? ra : 0) * d;
+        const float* xb = X + (size_t)(rb >= 0 ? rb : 0) * d;
+        double pa = 0.0, pb = 0.0;
+#pragma unroll 4
+        for (int e = gl; e < d; e += 16) {
+            const double qe = (double)qv[e];
+            const float va = xa[e], vb = xb[e];
+            const double ta = qe - (double)va, tb = qe - (double)vb;
+            pa += ta * ta;
+            pb += tb * tb;
         }
-        for (int o = 8; o > 0; o >>= 1) part += __shfl_xor(part, o, 16);
-        if (r >= 0 && gl == 0) {
-            const float up = __double2float_ru(part * (1.0 + g) * c.scale2);
-            if (up < kv[j]) kv[j] = up;
+        for (int o = 8; o > 0; o >>= 1) {
+            pa += __shfl_xor(pa, o, 16);
+            pb += __shfl_xor(pb, o, 16);
+        }
+        if (gl == 0) {
+            if (ra >= 0) {
+                const float up = __double2float_ru(pa * (1.0 + g) * c.scale2);
+                if (up < kv[ja]) kv[ja] = up;
+            }
+            if (rb >= 0) {
+                const float up = __double2float_ru(pb * (1.0 + g) * c.scale2);
+                if (up < kv[jb]) kv[jb] = up;
+            }
         }
     }
     __builtin_amdgcn_wave_barrier();
@@ -433,8 +455,9 @@ __global__ void k_merge(int64_t m, int k, const int* __restrict__ cnt, int* __re
             rank += (u < v) || (u == v && i < lane);
         }
         if (rank < k) {
+            const int r = kr[lane];
             kbest[q * k + rank] = v;
-            kbest_row[q * k + rank] = kr[lane];
+            kbest_row[q * k + rank] = r < -1 ? -2 - r : r;
         }
         if (rank == k - 1) kth = v;
     }
