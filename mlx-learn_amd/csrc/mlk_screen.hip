@@ -137,11 +137,11 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
     __shared__ __attribute__((aligned(1024))) char sA1[ABYTES];
     __shared__ __attribute__((aligned(1024))) char sA2[NS > 2 ? ABYTES : 16];
     __shared__ __attribute__((aligned(1024))) char sA3[NS > 3 ? ABYTES : 16];
-    // [0, 64): -a_x of the tile rows, [64, 128): their norms (float32)
-    __shared__ __attribute__((aligned(256))) float sB0[128];
-    __shared__ __attribute__((aligned(256))) float sB1[128];
-    __shared__ __attribute__((aligned(256))) float sB2[NS > 2 ? 128 : 4];
-    __shared__ __attribute__((aligned(256))) float sB3[NS > 3 ? 128 : 4];
+    // [0, 64): -a_x of the tile rows (float32)
+    __shared__ __attribute__((aligned(256))) float sB0[64];
+    __shared__ __attribute__((aligned(256))) float sB1[64];
+    __shared__ __attribute__((aligned(256))) float sB2[NS > 2 ? 64 : 4];
+    __shared__ __attribute__((aligned(256))) float sB3[NS > 3 ? 64 : 4];
     __shared__ int2 sPair[MODE == SCREEN_PAIRS ? NW : 1][MODE == SCREEN_PAIRS ? PAIR_STAGE : 1];
     __shared__ int sPairN[NW];
 
@@ -252,8 +252,7 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
     // position L % (CPR + 1); the last position of each row and every chunk
     // past the tile are padding (loaded out of range, i.e. zero). Source
     // offsets within a tile are lane constants; the tile base goes into the
-    // buffer resource (SGPRs). Waves 1 and 2 also load the rows' biases and
-    // norms.
+    // buffer resource (SGPRs). Wave 1 also loads the rows' biases.
     auto chunk_off = [&](int i) -> uint32_t {
         const int L = (wave + i * NW) * 64 + lane;
         const int row = L / (CPR + 1), pos = L % (CPR + 1);
@@ -267,7 +266,7 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
         for (int i = 0; i < LPW; ++i) voff[i] = chunk_off(i);
     }
     // tile loads this wave issues (uniform): its share of the NLOAD chunks, plus one
-    const int my_loads = (NLOAD > wave ? (NLOAD - wave + NW - 1) / NW : 0) + (wave == 1 || wave == 2);
+    const int my_loads = (NLOAD > wave ? (NLOAD - wave + NW - 1) / NW : 0) + (wave == 1);
 
     auto bufA = [&](auto I) -> char* {
         constexpr int i = decltype(I)::value;
@@ -295,10 +294,9 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
                 __builtin_amdgcn_raw_ptr_buffer_load_lds(ra, (lds_char*)(dstA + g * 1024), 16, off, 0, 0, 0);
             }
         }
-        if (wave == 1 || wave == 2) {
-            const float* src = wave == 1 ? a.bias + r0 : a.nx32 + r0;
-            const __amdgpu_buffer_rsrc_t rb = buffer_rsrc(src, (uint32_t)(rows * 4));
-            __builtin_amdgcn_raw_ptr_buffer_load_lds(rb, (lds_char*)(dstB + (wave - 1) * 64), 4, lane * 4, 0, 0, 0);
+        if (wave == 1) {
+            const __amdgpu_buffer_rsrc_t rb = buffer_rsrc(a.bias + r0, (uint32_t)(rows * 4));
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rb, (lds_char*)dstB, 4, lane * 4, 0, 0, 0);
         }
     };
     // wait until at most `stages` tiles of this wave's loads are outstanding
