@@ -1684,6 +1684,17 @@ static int knn_complete(mlk_index* index, const KnnCall& call, hipStream_t strea
     st.splits = call.splits;
     st.capacity = call.capacity;
     (void)index;
+    // regrouped queries: results to the caller's rows, enqueued before the
+    // read-back so that the common call (no fallback) ends with it; a fallback
+    // rewrites some rows by position and moves them again below
+    auto unpermute = [&]() -> int {
+        if (!call.qperm) return MLK_OK;
+        hipLaunchKernelGGL(k_unpermute_results, dim3((unsigned)((m * k + 255) / 256)), dim3(256), 0, stream,
+                           call.qperm, m, k, idx, dist, call.user_idx, call.user_dist);
+        MLK_CHECK_HIP(hipGetLastError());
+        return MLK_OK;
+    };
+    MLK_TRY(unpermute());
     // one read-back: the fallback count and the statistics sums
     struct {
         unsigned nf, pad;
@@ -1693,6 +1704,7 @@ static int knn_complete(mlk_index* index, const KnnCall& call, hipStream_t strea
     MLK_CHECK_HIP(hipMemcpyAsync(&words, dflag + 2, sizeof(words), hipMemcpyDeviceToHost, stream));
     MLK_CHECK_HIP(hipStreamSynchronize(stream));
     unsigned nf = words.nf;
+    const bool rewritten = nf != 0;  // some rows get ranked again below (by position)
     std::vector<int> fq_host;
     if (nf) {
         fq_host.resize(nf);
@@ -1777,11 +1789,7 @@ static int knn_complete(mlk_index* index, const KnnCall& call, hipStream_t strea
             MLK_CHECK_HIP(hipGetLastError());
         }
     }
-    if (call.qperm) {
-        hipLaunchKernelGGL(k_unpermute_results, dim3((unsigned)((m * k + 255) / 256)), dim3(256), 0, stream,
-                           call.qperm, m, k, idx, dist, call.user_idx, call.user_dist);
-        MLK_CHECK_HIP(hipGetLastError());
-    }
+    if (rewritten) MLK_TRY(unpermute());
     if (!events) {
         // no events: the outputs are ordered on the stream; the fallback's host
         // list must outlive its copy
