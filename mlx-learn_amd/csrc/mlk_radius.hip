@@ -397,10 +397,10 @@ __device__ __forceinline__ bool dist_idx_less(double da, int64_t ia, double db, 
     return da < db || (da == db && ia < ib);
 }
 
-// Entries of a row whose float32 keys are equal keep their index order from
-// the stable key sort; the thread at the start of each such run orders it by
-// (distance, index): insertion for short runs, an in-place heap sort for long
-// ones (many distinct distances inside one float32 step).
+// Entries of a row whose float32 keys are equal stay in their input order
+// after the stable key sort; the thread at the start of each such run orders
+// it by (distance, index): insertion for short runs, an in-place heap sort for
+// long ones (many distinct distances inside one float32 step).
 __global__ void k_row_fix_runs(const uint32_t* __restrict__ keys, const uint8_t* __restrict__ start, int64_t nnz,
                                double* __restrict__ dist, int64_t* __restrict__ idx) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -463,8 +463,8 @@ static int check_sort_count(int64_t count, const char* what) {
 
 // Every CSR row ordered by (distance, index). A stable segmented radix sort on
 // 32-bit keys (the float32 rounding of the distance) and 32-bit positions
-// orders the rows up to runs of equal keys, whose entries arrive in ascending
-// index order; the runs are then ordered exactly. Temporaries live in the
+// orders the rows up to runs of equal keys; the runs are then ordered exactly
+// (so the rows' input order does not matter). Temporaries live in the
 // index workspace (ws[14] sort scratch, ws[10..11] keys, ws[21..22] positions,
 // ws[23] row starts, ws[19..20] sorted distances and indices); the sorted
 // arrays are swapped into the result.
@@ -675,7 +675,11 @@ int radius_query_impl(mlk_index* index, const float* Q, int64_t m, double R2, in
     st.candidates = np;
     MLK_TRY(check_sort_count(np, "candidate pairs"));
 
-    // sort pairs by (query, row): keys (query << rb) | row, rb = bits of n
+    // sort pairs by (query, row): keys (query << rb) | row, rb = bits of n.
+    // Rows that are re-sorted by distance afterwards need only the query
+    // order (the distance sort orders every row by (distance, index) whatever
+    // its input order), so their sort skips the row bits (c4: 3 radix passes
+    // instead of 5).
     int rb = 1;
     while ((1ll << rb) < n) ++rb;
     MLK_TRY(ws[10].reserve((size_t)std::max<int64_t>(np, 1) * 8));   // keys in
@@ -689,10 +693,13 @@ int radius_query_impl(mlk_index* index, const float* Q, int64_t m, double R2, in
                            keys);
         int qbits = 1;
         while ((1ll << qbits) < m) ++qbits;
+        const int begin_bit = sort_rows ? rb : 0;
         size_t tmp_bytes = 0;
-        MLK_CHECK_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp_bytes, keys, skeys, (int)np, 0, rb + qbits, stream));
+        MLK_CHECK_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp_bytes, keys, skeys, (int)np, begin_bit, rb + qbits,
+                                                        stream));
         MLK_TRY(ws[14].reserve(tmp_bytes + 16));
-        MLK_CHECK_HIP(hipcub::DeviceRadixSort::SortKeys(ws[14].ptr, tmp_bytes, keys, skeys, (int)np, 0, rb + qbits, stream));
+        MLK_CHECK_HIP(hipcub::DeviceRadixSort::SortKeys(ws[14].ptr, tmp_bytes, keys, skeys, (int)np, begin_bit,
+                                                        rb + qbits, stream));
         const bool vec = d % 4 == 0 && ((uintptr_t)Q % 16) == 0 && ((uintptr_t)X % 16) == 0;
         if (vec)
             hipLaunchKernelGGL(k_classify_staged, dim3(blocks_for(np)), dim3(256), 0, stream, skeys, np, rb, X, Q, d,
