@@ -960,57 +960,72 @@ __global__ __launch_bounds__(256) void k_rank1_lane(const RankArgs a, int64_t m,
 
 // Rank for k = 1 with a group of eight lanes per query (rows of at most
 // 32 * NF4 features, d % 4 == 0, 16-byte aligned rows). The group holds its
-// query row in registers (lane gl owns the 16-byte chunks gl + 8 t), takes the
-// list entries round-robin for the certified-bound filter, and forms each
-// surviving candidate's float64 distance from coalesced 16-byte row segments,
-// reduced over the group in any order (covered by the order-free bound g).
-// Candidates are compared by their float64 intervals only; a query with an
-// undecided comparison (a near or exact tie) is marked ST_TIE and ranked again
-// by k_rank1_lane, whose exact comparator resolves it. Results and statistics
-// are those of k_rank1_lane.
+// query row in registers (lane gl owns the 16-byte chunks gl + 8 t) and takes
+// the list entries round-robin for the certified-bound filter. The rows that
+// pass are compacted, in list order, into the group's LDS slots; their float64
+// distances are then formed SB rows at a time from coalesced 16-byte segments
+// whose loads are issued together (one memory latency per SB rows), reduced
+// over the group by a butterfly (every lane gets the same sum; any order is
+// covered by the order-free bound g). Loop bounds and cross-lane steps are
+// uniform over the wave: groups with shorter lists, or no query, take part
+// with empty batches, so no shuffle runs under a partial mask. Candidates are
+// compared by their float64 intervals only; a query with an undecided
+// comparison (a near or exact tie) is marked ST_TIE and ranked again by
+// k_rank1_lane, whose exact comparator resolves it. The output distance is the
+// canonical sum: for d <= 128 the lanes that own the features form its
+// products and the group's first lane adds them in feature order from LDS.
+// Results and statistics are those of k_rank1_lane.
 template <int NF4>
 __global__ __launch_bounds__(256) void k_rank1_group(const RankArgs a, int64_t m) {
     constexpr int G = 8;
-    const int lane = threadIdx.x & 63, gl = lane % G, gbase = lane - gl;
+    constexpr int SB = NF4 <= 4 ? 4 : 2;  // surviving rows whose loads are issued together
+    constexpr bool LDS_SUM = NF4 <= 4;    // canonical products staged in LDS
+    __shared__ int srows[256 / G][4 * G];
+    __shared__ double sprod[LDS_SUM ? 256 / G : 1][LDS_SUM ? 32 * NF4 : 1];
+    const int lane = threadIdx.x & 63, gl = lane % G, gbase = lane - gl, grp = threadIdx.x / G;
     const int64_t q = ((int64_t)blockIdx.x * 256 + threadIdx.x) / G;
-    if (q >= m) return;  // whole groups leave together (G divides the block)
-    if (a.status[q]) {   // handled by the fallback
+    bool live = q < m;
+    if (live && a.status[q]) {  // handled by the fallback
         if (gl == 0 && a.qstats) a.qstats[3 * q] = a.qstats[3 * q + 1] = a.qstats[3 * q + 2] = 0;
-        return;
+        live = false;
     }
+    if (ballot(live) == 0) return;  // the whole wave leaves together
     const int d = a.d;
     const double g = f64_interval(d);
-    const float* qv = a.Q + q * d;
+    const int64_t qs = live ? q : 0;
     f32x4 qr[NF4];
 #pragma unroll
     for (int t = 0; t < NF4; ++t) {
         const int f = (t * G + gl) * 4;
-        qr[t] = f < d ? *(const f32x4*)(qv + f) : f32x4{0.f, 0.f, 0.f, 0.f};
+        qr[t] = live && f < d ? *(const f32x4*)(a.Q + qs * d + f) : f32x4{0.f, 0.f, 0.f, 0.f};
     }
-    const float tau = a.tau[q];
-    const double nqq = a.nq[q];
+    const float tau = live ? a.tau[q] : 0.f;
+    const double nqq = live ? a.nq[q] : 0.0;
     const double cut = (double)tau / a.c.scale2;
     int listed = 0, count = 0, kept = 0;
     bool over = false, tie = false;
     double best_s = INFINITY;
     long long best_i = -1;
     for (int sl = 0; sl < a.S; ++sl) {
-        int cs = a.cnt[q * a.S + sl];
+        int cs = live ? a.cnt[q * a.S + sl] : 0;
         over |= cs > a.cap;
         cs = cs < a.cap ? cs : a.cap;
         listed += cs;
-        const int2* lst = a.cand + (size_t)q * a.cap * a.S + sl;  // entry j at lst[j * S]
-        // 4 G entries per step: their loads, then their norm gathers, are issued together
-        for (int j0 = 0; j0 < cs; j0 += 4 * G) {
+        int cw = cs;  // longest list of the wave (cs is uniform in the group)
+#pragma unroll
+        for (int o = 32; o >= G; o >>= 1) cw = max(cw, __shfl_xor(cw, o));
+        const int2* lst = a.cand + (size_t)qs * a.cap * a.S + sl;  // entry j at lst[j * S]
+        for (int j0 = 0; j0 < cw; j0 += 4 * G) {
             int2 e[4];
             float nxv[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int j = j0 + gl + G * u;
-                e[u] = lst[(size_t)(j < cs ? j : cs - 1) * a.S];
+                e[u] = j < cs ? lst[(size_t)j * a.S] : make_int2(0, 0);
             }
 #pragma unroll
             for (int u = 0; u < 4; ++u) nxv[u] = a.nx32[e[u].x];
+            int ns = 0;  // rows of this chunk passing the bound filter (uniform in the group)
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 bool pass = false;
@@ -1019,36 +1034,60 @@ __global__ __launch_bounds__(256) void k_rank1_group(const RankArgs a, int64_t m
                     hit_bounds(__int_as_float(e[u].y), nxv[u], nqq, a.c, L, U);
                     pass = L <= (double)tau;
                 }
-                uint32_t mask = (uint32_t)(ballot(pass) >> gbase) & ((1u << G) - 1);  // uniform in the group
-                count += __popc(mask);
-                while (mask) {
-                    const int b = __builtin_ctz(mask);
-                    mask &= mask - 1;
-                    const int row = __shfl(e[u].x, gbase + b);
-                    const float* xv = a.X + (size_t)row * d;
-                    double part = 0.0;
+                const uint32_t mu = (uint32_t)(ballot(pass) >> gbase) & ((1u << G) - 1u);
+                if (pass) srows[grp][ns + __popc(mu & ((1u << gl) - 1u))] = e[u].x;
+                ns += __popc(mu);
+            }
+            count += ns;
+            int nw = ns;
 #pragma unroll
-                    for (int t = 0; t < NF4; ++t) {
-                        const int f = (t * G + gl) * 4;
-                        if (f < d) {
-                            const f32x4 x4 = *(const f32x4*)(xv + f);
+            for (int o = 32; o >= G; o >>= 1) nw = max(nw, __shfl_xor(nw, o));
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            for (int b0 = 0; b0 < nw; b0 += SB) {
+                int rows[SB];
 #pragma unroll
-                            for (int c = 0; c < 4; ++c) {
-                                const double dt = (double)qr[t][c] - (double)x4[c];
-                                part = fma(dt, dt, part);
-                            }
+                for (int t = 0; t < SB; ++t) rows[t] = b0 + t < ns ? srows[grp][b0 + t] : -1;
+                f32x4 xr[SB][NF4];
+#pragma unroll
+                for (int t = 0; t < SB; ++t) {
+#pragma unroll
+                    for (int tt = 0; tt < NF4; ++tt) {
+                        const int f = (tt * G + gl) * 4;
+                        xr[t][tt] = rows[t] >= 0 && f < d ? *(const f32x4*)(a.X + (size_t)rows[t] * d + f)
+                                                          : f32x4{0.f, 0.f, 0.f, 0.f};
+                    }
+                }
+                double part[SB];
+#pragma unroll
+                for (int t = 0; t < SB; ++t) {
+                    part[t] = 0.0;
+#pragma unroll
+                    for (int tt = 0; tt < NF4; ++tt) {
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) {
+                            const double dt = (double)qr[tt][c] - (double)xr[t][tt][c];
+                            part[t] = fma(dt, dt, part[t]);
                         }
                     }
+                }
 #pragma unroll
-                    for (int o = G / 2; o > 0; o >>= 1) part += __shfl_xor(part, o, G);
-                    const double sj = part;
+                for (int t = 0; t < SB; ++t) {
+#pragma unroll
+                    for (int o = G / 2; o > 0; o >>= 1) part[t] += __shfl_xor(part[t], o, G);
+                }
+#pragma unroll
+                for (int t = 0; t < SB; ++t) {
+                    if (rows[t] < 0) continue;
+                    const double sj = part[t];
                     if (!(sj * (1.0 - g) <= cut)) continue;
                     ++kept;
                     // the interval decides almost every comparison; an undecided one (a
                     // near or exact tie) leaves the query to the lane rank's exact comparator
                     if (best_i < 0 || sj * (1.0 + g) < best_s * (1.0 - g)) {
                         best_s = sj;
-                        best_i = row;
+                        best_i = rows[t];
                     } else if (!(best_s * (1.0 + g) < sj * (1.0 - g))) {
                         tie = true;
                     }
@@ -1056,24 +1095,53 @@ __global__ __launch_bounds__(256) void k_rank1_group(const RankArgs a, int64_t m
             }
         }
     }
-    if (gl != 0) return;
+    if (!live) return;
     if (over || kept < 1) {
-        a.status[q] = ST_OVERFLOW;
-        if (a.qstats) a.qstats[3 * q] = a.qstats[3 * q + 1] = a.qstats[3 * q + 2] = 0;
+        if (gl == 0) {
+            a.status[q] = ST_OVERFLOW;
+            if (a.qstats) a.qstats[3 * q] = a.qstats[3 * q + 1] = a.qstats[3 * q + 2] = 0;
+        }
         return;
     }
     if (tie) {
-        a.status[q] = ST_TIE;
+        if (gl == 0) a.status[q] = ST_TIE;
         return;
     }
-    if (a.qstats) {
-        a.qstats[3 * q] = listed;
-        a.qstats[3 * q + 1] = count;
-        a.qstats[3 * q + 2] = kept;
+    if (gl == 0) {
+        if (a.qstats) {
+            a.qstats[3 * q] = listed;
+            a.qstats[3 * q + 1] = count;
+            a.qstats[3 * q + 2] = kept;
+        }
+        a.out_idx[q] = best_i;
     }
-    a.out_idx[q] = best_i;
-    if (a.out_dist) {
-        const double sc = canonical_sq4(qv, a.X + (size_t)best_i * d, d);
+    if (!a.out_dist) return;
+    if constexpr (LDS_SUM) {
+        // canonical products by the lanes owning the features (the same roundings
+        // as canonical_sq4), summed left to right by the group's first lane
+        const float* xb = a.X + (size_t)best_i * d;
+#pragma unroll
+        for (int t = 0; t < NF4; ++t) {
+            const int f = (t * G + gl) * 4;
+            if (f < d) {
+                const f32x4 x4 = *(const f32x4*)(xb + f);
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const double dt = (double)qr[t][c] - (double)x4[c];
+                    sprod[grp][f + c] = mul_rn(dt, dt);
+                }
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (gl == 0) {
+            double sc = 0.0;
+            for (int i = 0; i < d; ++i) sc = add_rn(sc, sprod[grp][i]);
+            a.out_dist[q] = a.squared ? sc : sqrt(sc);
+        }
+    } else if (gl == 0) {
+        const double sc = canonical_sq4(a.Q + q * d, a.X + (size_t)best_i * d, d);
         a.out_dist[q] = a.squared ? sc : sqrt(sc);
     }
 }
@@ -1615,7 +1683,10 @@ static int knn_enqueue(mlk_index* index, const float* Q, int64_t m, int k, int64
         ra.qstats = qstats;  // written for every query by k_rank
         const bool vec = d % 4 == 0 && ((uintptr_t)Q % 16) == 0 && ((uintptr_t)X % 16) == 0;
         if (k == 1 && mr.n_unsafe == 0 && vec && d <= 256) {
-            hipLaunchKernelGGL(d <= 64 ? k_rank1_group<2> : d <= 128 ? k_rank1_group<4> : k_rank1_group<8>,
+            hipLaunchKernelGGL(d <= 64    ? k_rank1_group<2>
+                               : d <= 96  ? k_rank1_group<3>
+                               : d <= 128 ? k_rank1_group<4>
+                                          : k_rank1_group<8>,
                                dim3((unsigned)((m + 31) / 32)), dim3(256), 0, stream, ra, m);
             hipLaunchKernelGGL(k_rank1_lane<true>, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, stream, ra, m, 1);
         } else if (k == 1 && mr.n_unsafe == 0)
