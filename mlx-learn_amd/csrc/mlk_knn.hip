@@ -294,12 +294,27 @@ __global__ void k_merge(int64_t m, int k, const int* __restrict__ cnt, int* __re
     const int64_t q = (int64_t)blockIdx.x * 4 + w;
     if (q >= m) return;
     if (status[q]) return;
+    // sub-list bounds [seen, min(count, cap)), two sub-lists per lane (s0 + lane,
+    // s0 + lane + 64); the first 128 stay in registers for the append pass
     int fresh = 0;
     bool over = false;
-    for (int s = lane; s < S2; s += 64) {
-        const int cs = cnt[q * S2 + s];
-        over |= cs > cap;
-        fresh += (cs < cap ? cs : cap) - seen[q * S2 + s];
+    int lo0[2] = {0, 0}, hi0[2] = {0, 0};
+    for (int s0 = 0; s0 < S2; s0 += 128) {
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int s = s0 + lane + 64 * t;
+            if (s < S2) {
+                const int cs = cnt[q * S2 + s];
+                const int sn = seen[q * S2 + s];
+                over |= cs > cap;
+                const int hi = cs < cap ? cs : cap;
+                fresh += hi - sn;
+                if (s0 == 0) {
+                    lo0[t] = sn;
+                    hi0[t] = hi;
+                }
+            }
+        }
     }
     for (int o = 32; o > 0; o >>= 1) {
         fresh += __shfl_xor(fresh, o);
@@ -318,42 +333,68 @@ __global__ void k_merge(int64_t m, int k, const int* __restrict__ cnt, int* __re
         if (rows) rbuf[j] = kbest_row[q * k + j];
     }
     int base = k;
-    for (int s0 = 0; s0 < S2; s0 += 64) {
-        const int s = s0 + lane;
-        int lo = 0, hi = 0;
-        if (s < S2) {
-            const int cs = cnt[q * S2 + s];
-            hi = cs < cap ? cs : cap;
-            lo = seen[q * S2 + s];
-            seen[q * S2 + s] = hi;
-        }
-        const int mine = hi - lo;
-        int incl = mine;
-        for (int o = 1; o < 64; o <<= 1) {
-            const int v = __shfl_up(incl, o);
-            if (lane >= o) incl += v;
-        }
-        int at = base + incl - mine;
-        const double nqq = nq[q];
-        const size_t off = (size_t)q * cap * S2 + s;  // entry j at off + j * S2
-        for (int j0 = lo; j0 < hi; j0 += 8) {
-            // eight entries per batch: loads and norm gathers issued together
-            int2 e[8];
-            float nxv[8];
+    const double nqq = nq[q];
+    for (int s0 = 0; s0 < S2; s0 += 128) {
+        int lo[2], hi[2], at[2];
+        size_t off[2];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) e[u] = cand[off + (size_t)(j0 + u < hi ? j0 + u : hi - 1) * S2];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) nxv[u] = nx32[e[u].x];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                if (j0 + u >= hi) continue;
-                double L, U;
-                hit_bounds(__int_as_float(e[u].y), nxv[u], nqq, c, L, U);
-                if (rows) rbuf[at] = e[u].x;
-                buf[at++] = __float_as_uint(fmaxf(__double2float_ru(U), 0.0f));
+        for (int t = 0; t < 2; ++t) {
+            const int s = s0 + lane + 64 * t;
+            lo[t] = hi[t] = 0;
+            if (s0 == 0) {
+                lo[t] = lo0[t];
+                hi[t] = hi0[t];
+            } else if (s < S2) {
+                const int cs = cnt[q * S2 + s];
+                hi[t] = cs < cap ? cs : cap;
+                lo[t] = seen[q * S2 + s];
             }
+            if (s < S2) seen[q * S2 + s] = hi[t];
+            off[t] = (size_t)q * cap * S2 + s;  // entry j at off + j * S2
         }
-        base += __shfl(incl, 63);
+        // positions in sub-list order: s0 .. s0 + 63 (t = 0), then s0 + 64 .. (t = 1)
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int mine = hi[t] - lo[t];
+            int incl = mine;
+            for (int o = 1; o < 64; o <<= 1) {
+                const int v = __shfl_up(incl, o);
+                if (lane >= o) incl += v;
+            }
+            at[t] = base + incl - mine;
+            base += __shfl(incl, 63);
+        }
+        const int nmax = hi[0] - lo[0] > hi[1] - lo[1] ? hi[0] - lo[0] : hi[1] - lo[1];
+        for (int j0 = 0; j0 < nmax; j0 += 4) {
+            // four entries of each sub-list per batch: loads and norm gathers
+            // issued together, from valid slots (clamped to the sub-list's last
+            // new entry; a sub-list without new entries borrows the other's)
+            int2 e[2][4];
+            float nxv[2][4];
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const int tt = hi[t] > lo[t] ? t : 1 - t;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int j = lo[tt] + j0 + u < hi[tt] ? lo[tt] + j0 + u : hi[tt] - 1;
+                    e[t][u] = cand[off[tt] + (size_t)j * S2];
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < 2; ++t)
+#pragma unroll
+                for (int u = 0; u < 4; ++u) nxv[t][u] = nx32[e[t][u].x];
+#pragma unroll
+            for (int t = 0; t < 2; ++t)
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (lo[t] + j0 + u >= hi[t]) continue;
+                    double L, U;
+                    hit_bounds(__int_as_float(e[t][u].y), nxv[t][u], nqq, c, L, U);
+                    if (rows) rbuf[at[t]] = e[t][u].x;
+                    buf[at[t]++] = __float_as_uint(fmaxf(__double2float_ru(U), 0.0f));
+                }
+        }
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -407,30 +448,42 @@ __global__ void k_merge(int64_t m, int k, const int* __restrict__ cnt, int* __re
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    // float64 distances of the new rows, 16 lanes per row (any summation
-    // order); eight rows per pass, the loads of both groups of four issued
-    // before either sum
+    // float64 distances of the new rows, 8 lanes per row (any summation
+    // order); sixteen rows per pass, the loads of a 64-feature slice of both
+    // groups of eight issued before either sum
     const double g = f64_interval(d);
-    const int gl = lane & 15, grp = lane >> 4;
+    const int gl = lane & 7, grp = lane >> 3;
     const float* qv = Q + q * d;
-    for (int j0 = 0; j0 < K2; j0 += 8) {
-        const int ja = j0 + grp, jb = j0 + 4 + grp;
+    for (int j0 = 0; j0 < K2; j0 += 16) {
+        const int ja = j0 + grp, jb = j0 + 8 + grp;
         const int ra = ja < K2 ? kr[ja] : -1, rb = jb < K2 ? kr[jb] : -1;
         if (!ballot(ra >= 0 || rb >= 0)) continue;
         const float* xa = X + (size_t)(ra >= 0 ? ra : 0) * d;
         const float* xb = X + (size_t)(rb >= 0 ? rb : 0) * d;
         double pa = 0.0, pb = 0.0;
-#pragma unroll 4
-        for (int e = gl; e < d; e += 16) {
-            const double qe = (double)qv[e];
-            const float va = xa[e], vb = xb[e];
-            const double ta = qe - (double)va, tb = qe - (double)vb;
-            pa += ta * ta;
-            pb += tb * tb;
+        for (int e0 = 0; e0 < d; e0 += 64) {
+            // unconditional loads (features clamped to d - 1); padded terms zeroed
+            float qq[8], va[8], vb[8];
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const int e = e0 + gl + 8 * t;
+                const int ec = e < d ? e : d - 1;
+                qq[t] = qv[ec];
+                va[t] = xa[ec];
+                vb[t] = xb[ec];
+            }
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const bool in = e0 + gl + 8 * t < d;
+                const double ta = in ? (double)qq[t] - (double)va[t] : 0.0;
+                const double tb = in ? (double)qq[t] - (double)vb[t] : 0.0;
+                pa += ta * ta;
+                pb += tb * tb;
+            }
         }
-        for (int o = 8; o > 0; o >>= 1) {
-            pa += __shfl_xor(pa, o, 16);
-            pb += __shfl_xor(pb, o, 16);
+        for (int o = 4; o > 0; o >>= 1) {
+            pa += __shfl_xor(pa, o, 8);
+            pb += __shfl_xor(pb, o, 8);
         }
         if (gl == 0) {
             if (ra >= 0) {
@@ -610,37 +663,56 @@ __device__ void rank_query(const RankArgs& a, int64_t item, int64_t q, int* __re
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         int listed = 0;
         bool over = false;
-        for (int s0 = 0; s0 < a.S; s0 += 64) {
-            const int s = s0 + lane;
-            int cs = 0;
-            if (s < a.S) {
-                cs = a.cnt[q * a.S + s];
-                over |= cs > a.cap;
-                cs = cs < a.cap ? cs : a.cap;
+        // two sub-lists per lane (s0 + lane and s0 + lane + 64): their counts,
+        // then their entries, then the entries' norms are loaded together
+        for (int s0 = 0; s0 < a.S; s0 += 128) {
+            int cs[2];
+            size_t off[2];
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const int s = s0 + lane + 64 * t;
+                cs[t] = 0;
+                if (s < a.S) {
+                    cs[t] = a.cnt[q * a.S + s];
+                    over |= cs[t] > a.cap;
+                    cs[t] = cs[t] < a.cap ? cs[t] : a.cap;
+                }
+                listed += cs[t];
+                off[t] = (size_t)q * a.cap * a.S + s;  // entry j at off + j * S
             }
-            listed += cs;
-            const size_t off = (size_t)q * a.cap * a.S + s;  // entry j at off + j * S
-            // eight entries at a time: their loads and norm gathers are issued
-            // together, so a batch costs one memory latency per step
-            for (int j0 = 0; j0 < cs; j0 += 8) {
-                // unconditional loads from clamped (valid) slots, so the
-                // compiler issues them together; slots past cs are ignored
-                int2 e[8];
-                float nxv[8];
+            const int cm = cs[0] > cs[1] ? cs[0] : cs[1];
+            for (int j0 = 0; j0 < cm; j0 += 8) {
+                // unconditional loads from valid slots (a sub-list's slot j is
+                // clamped to its last entry; an empty sub-list borrows the other
+                // one's), so the compiler issues them together; slots past the
+                // counts are ignored
+                int2 e[2][8];
+                float nxv[2][8];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) e[u] = a.cand[off + (size_t)(j0 + u < cs ? j0 + u : cs - 1) * a.S];
+                for (int t = 0; t < 2; ++t) {
+                    const int tt = cs[t] > 0 ? t : 1 - t;
 #pragma unroll
-                for (int u = 0; u < 8; ++u) nxv[u] = a.nx32[e[u].x];
-#pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    if (j0 + u >= cs) continue;
-                    double L, U;
-                    hit_bounds(__int_as_float(e[u].y), nxv[u], nqq, a.c, L, U);
-                    if (L <= (double)tau) {
-                        const int at = atomicAdd(sN, 1);
-                        if (at < LC) li[at] = e[u].x;
+                    for (int u = 0; u < 8; ++u) {
+                        const int jc = j0 + u < cs[tt] ? j0 + u : cs[tt] - 1;
+                        e[t][u] = a.cand[off[tt] + (size_t)jc * a.S];
                     }
                 }
+#pragma unroll
+                for (int t = 0; t < 2; ++t)
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) nxv[t][u] = a.nx32[e[t][u].x];
+#pragma unroll
+                for (int t = 0; t < 2; ++t)
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        if (j0 + u >= cs[t]) continue;
+                        double L, U;
+                        hit_bounds(__int_as_float(e[t][u].y), nxv[t][u], nqq, a.c, L, U);
+                        if (L <= (double)tau) {
+                            const int at = atomicAdd(sN, 1);
+                            if (at < LC) li[at] = e[t][u].x;
+                        }
+                    }
             }
         }
         for (int o = 32; o > 0; o >>= 1) over = __shfl_xor((int)over, o) | over;
@@ -664,42 +736,42 @@ __device__ void rank_query(const RankArgs& a, int64_t item, int64_t q, int* __re
         count = total + extra;
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        // float64 distances, 16 lanes per candidate (any summation order: the
+        // float64 distances, 8 lanes per candidate (any summation order: the
         // interval tests use the order-free bound g)
         const double cut = (double)tau / a.c.scale2;
         int kept = 0;
-        const int gl = lane & 15;
-        for (int b = 0; b < count; b += 8) {
-            // 16 lanes per candidate, two candidates per lane group; all loads
+        const int gl = lane & 7;
+        for (int b = 0; b < count; b += 16) {
+            // 8 lanes per candidate, two candidates per lane group; all loads
             // of a 128-feature slice are issued before the arithmetic
-            const int j0 = b + (lane >> 4), j1 = j0 + 4;
+            const int j0 = b + (lane >> 3), j1 = j0 + 8;
             const float* x0 = a.X + (size_t)(j0 < count ? li[j0] : 0) * d;
             const float* x1 = a.X + (size_t)(j1 < count ? li[j1] : 0) * d;
             double s0 = 0.0, s1 = 0.0;
             for (int e0 = 0; e0 < d; e0 += 128) {
                 // unconditional loads (features clamped to d - 1, rows to row 0
                 // past the list); padded terms are zeroed after the loads
-                float qq[8], xa[8], xb[8];
+                float qq[16], xa[16], xb[16];
 #pragma unroll
-                for (int t = 0; t < 8; ++t) {
-                    const int e = e0 + gl + 16 * t;
+                for (int t = 0; t < 16; ++t) {
+                    const int e = e0 + gl + 8 * t;
                     const int ec = e < d ? e : d - 1;
                     qq[t] = qv[ec];
                     xa[t] = x0[ec];
                     xb[t] = x1[ec];
                 }
 #pragma unroll
-                for (int t = 0; t < 8; ++t) {
-                    const bool in = e0 + gl + 16 * t < d;
+                for (int t = 0; t < 16; ++t) {
+                    const bool in = e0 + gl + 8 * t < d;
                     const double ta = in ? (double)qq[t] - (double)xa[t] : 0.0;
                     const double tb = in ? (double)qq[t] - (double)xb[t] : 0.0;
                     s0 += ta * ta;
                     s1 += tb * tb;
                 }
             }
-            for (int o = 8; o > 0; o >>= 1) {
-                s0 += __shfl_xor(s0, o, 16);
-                s1 += __shfl_xor(s1, o, 16);
+            for (int o = 4; o > 0; o >>= 1) {
+                s0 += __shfl_xor(s0, o, 8);
+                s1 += __shfl_xor(s1, o, 8);
             }
             if (gl == 0) {
                 if (j0 < count) {
@@ -811,9 +883,29 @@ __device__ void rank_query(const RankArgs& a, int64_t item, int64_t q, int* __re
     for (int e = lane; e < d; e += 64) sq_[e] = qv[e];
     for (int r0 = 0; r0 < k; r0 += ch) {
         const int nr = k - r0 < ch ? k - r0 : ch;
-        for (int rr = 0; rr < nr; ++rr) {
-            const float* xv = a.X + (size_t)orow[r0 + rr] * d;
-            for (int e = lane; e < d; e += 64) stage[rr * rs + e] = xv[e];
+        // 8 rows x 128 features per batch, all loads issued before the stores
+        // (rows clamped to the chunk's last, features to d - 1)
+        for (int e0 = 0; e0 < d; e0 += 128) {
+            for (int rr0 = 0; rr0 < nr; rr0 += 8) {
+                float v[8][2];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int rr = rr0 + u < nr ? rr0 + u : nr - 1;
+                    const float* xv = a.X + (size_t)orow[r0 + rr] * d;
+#pragma unroll
+                    for (int h2 = 0; h2 < 2; ++h2) {
+                        const int e = e0 + lane + 64 * h2;
+                        v[u][h2] = xv[e < d ? e : d - 1];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+#pragma unroll
+                    for (int h2 = 0; h2 < 2; ++h2) {
+                        const int e = e0 + lane + 64 * h2;
+                        if (rr0 + u < nr && e < d) stage[(rr0 + u) * rs + e] = v[u][h2];
+                    }
+            }
         }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
