@@ -1891,6 +1891,7 @@ static int knn_complete(mlk_index* index, const KnnCall& call, hipStream_t strea
     st.fallback_queries = nf;
     st.candidates = (int64_t)words.cnts[0];
     st.reranked = (int64_t)words.cnts[2];
+    if (trial_abandon_fallback(nf, m)) return MLK_OK;  // autotuner trial: rejected, outputs unused
     if (nf) {
         std::sort(fq_host.begin(), fq_host.end());
         MLK_CHECK_HIP(hipMemcpyAsync(fq, fq_host.data(), nf * sizeof(int), hipMemcpyHostToDevice, stream));

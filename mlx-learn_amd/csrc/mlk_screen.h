@@ -257,4 +257,10 @@ int knn_capacity(int prec, int splits, int k);
 // *tuned reports whether a cached plan was used.
 Plan resolve_plan(int kind, int prec, int dp, int64_t n, int64_t m, int k, bool* tuned = nullptr);
 
+// An autotuner trial call on this thread whose exact fallback would take nf of
+// its m queries (more than m / 256 + 16, a cost no plan recovers from) is cut
+// short: the call skips the fallback (its outputs are the tuner's scratch) and
+// the tuner rejects the plan. Returns whether the call is to stop there.
+bool trial_abandon_fallback(int64_t nf, int64_t m);
+
 }  // namespace mlk

@@ -91,6 +91,14 @@ struct TunedEntry {
 static std::mutex g_plan_lock;
 static std::map<PlanKey, TunedEntry> g_plans;
 static thread_local const Plan* t_trial = nullptr;
+static thread_local bool t_trial_abandoned = false;
+static thread_local bool t_trial_may_abandon = false;  // alternatives only, never the starting plan
+
+bool trial_abandon_fallback(int64_t nf, int64_t m) {
+    if (!t_trial || !t_trial_may_abandon || nf <= m / 256 + 16) return false;
+    t_trial_abandoned = true;
+    return true;
+}
 
 static bool lookup(const PlanKey& key, TunedEntry* out) {
     std::lock_guard<std::mutex> g(g_plan_lock);
@@ -309,17 +317,24 @@ int autotune_impl(mlk_index* index, const float* Q, int64_t m, int k, double R2,
         return rc;
     };
     int trials = 0;
-    auto time_plan = [&](const Plan& p, double& ms, int64_t& fallback) -> int {
+    auto time_plan = [&](const Plan& p, double& ms, int64_t& fallback, bool may_abandon) -> int {
         ms = std::numeric_limits<double>::infinity();
+        t_trial_abandoned = false;
+        t_trial_may_abandon = may_abandon;
         for (int rep = 0; rep < 2; ++rep) {
             MLK_CHECK_HIP(hipEventRecord(e0, stream));
             MLK_TRY(call(p));
             MLK_CHECK_HIP(hipEventRecord(e1, stream));
             MLK_CHECK_HIP(hipEventSynchronize(e1));
+            if (t_trial_abandoned) {  // too many fallback queries: the plan loses
+                ms = std::numeric_limits<double>::infinity();
+                break;
+            }
             float t = 0.0f;
             MLK_CHECK_HIP(hipEventElapsedTime(&t, e0, e1));
             ms = std::min(ms, (double)t);
         }
+        t_trial_may_abandon = false;
         fallback = stats().fallback_queries;
         ++trials;
         return MLK_OK;
@@ -329,7 +344,7 @@ int autotune_impl(mlk_index* index, const float* Q, int64_t m, int k, double R2,
     Plan best = base;
     double best_ms = 0.0;
     int64_t best_fb = 0;
-    MLK_TRY(time_plan(best, best_ms, best_fb));
+    MLK_TRY(time_plan(best, best_ms, best_fb, false));
     const double default_ms = best_ms;
     const double mem_limit = std::max(4.0 * list_bytes(base, m), 1024.0 * 1024.0 * 1024.0);
     auto consider = [&](const Plan& p) -> int {
@@ -338,7 +353,7 @@ int autotune_impl(mlk_index* index, const float* Q, int64_t m, int k, double R2,
         if (kind == PLAN_KNN && list_bytes(p, m) > mem_limit) return 0;
         double ms = 0.0;
         int64_t fb = 0;
-        MLK_TRY(time_plan(p, ms, fb));
+        MLK_TRY(time_plan(p, ms, fb, true));
         if (ms < 0.98 * best_ms) {
             best = p;
             best_ms = ms;
