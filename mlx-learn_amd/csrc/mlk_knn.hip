@@ -1721,8 +1721,8 @@ static int knn_enqueue(mlk_index* index, const float* Q, int64_t m, int k, int64
         sa.QB = (int)QB;
         sa.self_off = q_offset;
         sa.exclude = exclude;
-        sa.nchunk = screen_chunks(prec, dp);
-        sa.qgroup = sa.nchunk > 1 ? DEEP_QGROUP : 1;
+        sa.nchunk = screen_nchunk(SCREEN_LISTS, prec, dp, plan.shape);
+        sa.qgroup = screen_qgroup(prec, dp);
         const unsigned blocks = (unsigned)screen_grid(sa);
         auto merge = [&]() -> int {
             if (lane_merge)

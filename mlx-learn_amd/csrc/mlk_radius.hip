@@ -641,8 +641,8 @@ int radius_query_impl(mlk_index* index, const float* Q, int64_t m, double R2, in
             sa.QB = (int)QB;
             sa.self_off = q_offset;
             sa.exclude = exclude;
-            sa.nchunk = screen_chunks(prec, dp);
-            sa.qgroup = sa.nchunk > 1 ? DEEP_QGROUP : 1;
+            sa.nchunk = screen_nchunk(SCREEN_PAIRS, prec, dp, plan.shape);
+            sa.qgroup = screen_qgroup(prec, dp);
             sa.pairs = ws[9].as<int2>();
             sa.qmap = qmap;
             sa.pair_count = dcount;

@@ -213,6 +213,18 @@ inline int screen_chunk(int prec, int dp) {
 }
 inline int screen_chunks(int prec, int dp) { return dp / screen_chunk(prec, dp); }
 
+// Chunks per row of the kernel that a plan's shape selects: the deep kernels
+// walk rows in chunks (query fragments reloaded per chunk); wide bf16 / fp16
+// rows screened for k-NN lists with two stages are split in two chunks whose
+// query fragments all stay in registers (stage c multiplies chunk c).
+inline int screen_nchunk(int mode, int prec, int dp, ScreenShape shape) {
+    if (prec != PREC_FP32 && dp > 128 && dp <= WIDE_MAX) return mode == SCREEN_LISTS && shape.ns == 2 ? 2 : 1;
+    return screen_chunks(prec, dp);
+}
+// Query blocks per XCD block group: the deep kernels, which reload query
+// fragments per chunk, group their blocks so resident ones share query rows.
+inline int screen_qgroup(int prec, int dp) { return screen_chunks(prec, dp) > 1 ? DEEP_QGROUP : 1; }
+
 // The screen precision a request runs: every request runs the precision it
 // names, at every width, for k-NN and radius search alike.
 inline int effective_prec(int prec, int d) {

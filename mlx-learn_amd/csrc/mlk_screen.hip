@@ -108,8 +108,14 @@ __device__ __forceinline__ void tile_barrier() { asm volatile("s_barrier" ::: "m
 // multiplied and tested one after the other, so one accumulator set is live
 // and the step's barrier and tile load serve twice the MFMA work; their A
 // fragments are read three k-steps ahead of the MFMAs that use them.
-template <int PREC, int DP, int QT, int RTW, int NS, int MODE, bool CH = false>
-__global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
+// Chunked rows (CH): step u multiplies feature chunk u % nc of tile u / nc and
+// the accumulators carry over a tile's chunks. Deep rows reload the query
+// fragments of each chunk; with RC = NS resident chunks (wide rows in two
+// halves) the fragments of every chunk stay in registers and LDS stage c
+// always holds chunk c, so the chunk index is a compile-time constant.
+template <int PREC, int DP, int QT, int RTW, int NS, int MODE, bool CH = false, int RC = 1>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RC > 1 ? 2 : 1))) void k_screen(
+    const ScreenArgs a) {
     // the body uses device-only types (buffer resources): compile it in the
     // device pass only, so the host pass still emits the kernel's launch stub
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -129,6 +135,7 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
     static_assert(QT == 1 || QT == 2 || QT == 4, "query tiles");
     static_assert(RTW == 1 || RTW == 2, "row tiles per step");
     static_assert(!CH || NS == 2, "deep rows: two stages (each step waits for its query reload)");
+    static_assert(RC == 1 || (CH && RC == NS), "resident chunks: one LDS stage per chunk");
     static_assert(NS * (ABYTES + 512) + (MODE == SCREEN_PAIRS ? NW * PAIR_STAGE * 8 : 0) <= 160 * 1024,
                   "LDS of one block");
     // distinct buffers (not one array) so the compiler can tell the tile
@@ -210,9 +217,11 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
 
     constexpr int NB = PREC == PREC_FP32 ? DP / 2 : DP / 16;
     typedef typename std::conditional<PREC == PREC_FP32, float, i32x4>::type BFrag;
-    BFrag bq[QT][NB];
+    // query fragments: one chunk (deep rows reload them per chunk), or all RC
+    // chunks of the row (wide rows split in RC chunks: stage c holds chunk c)
+    BFrag bq[QT][NB * RC];
     // deep rows: fragments of feature chunk cc into dst
-    auto load_query = [&](BFrag(&dst)[QT][NB], int cc) {
+    auto load_query = [&](BFrag(&dst)[QT][NB * RC], int cc) {
 #pragma unroll
         for (int qt = 0; qt < QT; ++qt) {
             const T* qp = (const T*)a.qs + (size_t)(qrow0 + qt * 32 + col) * qstride + (size_t)cc * DP + h * (DP / 2);
@@ -242,7 +251,9 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
             }
         } else {
 #pragma unroll
-            for (int j = 0; j < NB; ++j) bq[qt][j] = *(const i32x4*)(qp + 8 * j);
+            for (int c = 0; c < RC; ++c)
+#pragma unroll
+                for (int j = 0; j < NB; ++j) bq[qt][c * NB + j] = *(const i32x4*)(qp + c * DP + 8 * j);
         }
     }
 
@@ -324,6 +335,7 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
     auto step = [&](auto CUR, int64_t u) {
         constexpr int c = decltype(CUR)::value;
         constexpr int nxt = (c + NS - 1) % NS;
+        constexpr int cb = RC > 1 ? c * NB : 0;  // resident chunks: stage c multiplies chunk c
         const bool full = u + NS - 1 < nsteps;
         const int64_t t = CH ? u / nc : u;
         const int cc = CH ? (int)(u - t * nc) : 0;  // feature chunk of this step
@@ -366,16 +378,29 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
 #pragma unroll
                     for (int j = 0; j < NB; ++j) av[j] = *(const i32x4*)(arow + j * 16);
 #pragma unroll
-                    for (int qt = 0; qt < QT; ++qt) acc[rt][qt] = mfma16<PREC>(av[0], bq[qt][0], init);
+                    for (int qt = 0; qt < QT; ++qt) acc[rt][qt] = mfma16<PREC>(av[0], bq[qt][cb], init);
 #pragma unroll
                     for (int j = 1; j < NB; ++j)
 #pragma unroll
-                        for (int qt = 0; qt < QT; ++qt) acc[rt][qt] = mfma16<PREC>(av[j], bq[qt][j], acc[rt][qt]);
+                        for (int qt = 0; qt < QT; ++qt) acc[rt][qt] = mfma16<PREC>(av[j], bq[qt][cb + j], acc[rt][qt]);
                     __builtin_amdgcn_sched_group_barrier(0x100, 4 + PF, 0);
 #pragma unroll
                     for (int j = 0; j < NB; ++j) {
                         __builtin_amdgcn_sched_group_barrier(0x008, QT, 0);
                         if (j + PF < NB) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+                    }
+                } else if constexpr (RC > 1) {
+                    // resident chunks: the biases start the accumulators in the
+                    // first chunk only (no separate bias registers stay live)
+                    if (cc == 0) {
+#pragma unroll
+                        for (int qt = 0; qt < QT; ++qt) acc[rt][qt] = init;
+                    }
+#pragma unroll
+                    for (int j = 0; j < NB; ++j) {
+                        const i32x4 av = *(const i32x4*)(arow + j * 16);
+#pragma unroll
+                        for (int qt = 0; qt < QT; ++qt) acc[rt][qt] = mfma16<PREC>(av, bq[qt][cb + j], acc[rt][qt]);
                     }
                 } else {
                     // the first k-step reads the bias registers as its C operand
@@ -383,12 +408,12 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
                     const i32x4 a0 = *(const i32x4*)(arow);
 #pragma unroll
                     for (int qt = 0; qt < QT; ++qt)
-                        acc[rt][qt] = mfma16<PREC>(a0, bq[qt][0], cc == 0 ? init : acc[rt][qt]);
+                        acc[rt][qt] = mfma16<PREC>(a0, bq[qt][cb], cc == 0 ? init : acc[rt][qt]);
 #pragma unroll
                     for (int j = 1; j < NB; ++j) {
                         const i32x4 av = *(const i32x4*)(arow + j * 16);
 #pragma unroll
-                        for (int qt = 0; qt < QT; ++qt) acc[rt][qt] = mfma16<PREC>(av, bq[qt][j], acc[rt][qt]);
+                        for (int qt = 0; qt < QT; ++qt) acc[rt][qt] = mfma16<PREC>(av, bq[qt][cb + j], acc[rt][qt]);
                     }
                 }
             };
@@ -480,7 +505,7 @@ __global__ __launch_bounds__(256) void k_screen(const ScreenArgs a) {
                 for (int rt = 0; rt < RTW; ++rt) mfma_rt(rt);
                 // deep rows: the next step's fragments replace this step's once its
                 // MFMAs are issued (two stages: the step's final wait covers them)
-                if constexpr (CH) {
+                if constexpr (CH && RC == 1) {
                     if (u + 1 < nsteps) load_query(bq, cc + 1 == nc ? 0 : cc + 1);
                 }
                 // deep rows: the test only after the tile's last chunk
@@ -764,14 +789,29 @@ static ScreenFn pick_pp(int dp) {
 }
 
 // Wide rows (bf16 / fp16, padded width in {256, 384, 512, 640, 800}): one
-// query tile per wave (its fragments for the whole width stay in registers),
-// 32-row tiles; three LDS stages for k-NN lists, two beside the pair staging.
+// query tile per wave, its fragments for the whole width in registers, 32-row
+// tiles. Shape ns = 3: whole rows per LDS stage (three stages for k-NN lists,
+// two beside the pair staging; 51 KB stages at 800 features, so one block per
+// CU). Shape ns = 2: rows split in two chunks, stage c holding chunk c of a
+// tile (26 KB stages at 800 features), the accumulators carried over the two
+// chunks, at most 256 registers: two blocks per CU (k-NN lists only: beside
+// the pair staging the 800-feature kernel would spill).
 // Deeper rows (multiples of DEEP_CHUNK features): the deep kernel walks each
 // 64-row tile in DEEP_CHUNK-feature chunks, reloading the query fragments of
 // each chunk (they do not fit in registers for the whole row).
 template <int PREC, int MODE>
-static ScreenFn pick_wide(int dp) {
+static ScreenFn pick_wide(int dp, int ns) {
     constexpr int NS = MODE == SCREEN_PAIRS ? 2 : 3;
+    if constexpr (MODE == SCREEN_LISTS) {
+        if (ns == 2) switch (dp) {
+            case 256: return k_screen<PREC, 128, 1, 1, 2, MODE, true, 2>;
+            case 384: return k_screen<PREC, 192, 1, 1, 2, MODE, true, 2>;
+            case 512: return k_screen<PREC, 256, 1, 1, 2, MODE, true, 2>;
+            case 640: return k_screen<PREC, 320, 1, 1, 2, MODE, true, 2>;
+            case 800: return k_screen<PREC, 400, 1, 1, 2, MODE, true, 2>;
+            default: break;
+        }
+    }
     switch (dp) {
         case 256: return k_screen<PREC, 256, 1, 1, NS, MODE>;
         case 384: return k_screen<PREC, 384, 1, 1, NS, MODE>;
@@ -792,7 +832,7 @@ static ScreenFn pick_shape(int dp, int qt, int ns) {
         // row, the bf16 deep chunk's size), 64-row tiles, two stages
         if constexpr (PREC == PREC_FP32)
             return dp % FP32_CHUNK == 0 ? k_screen<PREC, FP32_CHUNK, 1, 2, 2, MODE, true> : nullptr;
-        else return pick_wide<PREC, MODE>(dp);
+        else return pick_wide<PREC, MODE>(dp, ns);
     }
     if constexpr (PREC == PREC_FP32) {
         // float32 operands: two query tiles would need 128 registers of fragments
@@ -830,7 +870,9 @@ int screen_dp(int prec, int d) {
 // (2.07 ms per call) beat one (2.28 ms); float32 operands keep one (two would
 // need 128 registers of query fragments), and so do radius pairs.
 ScreenShape default_shape(int mode, int prec, int dp) {
-    if (prec != PREC_FP32 && dp > 128 && dp <= WIDE_MAX) return ScreenShape{1, 3};
+    // wide k-NN rows: two chunks per row, two blocks per CU (60k x 784 pixels, 10,000
+    // queries: 1.36 ms per call against 2.10 ms with whole-row stages)
+    if (prec != PREC_FP32 && dp > 128 && dp <= WIDE_MAX) return ScreenShape{1, mode == SCREEN_LISTS ? 2 : 3};
     if (mode == SCREEN_LISTS && prec != PREC_FP32 && dp <= 128) return ScreenShape{2, 2};
     return ScreenShape{1, 2};
 }
@@ -848,9 +890,13 @@ ScreenFn pick_screen(int mode, int prec, int dp, ScreenShape shape) {
 
 std::vector<ScreenShape> screen_shapes(int mode, int prec, int dp) {
     std::vector<ScreenShape> out;
-    if (dp > 128) {  // wide rows: one shape, whatever is asked for
-        const ScreenShape only = default_shape(mode, prec, dp);
-        if (pick_screen(mode, prec, dp, only)) out.push_back(only);
+    if (dp > 128) {  // wide rows: whole-row stages and two-chunk stages; deeper rows: one shape
+        const ScreenShape first = default_shape(mode, prec, dp);
+        if (pick_screen(mode, prec, dp, first)) out.push_back(first);
+        if (mode == SCREEN_LISTS && prec != PREC_FP32 && dp <= WIDE_MAX) {
+            const ScreenShape other{1, first.ns == 2 ? 3 : 2};
+            if (pick_screen(mode, prec, dp, other)) out.push_back(other);
+        }
         return out;
     }
     for (int qt : {1, 2, 4})

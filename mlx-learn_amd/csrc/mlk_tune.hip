@@ -114,9 +114,12 @@ Plan resolve_plan(int kind, int prec, int dp, int64_t n, int64_t m, int k, bool*
     const Tuning t = tuning_snapshot();
     ScreenShape knob{0, 0};
     if (kind == PLAN_KNN && dp <= 128 && (t.query_tiles || t.stages)) {
-        knob = default_shape(SCREEN_LISTS, prec, dp);  // wide rows have one shape
+        knob = default_shape(SCREEN_LISTS, prec, dp);
         if (t.query_tiles) knob.qt = t.query_tiles;
         if (t.stages) knob.ns = t.stages;
+    } else if (kind == PLAN_KNN && prec != PREC_FP32 && dp <= WIDE_MAX && t.stages) {
+        // wide rows: one query tile; two stages = rows in two chunks, three = whole rows
+        knob = ScreenShape{1, t.stages};
     }
     Plan p;
     TunedEntry e;

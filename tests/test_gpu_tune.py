@@ -62,6 +62,32 @@ def test_every_screen_shape_matches_oracle(mlk, precision):
             index.close()
 
 
+@pytest.mark.parametrize("precision", ["bfloat16", "float16"])
+def test_wide_shapes_match_oracle(mlk, precision):
+    """Rows wider than 128 features in both wide shapes: whole rows per LDS stage (three stages) and
+    rows split in two chunks whose query fragments all stay in registers (two stages), at every wide
+    width, with ragged row and query counts, duplicated rows and queries equal to rows."""
+    from mlxlearn_amd import _lib
+    r = np.random.default_rng(4500)
+    for d in (129, 300, 384, 500, 640, 784, 800):
+        X = r.normal(size=(3001, d)).astype(np.float32)
+        X[50:55] = X[49]
+        Q = np.concatenate([X[:3], X[50:51], r.normal(size=(293, d)).astype(np.float32)])
+        want = oracle.knn(X, Q, 7)
+        want1 = oracle.knn(X, Q, 1)
+        index = mlk.DeviceNeighborIndex(X)
+        try:
+            for ns in (2, 3):
+                _lib.set_tuning(stages=ns)
+                assert index.plan(Q.shape[0], 7, precision=precision)["stages"] == ns
+                assert_parity(knn(index, Q, 7, precision), want)
+                assert _lib.stats()["fallback_queries"] == 0, (d, ns)
+                assert_parity(knn(index, Q, 1, precision), want1)
+        finally:
+            _lib.set_tuning()
+            index.close()
+
+
 @pytest.mark.parametrize("precision", ["float32", "bfloat16"])
 def test_autotuned_knn_plan_is_used_and_exact(mlk, precision):
     from mlxlearn_amd import _lib
