@@ -1472,7 +1472,10 @@ static size_t knn_list_bytes(const mlk_index* index, int prec, int64_t m, int k)
     const KnnLayout L = knn_layout(index, prec, m, k, hooks_snapshot(), 0);
     return L.all_fallback ? 0 : (size_t)L.m_pad * L.S2 * (size_t)L.cap * sizeof(int2);
 }
-constexpr size_t KNN_LIST_BUDGET = size_t(4) << 30;  // candidate-list bytes of one chunk of queries
+// candidate-list bytes of one chunk of queries: 10^7 k-means rows over 65,536
+// centres (15.4 GB of lists) run in one piece (four chunks of a 4 GiB budget
+// cost 4 % more screen time in launch tails), far less than HBM holds
+constexpr size_t KNN_LIST_BUDGET = size_t(16) << 30;
 constexpr int64_t KNN_CHUNK_MIN = 65536;              // smallest chunk of a split call
 
 size_t knn_fallback_bytes_per_query(int64_t n) {
@@ -2115,10 +2118,12 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
     own.ws = index->ws;
     CallMem& mem = mem_in ? *mem_in : own;
     KnnCall call;
-    if (!mem.fixed() && index->mirror[prec].ready && m > KNN_CHUNK_MIN) {
+    if (!mem.fixed() && m > KNN_CHUNK_MIN) {
         // Candidate lists beyond KNN_LIST_BUDGET bytes: the queries go in chunks
         // (each with its own plan), which bounds the workspace instead of growing
-        // it to the whole call (10^7 k-means rows would need 15 GB of lists).
+        // it to the whole call. The plan that sizes the lists needs the screen
+        // mirror (its width and unsafe rows), so a first call prepares it here.
+        MLK_TRY(prepare_mirror(index, prec, stream));
         const size_t lists = knn_list_bytes(index, prec, m, k);
         if (lists > KNN_LIST_BUDGET) {
             const int64_t parts = (int64_t)((lists + KNN_LIST_BUDGET - 1) / KNN_LIST_BUDGET);

@@ -393,6 +393,71 @@ __global__ void k_row_gather(const uint32_t* __restrict__ pos, int64_t nnz, cons
     idx_out[i] = idx[p];
 }
 
+// Rows of (lo, hi] entries, one block per row (grid: every row; other rows
+// leave at once): the row's float32 distance keys and local positions are
+// sorted in LDS by a block radix sort, then the block writes the row's
+// (distance, index) entries in key order, coalesced, with their keys (for the
+// exact repair of equal-key runs). Padding slots carry the largest key and
+// sort last.
+template <int IPT>
+__global__ __launch_bounds__(256) void k_row_sort_block(const int64_t* __restrict__ indptr, int lo, int hi,
+                                                        const double* __restrict__ dist,
+                                                        const int64_t* __restrict__ idx, double* __restrict__ dist_out,
+                                                        int64_t* __restrict__ idx_out, uint32_t* __restrict__ keys_out) {
+    using Sort = hipcub::BlockRadixSort<uint32_t, 256, IPT, uint32_t>;
+    __shared__ typename Sort::TempStorage tmp;
+    const int64_t q = blockIdx.x;
+    const int64_t b = indptr[q];
+    const int64_t L = indptr[q + 1] - b;
+    if (L <= lo || L > hi) return;  // uniform over the block
+    uint32_t key[IPT], pos[IPT];
+#pragma unroll
+    for (int i = 0; i < IPT; ++i) {
+        const int j = i * 256 + (int)threadIdx.x;
+        key[i] = j < L ? __float_as_uint(__double2float_rn(dist[b + j])) : 0xffffffffu;
+        pos[i] = (uint32_t)j;
+    }
+    Sort(tmp).SortBlockedToStriped(key, pos);
+#pragma unroll
+    for (int i = 0; i < IPT; ++i) {
+        const int r = i * 256 + (int)threadIdx.x;  // rank of this slot
+        if (r < L) {
+            const int64_t p = b + pos[i];
+            dist_out[b + r] = dist[p];
+            idx_out[b + r] = idx[p];
+            keys_out[b + r] = key[i];
+        }
+    }
+}
+
+// Rows of more than `above` entries, listed for the device-wide segmented sort.
+__global__ void k_row_list_big(const int64_t* __restrict__ indptr, int64_t m, int64_t above, int* __restrict__ rows,
+                               unsigned* __restrict__ count, int64_t* __restrict__ seg_begin,
+                               int64_t* __restrict__ seg_end) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= m) return;
+    const int64_t b = indptr[q], e = indptr[q + 1];
+    if (e - b <= above) return;
+    const unsigned at = atomicAdd(count, 1u);
+    rows[at] = (int)q;
+    seg_begin[at] = b;
+    seg_end[at] = e;
+}
+
+// Entries of the listed rows in the order of their sorted positions.
+__global__ __launch_bounds__(256) void k_row_gather_rows(const int* __restrict__ rows, const int64_t* __restrict__ indptr,
+                                                         const uint32_t* __restrict__ pos,
+                                                         const double* __restrict__ dist,
+                                                         const int64_t* __restrict__ idx, double* __restrict__ dist_out,
+                                                         int64_t* __restrict__ idx_out) {
+    const int64_t q = rows[blockIdx.x];
+    for (int64_t i = indptr[q] + threadIdx.x; i < indptr[q + 1]; i += 256) {
+        const uint32_t p = pos[i];
+        dist_out[i] = dist[p];
+        idx_out[i] = idx[p];
+    }
+}
+
 __device__ __forceinline__ bool dist_idx_less(double da, int64_t ia, double db, int64_t ib) {
     return da < db || (da == db && ia < ib);
 }
@@ -461,13 +526,18 @@ static int check_sort_count(int64_t count, const char* what) {
     return MLK_OK;
 }
 
-// Every CSR row ordered by (distance, index). A stable segmented radix sort on
-// 32-bit keys (the float32 rounding of the distance) and 32-bit positions
-// orders the rows up to runs of equal keys; the runs are then ordered exactly
-// (so the rows' input order does not matter). Temporaries live in the
-// index workspace (ws[14] sort scratch, ws[10..11] keys, ws[21..22] positions,
-// ws[23] row starts, ws[19..20] sorted distances and indices); the sorted
-// arrays are swapped into the result.
+// Every CSR row ordered by (distance, index). Rows are first ordered by 32-bit
+// keys (the float32 rounding of the distance), up to runs of equal keys, which
+// are then ordered exactly (so neither the rows' input order nor the key
+// sort's stability matters). Rows of up to ROW_SORT_BLOCK entries are sorted
+// in LDS, one block each (two block sizes); longer rows (few: a query inside a
+// dense clump) go through the device-wide segmented radix sort on (key,
+// position) and a gather. Temporaries live in the index workspace (ws[14] sort
+// scratch, ws[10..11] keys, ws[21..22] positions, ws[23] row starts, ws[19..20]
+// sorted distances and indices, ws[24] the long rows); the sorted arrays are
+// swapped into the result.
+constexpr int ROW_SORT_SMALL = 256 * 4;   // rows sorted by the 4-items-per-thread block
+constexpr int ROW_SORT_BLOCK = 256 * 32;  // longest row sorted by one block
 static int sort_rows_by_distance(mlk_radius_result* res, const int64_t* indptr, int64_t m, int64_t nnz, DevBuf* ws,
                                  hipStream_t stream) {
     if (!nnz) return MLK_OK;
@@ -476,33 +546,55 @@ static int sort_rows_by_distance(mlk_radius_result* res, const int64_t* indptr, 
     DevBuf& tmp = ws[14];
     DevBuf& dist_out = ws[19];
     DevBuf& idx_out = ws[20];
-    MLK_TRY(ws[10].reserve((size_t)nnz * 4));
     MLK_TRY(ws[11].reserve((size_t)nnz * 4));
-    MLK_TRY(ws[21].reserve((size_t)nnz * 4));
-    MLK_TRY(ws[22].reserve((size_t)nnz * 4));
     MLK_TRY(ws[23].reserve((size_t)nnz));
     MLK_TRY(dist_out.reserve((size_t)nnz * 8));
     MLK_TRY(idx_out.reserve((size_t)nnz * 8));
-    uint32_t* keys = ws[10].as<uint32_t>();
+    MLK_TRY(ws[24].reserve((size_t)m * (sizeof(int) + 2 * sizeof(int64_t)) + 64));
     uint32_t* keys_out = ws[11].as<uint32_t>();
-    uint32_t* pos = ws[21].as<uint32_t>();
-    uint32_t* pos_out = ws[22].as<uint32_t>();
     uint8_t* start = ws[23].as<uint8_t>();
-    hipLaunchKernelGGL(k_row_sort_keys, dim3(blocks_for(nnz)), dim3(256), 0, stream, res->dist.as<double>(), nnz, keys,
-                       pos);
+    const double* dist = res->dist.as<double>();
+    const int64_t* idx = res->indices.as<int64_t>();
+    double* dout = dist_out.as<double>();
+    int64_t* iout = idx_out.as<int64_t>();
     MLK_CHECK_HIP(hipMemsetAsync(start, 0, (size_t)nnz, stream));
     hipLaunchKernelGGL(k_row_starts, dim3(blocks_for(m)), dim3(256), 0, stream, indptr, m, start);
-    size_t tmp_bytes = 0;
-    MLK_CHECK_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(nullptr, tmp_bytes, keys, keys_out, pos, pos_out,
-                                                              (int)nnz, (int)m, indptr, indptr + 1, 0, 31, stream));
-    MLK_TRY(tmp.reserve(tmp_bytes + 16));
-    MLK_CHECK_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(tmp.ptr, tmp_bytes, keys, keys_out, pos, pos_out,
-                                                              (int)nnz, (int)m, indptr, indptr + 1, 0, 31, stream));
-    hipLaunchKernelGGL(k_row_gather, dim3(blocks_for(nnz)), dim3(256), 0, stream, pos_out, nnz,
-                       res->dist.as<double>(), res->indices.as<int64_t>(), dist_out.as<double>(),
-                       idx_out.as<int64_t>());
-    hipLaunchKernelGGL(k_row_fix_runs, dim3(blocks_for(nnz)), dim3(256), 0, stream, keys_out, start, nnz,
-                       dist_out.as<double>(), idx_out.as<int64_t>());
+    hipLaunchKernelGGL(k_row_sort_block<4>, dim3((unsigned)m), dim3(256), 0, stream, indptr, 0, ROW_SORT_SMALL, dist,
+                       idx, dout, iout, keys_out);
+    hipLaunchKernelGGL(k_row_sort_block<32>, dim3((unsigned)m), dim3(256), 0, stream, indptr, ROW_SORT_SMALL,
+                       ROW_SORT_BLOCK, dist, idx, dout, iout, keys_out);
+    // longer rows: listed on the device, counted on the host
+    char* lw = (char*)ws[24].ptr;
+    unsigned* nbig_d = (unsigned*)lw;
+    int64_t* seg_begin = (int64_t*)(lw + 64);
+    int64_t* seg_end = seg_begin + m;
+    int* big_rows = (int*)(seg_end + m);
+    MLK_CHECK_HIP(hipMemsetAsync(nbig_d, 0, sizeof(unsigned), stream));
+    hipLaunchKernelGGL(k_row_list_big, dim3(blocks_for(m)), dim3(256), 0, stream, indptr, m, (int64_t)ROW_SORT_BLOCK,
+                       big_rows, nbig_d, seg_begin, seg_end);
+    unsigned nbig = 0;
+    MLK_CHECK_HIP(hipMemcpyAsync(&nbig, nbig_d, sizeof(unsigned), hipMemcpyDeviceToHost, stream));
+    MLK_CHECK_HIP(hipStreamSynchronize(stream));
+    if (nbig) {
+        MLK_TRY(ws[10].reserve((size_t)nnz * 4));
+        MLK_TRY(ws[21].reserve((size_t)nnz * 4));
+        MLK_TRY(ws[22].reserve((size_t)nnz * 4));
+        uint32_t* keys = ws[10].as<uint32_t>();
+        uint32_t* pos = ws[21].as<uint32_t>();
+        uint32_t* pos_out = ws[22].as<uint32_t>();
+        hipLaunchKernelGGL(k_row_sort_keys, dim3(blocks_for(nnz)), dim3(256), 0, stream, dist, nnz, keys, pos);
+        size_t tmp_bytes = 0;
+        MLK_CHECK_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(nullptr, tmp_bytes, keys, keys_out, pos, pos_out,
+                                                                  (int)nnz, (int)nbig, seg_begin, seg_end, 0, 31,
+                                                                  stream));
+        MLK_TRY(tmp.reserve(tmp_bytes + 16));
+        MLK_CHECK_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(tmp.ptr, tmp_bytes, keys, keys_out, pos, pos_out,
+                                                                  (int)nnz, (int)nbig, seg_begin, seg_end, 0, 31,
+                                                                  stream));
+        hipLaunchKernelGGL(k_row_gather_rows, dim3(nbig), dim3(256), 0, stream, big_rows, indptr, pos_out, dist, idx,
+                           dout, iout);
+    }
+    hipLaunchKernelGGL(k_row_fix_runs, dim3(blocks_for(nnz)), dim3(256), 0, stream, keys_out, start, nnz, dout, iout);
     MLK_CHECK_HIP(hipGetLastError());
     std::swap(res->dist.ptr, dist_out.ptr);
     std::swap(res->dist.bytes, dist_out.bytes);
