@@ -241,8 +241,14 @@ __global__ __launch_bounds__(256) void k_mirror(const float* __restrict__ X, int
         if (nt_out) nt_out[row] = real ? st : 0.0;
         if (d2_out) d2_out[row] = real ? sd : 0.0;
         flags[row] = (real && safe) ? 1 : 0;
-        // one flag for the whole mirror: skip the (same-address) atomic once it is set
-        if (real && safe && !exact && __hip_atomic_load(inexact, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u)
+    }
+    // one flag for the whole mirror (null: not wanted). Every check of it is an
+    // access to one address, serialised in the L2, so one lane per wave checks
+    // for the wave's rows, and the atomic is skipped once the flag is set.
+    if (inexact) {
+        const unsigned long long any = ballot(r == 0 && real && safe && !exact);
+        if (any && lane == __builtin_ctzll(any) &&
+            __hip_atomic_load(inexact, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u)
             atomicOr(inexact, 1u);
     }
 }

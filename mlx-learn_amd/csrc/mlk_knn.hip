@@ -1609,10 +1609,11 @@ static int knn_enqueue(mlk_index* index, const float* Q, int64_t m, int k, int64
         qperm = qp;
         qcell = static_cast<const int*>(oscr);
     }
-    MLK_TRY(mirror_rows(prec, Q, m, m_pad, d, dp, index->centre.as<float>(), mr.scale, qmirror, nq, qflags, dflag,
-                        qnt, qd2, stream));
     // the queries' rounding only matters when every database value is exact
-    // in the screen type (integer data); otherwise no wait here
+    // in the screen type (integer data): only then is it recorded and read
+    // back (otherwise no flag traffic and no wait here)
+    MLK_TRY(mirror_rows(prec, Q, m, m_pad, d, dp, index->centre.as<float>(), mr.scale, qmirror, nq, qflags,
+                        mr.exact ? dflag : nullptr, qnt, qd2, stream));
     unsigned q_inexact = 1;
     if (mr.exact && !capture) {
         MLK_CHECK_HIP(hipMemcpyAsync(&q_inexact, dflag, 4, hipMemcpyDeviceToHost, stream));

@@ -691,11 +691,16 @@ int radius_query_impl(mlk_index* index, const float* Q, int64_t m, double R2, in
         Qs = ws[29].as<float>();
         qmap = ws[28].as<int>();
     }
+    // the queries' rounding is recorded (and read back) only for an exact
+    // database mirror (integer data); otherwise the queries count as inexact
     MLK_TRY(mirror_rows(prec, Qs, m, m_pad, d, dp, index->centre.as<float>(), mr.scale, ws[0].ptr, ws[1].as<double>(),
-                        ws[2].as<uint8_t>(), dflag, ws[25].as<double>(), ws[26].as<double>(), stream));
+                        ws[2].as<uint8_t>(), mr.exact ? dflag : nullptr, ws[25].as<double>(), ws[26].as<double>(),
+                        stream));
     unsigned q_inexact = 1;
-    MLK_CHECK_HIP(hipMemcpyAsync(&q_inexact, dflag, 4, hipMemcpyDeviceToHost, stream));
-    MLK_CHECK_HIP(hipStreamSynchronize(stream));
+    if (mr.exact) {
+        MLK_CHECK_HIP(hipMemcpyAsync(&q_inexact, dflag, 4, hipMemcpyDeviceToHost, stream));
+        MLK_CHECK_HIP(hipStreamSynchronize(stream));
+    }
     const Consts c = make_consts(prec, dp, d, mr.exact, q_inexact == 0, mr.scale);
     MLK_TRY(prepare_bias(index, prec, c, stream));
     const double tau = R2 * c.scale2 * (1.0 + 0x1p-50);
