@@ -104,6 +104,7 @@ __global__ void k_colsum(const float* __restrict__ X, int64_t n, int d, int64_t 
     bool frac = false;
     for (int c = threadIdx.x; c < d; c += blockDim.x) {
         double s = 0.0;
+#pragma unroll 8
         for (int64_t r = r0; r < r1; ++r) {
             float v = X[r * d + c];
             s += (double)v;
@@ -434,7 +435,9 @@ int create_index(const float* X, int64_t n, int d, hipStream_t stream, mlk_index
             break;
         }
         // Centre: float64 column means; integral data get an integer centre.
-        const int64_t rows_per_block = 4096;
+        // About 1,024 blocks of row sums (a k-means centroid index of 65,536
+        // rows is built on every assignment: 16 blocks of 4,096 rows took 1.1 ms)
+        const int64_t rows_per_block = std::max<int64_t>(64, (n + 1023) / 1024);
         const int64_t blocks = (n + rows_per_block - 1) / rows_per_block;
         DevBuf partial;
         if ((rc = partial.reserve((size_t)blocks * d * sizeof(double) + 16))) break;
