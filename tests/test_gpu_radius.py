@@ -243,3 +243,22 @@ def test_sorted_rows_with_distances_inside_one_float32_step(mlk, precision):
     R2 = 9.0
     want = oracle.radius(X, Q, R2, sort_results=True)
     assert_csr(gpu_radius(mlk, X, Q, R2, precision, sort_results=True), want, "float32-step runs")
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_sorted_rows_of_every_length_class(mlk, precision):
+    """Sorted rows of up to 1,024, up to 8,192 and more than 8,192 entries (each length class sorted by its
+    own route: the small and the large LDS block sort, and the device-wide segmented sort), with exact
+    duplicate rows (runs of equal distances inside a row) and a query equal to a row."""
+    r = np.random.default_rng(4242)
+    X = r.normal(size=(30_000, 4)).astype(np.float32)
+    X[1000:1400] = X[7]
+    X[2000:2003] = X[2003]
+    Q = np.array([[0, 0, 0, 0], [1, 1, 1, 1], X[7], [2.5, 2.5, 2.5, 2.5], [1.5, 0, 0, 0], [1.6, 1.6, 0, 0]],
+                 np.float32)
+    R2 = 4.0
+    want = oracle.radius(X, Q, R2, sort_results=True)
+    counts = np.diff(want[0])
+    assert counts.max() > 8192 and ((counts > 1024) & (counts <= 8192)).any() and (counts <= 1024).any()
+    assert_csr(gpu_radius(mlk, X, Q, R2, precision, sort_results=True), want, "length classes")
+

@@ -123,6 +123,32 @@ def test_autotuned_knn_plan_is_used_and_exact(mlk, precision):
         index.close()
 
 
+def test_autotune_rejects_plans_that_fall_back_widely(mlk):
+    """Under a one-slot list capacity (test hook) every plan's lists overflow and most queries take the
+    exact fallback: the tuner times its starting plan in full and rejects every alternative as soon as
+    its fallback count is known (without scanning for it), so the starting plan stays; calls stay exact."""
+    from mlxlearn_amd import _lib
+    _lib.clear_plans()
+    X, Q = oracle.gen_c2(seed=5, n=20_000, m=600)
+    want = oracle.knn(X, Q, 5)
+    index = mlk.DeviceNeighborIndex(X)
+    try:
+        _lib.set_test_hooks(_lib.HOOK_CAPACITY, 1)
+        start = index.plan(600, 5, precision="bfloat16")
+        plan = index.autotune(Q, 5, precision="bfloat16", max_trials=8)
+        assert plan["tuned"] and plan["trials"] >= 2
+        assert (plan["splits"], plan["query_tiles"], plan["stages"]) == \
+            (start["splits"], start["query_tiles"], start["stages"])
+        assert plan["tuned_ms"] == plan["default_ms"]
+        got = knn(index, Q, 5, "bfloat16")
+        assert _lib.stats()["fallback_queries"] > 600 // 256 + 16
+        assert_parity(got, want)
+    finally:
+        _lib.set_test_hooks(0, 0)
+        _lib.clear_plans()
+        index.close()
+
+
 def test_autotuned_radius_plan_is_exact(mlk):
     from mlxlearn_amd import _lib
     _lib.clear_plans()
