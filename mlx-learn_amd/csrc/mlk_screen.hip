@@ -842,6 +842,10 @@ static ScreenFn pick_shape(int dp, int qt, int ns) {
         return nullptr;
     } else {
         if (qt == 1 && ns == 2) return pick_dp<PREC, 1, 2, MODE>(dp);
+        // two query tiles: every LDS fragment and bias read feeds two MFMAs
+        // (radius pairs at 32 features: two MFMAs per tile and query tile)
+        if constexpr (MODE == SCREEN_PAIRS)
+            if (qt == 2 && ns == 2) return pick_dp<PREC, 2, 2, MODE>(dp);
         if constexpr (MODE == SCREEN_LISTS) {
             // four query tiles per wave: every LDS fragment feeds four MFMAs and a
             // block's 512 queries share each database tile (two waves per SIMD)
@@ -865,15 +869,17 @@ int screen_dp(int prec, int d) {
     return (d + DEEP_CHUNK - 1) / DEEP_CHUNK * DEEP_CHUNK;  // deep kernels
 }
 
-// Measured on the 1M x 128 mixture workload: for k-NN lists on bf16 / fp16
-// rows of at most 128 features, two query tiles per wave with 64-row steps
-// (2.07 ms per call) beat one (2.28 ms); float32 operands keep one (two would
-// need 128 registers of query fragments), and so do radius pairs.
+// Measured: for bf16 / fp16 rows of at most 128 features, two query tiles per
+// wave with 64-row steps: k-NN lists on the 1M x 128 mixture 2.07 ms per call
+// against 2.28 ms with one; radius pairs on the 2M x 32 clumped workload 37.8 ms
+// per call against 41.6 ms (at 32 features a tile's bias and fragment reads and
+// its barrier then serve four MFMAs instead of two). float32 operands keep one
+// query tile (two would need 128 registers of query fragments).
 ScreenShape default_shape(int mode, int prec, int dp) {
     // wide k-NN rows: two chunks per row, two blocks per CU (60k x 784 pixels, 10,000
     // queries: 1.36 ms per call against 2.10 ms with whole-row stages)
     if (prec != PREC_FP32 && dp > 128 && dp <= WIDE_MAX) return ScreenShape{1, mode == SCREEN_LISTS ? 2 : 3};
-    if (mode == SCREEN_LISTS && prec != PREC_FP32 && dp <= 128) return ScreenShape{2, 2};
+    if (prec != PREC_FP32 && dp <= 128) return ScreenShape{2, 2};
     return ScreenShape{1, 2};
 }
 

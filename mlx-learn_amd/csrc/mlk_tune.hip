@@ -113,8 +113,8 @@ Plan resolve_plan(int kind, int prec, int dp, int64_t n, int64_t m, int k, bool*
     if (t_trial) return *t_trial;
     const Tuning t = tuning_snapshot();
     ScreenShape knob{0, 0};
-    if (kind == PLAN_KNN && dp <= 128 && (t.query_tiles || t.stages)) {
-        knob = default_shape(SCREEN_LISTS, prec, dp);
+    if (dp <= 128 && (t.query_tiles || t.stages)) {
+        knob = default_shape(kind == PLAN_KNN ? SCREEN_LISTS : SCREEN_PAIRS, prec, dp);
         if (t.query_tiles) knob.qt = t.query_tiles;
         if (t.stages) knob.ns = t.stages;
     } else if (kind == PLAN_KNN && prec != PREC_FP32 && dp <= WIDE_MAX && t.stages) {

@@ -149,6 +149,33 @@ def test_autotune_rejects_plans_that_fall_back_widely(mlk):
         index.close()
 
 
+@pytest.mark.parametrize("precision", ["bfloat16", "float16"])
+def test_radius_shapes_match_oracle(mlk, precision):
+    """Radius pairs screened with one and with two query tiles per wave (each LDS fragment feeding two
+    MFMAs), sorted rows, at narrow widths, clumped rows and queries equal to rows."""
+    from mlxlearn_amd import _lib
+    r = np.random.default_rng(4600)
+    for d in (20, 32, 64, 128):
+        X = r.uniform(size=(12_000, d)).astype(np.float32)
+        X[:2000] = X[:2000] * 0.05 + 0.5
+        X[3000:3004] = X[17]
+        Q = np.concatenate([X[:300], r.uniform(size=(213, d)).astype(np.float32)])
+        R2 = 0.9 * d / 12  # about the clump centre's distance from a uniform query: rows of 0 to ~5,000
+        want = oracle.radius(X, Q, R2, sort_results=True)
+        index = mlk.DeviceNeighborIndex(X)
+        try:
+            for qt in (1, 2):
+                _lib.set_tuning(query_tiles=qt, stages=2)
+                assert index.plan(Q.shape[0], radius=True, precision=precision)["query_tiles"] == qt
+                indptr, indices, dist = index.radius(Q, R2, precision=precision, sort_results=True)
+                np.testing.assert_array_equal(indptr.cpu().numpy(), want[0])
+                np.testing.assert_array_equal(indices.cpu().numpy(), want[1])
+                np.testing.assert_allclose(dist.cpu().numpy(), want[2], rtol=1e-12, atol=0)
+        finally:
+            _lib.set_tuning()
+            index.close()
+
+
 def test_autotuned_radius_plan_is_exact(mlk):
     from mlxlearn_amd import _lib
     _lib.clear_plans()
