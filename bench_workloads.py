@@ -51,10 +51,16 @@ def cold(fn):
     return round((time.perf_counter() - t0) * 1e3, 3)
 
 
-def timed(fn, steps, warmup):
+def timed(fn, steps, warmup, sync_each=True):
     """Mean seconds per step over `steps` timed steps (bracketed by device
     synchronisation), the last output, and the spread of the per-step times
-    (median and interquartile range)."""
+    (median and interquartile range).
+
+    sync_each=False: steps whose call returns only once its results are
+    complete (a k-NN query returns after reading its status back) are not
+    followed by a device-wide synchronisation each; it would only add a host
+    round trip per step. The timed region is synchronised on both sides either
+    way."""
     import torch
     for _ in range(warmup):
         fn()
@@ -65,7 +71,8 @@ def timed(fn, steps, warmup):
     for _ in range(steps):
         ts = time.perf_counter()
         out = fn()
-        torch.cuda.synchronize()
+        if sync_each:
+            torch.cuda.synchronize()
         per.append(time.perf_counter() - ts)
     total = time.perf_counter() - t0
     q1, med, q3 = np.percentile(np.array(per) * 1e3, [25, 50, 75])

@@ -38,7 +38,9 @@ struct Status {
 // owner goes out of scope, on every return path.
 // Events come from a per-device pool (creating and destroying a dozen events
 // per call costs more than a small call's kernels); released events are
-// re-recorded by later calls.
+// re-recorded by later calls. They are timing events: waiting on one orders
+// the host after the work before it, but device memory is read back through
+// stream-ordered copies, never through an event.
 hipError_t event_acquire(int device, hipEvent_t* out);
 void event_release(int device, hipEvent_t e);
 

@@ -29,6 +29,7 @@
 // far), so the kept set contains every true neighbour; the rank stage orders
 // the kept set exactly. Overflow of a candidate list withdraws the
 // certificate and sends the query to the exact fallback.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -1512,6 +1513,9 @@ struct KnnCall {
     RankArgs ra{};
     Consts c{};
     int64_t rounds = 0, splits = 0, capacity = 0;
+    // timed calls: the start of the chain's first kernel and the end of its
+    // status kernel, bound to those launches (no marker packets in the stream)
+    hipEvent_t ev_first = nullptr, ev_status = nullptr;
 };
 
 // Enqueue a call's device chain on `stream`, up to the per-query status
@@ -1530,10 +1534,9 @@ static int knn_enqueue(mlk_index* index, const float* Q, int64_t m, int k, int64
     const int prec = effective_prec((int)((flags & MLK_PREC_MASK) >> 4), d);
     const bool exclude = (flags & MLK_EXCLUDE_SELF) != 0;
     st.screen_precision = (uint32_t)prec << 4;  // the precision actually screened
-    hipEvent_t ev1 = nullptr, ev2 = nullptr;
     if (timed) {
-        MLK_CHECK_HIP(events->make(&ev1));
-        MLK_CHECK_HIP(events->make(&ev2));
+        MLK_CHECK_HIP(events->make(&call.ev_first));
+        MLK_CHECK_HIP(events->make(&call.ev_status));
     }
 
     if (fixed && !index->mirror[prec].ready) {
@@ -1580,7 +1583,11 @@ static int knn_enqueue(mlk_index* index, const float* Q, int64_t m, int k, int64
     MLK_TRY(mem.get(6, 64, &dflag));
     // flag words: [0] query rounding inexact, [2] fallback count, [4..10) three
     // 64-bit statistics sums (cleared by a kernel: graphs replay no memset nodes)
-    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, stream, (int*)dflag, (int64_t)10, 0);
+    if (timed)
+        hipExtLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, stream, call.ev_first, nullptr, 0, (int*)dflag,
+                              (int64_t)10, 0);
+    else
+        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, stream, (int*)dflag, (int64_t)10, 0);
     // Query locality order (calls without self-exclusion that may allocate):
     // the chain below runs on the regrouped queries, the start threshold of
     // each block of them samples rows of their own neighbourhood, and the
@@ -1705,7 +1712,6 @@ static int knn_enqueue(mlk_index* index, const float* Q, int64_t m, int k, int64
         MLK_CHECK_HIP(hipGetLastError());
     }
 
-    if (timed) MLK_CHECK_HIP(hipEventRecord(ev1, stream));
     const int* rank_stats = nullptr;  // per-query counts written by the rank (screened calls)
     bool big_rank = false;
     RankArgs ra{};
@@ -1755,17 +1761,17 @@ static int knn_enqueue(mlk_index* index, const float* Q, int64_t m, int k, int64
                     round_events->push_back(ea);
                     round_events->push_back(eb);
                 }
-                MLK_CHECK_HIP(hipEventRecord(ea, stream));
+                // the launch's own start and end times
+                hipExtLaunchKernelGGL(screen, dim3(blocks), dim3(256), 0, stream, ea, eb, 0, sa);
+            } else {
+                hipLaunchKernelGGL(screen, dim3(blocks), dim3(256), 0, stream, sa);
             }
-            hipLaunchKernelGGL(screen, dim3(blocks), dim3(256), 0, stream, sa);
             MLK_CHECK_HIP(hipGetLastError());
-            if (timed) MLK_CHECK_HIP(hipEventRecord(eb, stream));
             ++st.rounds;
             if (r + 1 < plan.rounds.size()) MLK_TRY(merge());
         }
         // final overflow check (the last round has no merge)
         MLK_TRY(merge());
-        if (timed) MLK_CHECK_HIP(hipEventRecord(ev2, stream));
 
         ra.X = X; ra.n = n; ra.d = d; ra.Q = Q; ra.m = m; ra.k = k;
         ra.cnt = sa.cnt; ra.cand = sa.cand; ra.nx32 = sa.nx32; ra.nq = nq;
@@ -1795,7 +1801,6 @@ static int knn_enqueue(mlk_index* index, const float* Q, int64_t m, int k, int64
         MLK_CHECK_HIP(hipGetLastError());
         rank_stats = ra.qstats;
     } else {
-        if (timed) MLK_CHECK_HIP(hipEventRecord(ev2, stream));
         hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, stream, status, m,
                            ST_FALLBACK);
         MLK_CHECK_HIP(hipGetLastError());
@@ -1804,8 +1809,13 @@ static int knn_enqueue(mlk_index* index, const float* Q, int64_t m, int k, int64
     // queries without a certificate -> exact fallback
     int* fq;
     MLK_TRY(mem.get(11, (size_t)m * sizeof(int), &fq));
-    hipLaunchKernelGGL(k_finish_status, dim3((unsigned)std::min<int64_t>(1024, (m + 255) / 256)), dim3(256), 0,
-                       stream, m, status, fq, dflag + 2, rank_stats, (unsigned long long*)(dflag + 4));
+    const dim3 gs((unsigned)std::min<int64_t>(1024, (m + 255) / 256));
+    if (timed)
+        hipExtLaunchKernelGGL(k_finish_status, gs, dim3(256), 0, stream, nullptr, call.ev_status, 0, m, status, fq,
+                              dflag + 2, rank_stats, (unsigned long long*)(dflag + 4));
+    else
+        hipLaunchKernelGGL(k_finish_status, gs, dim3(256), 0, stream, m, status, fq, dflag + 2, rank_stats,
+                           (unsigned long long*)(dflag + 4));
     MLK_CHECK_HIP(hipGetLastError());
     call.m = m; call.n = n; call.q_offset = q_offset; call.d = d; call.k = k; call.prec = prec;
     call.flags = flags; call.exclude = exclude; call.Q = Q; call.X = X; call.dist = dist; call.idx = idx;
@@ -1854,14 +1864,22 @@ static int knn_complete(mlk_index* index, const KnnCall& call, hipStream_t strea
     // regrouped queries: results to the caller's rows, enqueued before the
     // read-back so that the common call (no fallback) ends with it; a fallback
     // rewrites some rows by position and moves them again below
-    auto unpermute = [&]() -> int {
+    // the end of the chain before the read-back (timed calls)
+    hipEvent_t ev_end = call.ev_status;
+    auto unpermute = [&](hipEvent_t stop) -> int {
         if (!call.qperm) return MLK_OK;
-        hipLaunchKernelGGL(k_unpermute_results, dim3((unsigned)((m * k + 255) / 256)), dim3(256), 0, stream,
-                           call.qperm, m, k, idx, dist, call.user_idx, call.user_dist);
+        const dim3 g((unsigned)((m * k + 255) / 256));
+        if (stop)
+            hipExtLaunchKernelGGL(k_unpermute_results, g, dim3(256), 0, stream, nullptr, stop, 0, call.qperm, m, k,
+                                  idx, dist, call.user_idx, call.user_dist);
+        else
+            hipLaunchKernelGGL(k_unpermute_results, g, dim3(256), 0, stream, call.qperm, m, k, idx, dist,
+                               call.user_idx, call.user_dist);
         MLK_CHECK_HIP(hipGetLastError());
         return MLK_OK;
     };
-    MLK_TRY(unpermute());
+    if (events && call.qperm) MLK_CHECK_HIP(events->make(&ev_end));
+    MLK_TRY(unpermute(events ? ev_end : nullptr));
     // one read-back: the fallback count and the statistics sums
     struct {
         unsigned nf, pad;
@@ -1957,23 +1975,26 @@ static int knn_complete(mlk_index* index, const KnnCall& call, hipStream_t strea
             MLK_CHECK_HIP(hipGetLastError());
         }
     }
-    if (rewritten) MLK_TRY(unpermute());
+    if (rewritten) MLK_TRY(unpermute(nullptr));
     if (!events) {
         // no events: the outputs are ordered on the stream; the fallback's host
         // list must outlive its copy
         if (nf) MLK_CHECK_HIP(hipStreamSynchronize(stream));
         return MLK_OK;
     }
-    hipEvent_t ev3;
-    MLK_CHECK_HIP(events->make(&ev3));
-    MLK_CHECK_HIP(hipEventRecord(ev3, stream));
-    MLK_CHECK_HIP(hipEventSynchronize(ev3));
+    if (nf) {
+        // the fallback's kernels end the call: one more event and wait
+        MLK_CHECK_HIP(events->make(&ev_end));
+        MLK_CHECK_HIP(hipEventRecord(ev_end, stream));
+        MLK_CHECK_HIP(hipEventSynchronize(ev_end));
+    }  // else every event was reached before the read-back's wait ended
+    if (!ev0) ev0 = call.ev_first;
     float ms = 0.0f;
     // screen kernels only: sum of the per-round launch intervals
     st.screen_ms = 0.0;
     for (size_t i = 0; i + 1 < round_events.size(); i += 2)
         if (hipEventElapsedTime(&ms, round_events[i], round_events[i + 1]) == hipSuccess) st.screen_ms += ms;
-    if (hipEventElapsedTime(&ms, ev0, ev3) == hipSuccess) st.total_ms = ms;
+    if (hipEventElapsedTime(&ms, ev0, ev_end) == hipSuccess) st.total_ms = ms;
     return MLK_OK;
 }
 
@@ -2155,12 +2176,9 @@ int knn_query_impl(mlk_index* index, const float* Q, int64_t m, int k, int64_t q
         return knn_complete(index, call, stream, mem, nullptr, {}, nullptr);
     }
     Events events;  // every event of this call
-    hipEvent_t ev0;
-    MLK_CHECK_HIP(events.make(&ev0));
-    MLK_CHECK_HIP(hipEventRecord(ev0, stream));
     std::vector<hipEvent_t> round_events;
     MLK_TRY(knn_enqueue(index, Q, m, k, q_offset, flags, dist, idx, stream, mem, false, call, &events, &round_events));
-    return knn_complete(index, call, stream, mem, &events, round_events, ev0);
+    return knn_complete(index, call, stream, mem, &events, round_events, nullptr);
 }
 
 }  // namespace mlk

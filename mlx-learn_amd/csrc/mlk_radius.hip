@@ -13,6 +13,7 @@
 // sqrt(min(s, R2)). This replaces the reference's per-tile device screen with
 // host finishing (radius_metal_v1.py:257-611, _radius_cpu_native.cpp:129-200)
 // by a device-only pipeline.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
@@ -722,7 +723,6 @@ int radius_query_impl(mlk_index* index, const float* Q, int64_t m, double R2, in
         MLK_TRY(ws[9].reserve((size_t)cap * sizeof(int2)));
         cap = ws[9].bytes / sizeof(int2);
         MLK_CHECK_HIP(hipMemsetAsync(dcount, 0, sizeof(unsigned long long), stream));
-        MLK_CHECK_HIP(hipEventRecord(e0, stream));
         if (screen) {
             ScreenArgs sa{};
             sa.xs = mr.xs.ptr;
@@ -744,13 +744,13 @@ int radius_query_impl(mlk_index* index, const float* Q, int64_t m, double R2, in
             sa.qmap = qmap;
             sa.pair_count = dcount;
             sa.pair_cap = cap;
-            hipLaunchKernelGGL(screen, dim3((unsigned)screen_grid(sa)), dim3(256), 0, stream, sa);
+            // the events take the launch's own start and end times
+            hipExtLaunchKernelGGL(screen, dim3((unsigned)screen_grid(sa)), dim3(256), 0, stream, e0, e1, 0, sa);
             MLK_CHECK_HIP(hipGetLastError());
         } else {
             set_error("no radius screen kernel for this row width");
             return MLK_EINVAL;
         }
-        MLK_CHECK_HIP(hipEventRecord(e1, stream));
         if (mr.n_unsafe) {
             hipLaunchKernelGGL(k_unsafe_pairs, dim3(blocks_for(m * mr.n_unsafe)), dim3(256), 0, stream, m,
                                mr.unsafe.as<int>(), (int)mr.n_unsafe, q_offset, (int)exclude, qmap, ws[9].as<int2>(),
