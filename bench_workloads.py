@@ -136,7 +136,7 @@ def run_c1(args):
         first = cold(knn_step(index, Qd, 5, prec))
         if args.autotune:
             tuned[prec] = tune(index, Qd, 5, precision=prec, max_trials=16)
-        t, (d, i, st), spread[prec] = timed(knn_step(index, Qd, 5, prec), args.steps, args.warmup)
+        t, (d, i, st), spread[prec] = timed(knn_step(index, Qd, 5, prec), args.steps, args.warmup, sync_each=False)
         spread[prec]["first_call_ms"] = first
         res[prec] = (t, d.cpu().numpy(), i.cpu().numpy(), st)
         log(f"c1 {prec}: {t * 1e3:.3f} ms/step, stats {st}")
@@ -144,7 +144,8 @@ def run_c1(args):
     st_eager = res["float32"][3]
     # serving-loop form: the fp32 call captured once in a HIP graph, then replayed
     gstep = knn_graph_step(index, Qd, 5, "float32")
-    tg, (dg, ig, _), spread["float32_graph"] = timed(gstep, max(args.steps, 20), max(args.warmup, 2))
+    tg, (dg, ig, _), spread["float32_graph"] = timed(gstep, max(args.steps, 20), max(args.warmup, 2),
+                                                     sync_each=False)
     log(f"c1 float32 graph replay: {tg * 1e3:.3f} ms/step")
     wd, wi = oracle.knn(X, Q, 5)
     t, d, i, st = res["float32"]
@@ -433,7 +434,8 @@ def run_deep(args):
         first = cold(knn_step(index, Qd, k, prec))
         if args.autotune:
             tuned[prec] = tune(index, Qd, k, precision=prec, max_trials=16)
-        t, (dd, ii, st), spread[prec] = timed(knn_step(index, Qd, k, prec), args.steps, args.warmup)
+        t, (dd, ii, st), spread[prec] = timed(knn_step(index, Qd, k, prec), args.steps, args.warmup,
+                                              sync_each=False)
         spread[prec]["first_call_ms"] = first
         res[prec] = (t, dd.cpu().numpy(), ii.cpu().numpy(), st)
         log(f"deep {prec}: {t * 1e3:.3f} ms/step, stats {st}")
