@@ -1534,10 +1534,7 @@ static int knn_enqueue(mlk_index* index, const float* Q, int64_t m, int k, int64
     const int prec = effective_prec((int)((flags & MLK_PREC_MASK) >> 4), d);
     const bool exclude = (flags & MLK_EXCLUDE_SELF) != 0;
     st.screen_precision = (uint32_t)prec << 4;  // the precision actually screened
-    if (timed) {
-        MLK_CHECK_HIP(events->make(&call.ev_first));
-        MLK_CHECK_HIP(events->make(&call.ev_status));
-    }
+    if (timed) MLK_CHECK_HIP(events->make(&call.ev_first));
 
     if (fixed && !index->mirror[prec].ready) {
         set_error("the index has no screen mirror for this precision yet: one mlk_knn_query call of the same "
@@ -1810,12 +1807,16 @@ static int knn_enqueue(mlk_index* index, const float* Q, int64_t m, int k, int64
     int* fq;
     MLK_TRY(mem.get(11, (size_t)m * sizeof(int), &fq));
     const dim3 gs((unsigned)std::min<int64_t>(1024, (m + 255) / 256));
-    if (timed)
+    // the end of the timed chain, unless regrouped results are moved after it
+    // (then the move's launch carries the end event instead)
+    if (timed && !qperm) {
+        MLK_CHECK_HIP(events->make(&call.ev_status));
         hipExtLaunchKernelGGL(k_finish_status, gs, dim3(256), 0, stream, nullptr, call.ev_status, 0, m, status, fq,
                               dflag + 2, rank_stats, (unsigned long long*)(dflag + 4));
-    else
+    } else {
         hipLaunchKernelGGL(k_finish_status, gs, dim3(256), 0, stream, m, status, fq, dflag + 2, rank_stats,
                            (unsigned long long*)(dflag + 4));
+    }
     MLK_CHECK_HIP(hipGetLastError());
     call.m = m; call.n = n; call.q_offset = q_offset; call.d = d; call.k = k; call.prec = prec;
     call.flags = flags; call.exclude = exclude; call.Q = Q; call.X = X; call.dist = dist; call.idx = idx;
